@@ -1,0 +1,165 @@
+/*
+ * modlin -- MI355X (gfx950) implementation of the hot paths of modlin-rs:
+ * rank over F_p and Berlekamp-Massey linear complexity, with the reference's
+ * Field contract and p-values.  C ABI: plain pointers, sizes and status codes;
+ * no exceptions or aborts cross it.
+ *
+ * Reference interfaces replaced (paths relative to the modlin-rs tree):
+ *   modlin_field_init         Field::new                     src/fp.rs:49-61
+ *   modlin_reduce             Field::reduce                  src/fp.rs:63-67
+ *   modlin_reduce_u128        Field::reduce_u128             src/fp.rs:69-79
+ *   modlin_rank               fp::rank                       src/fp.rs:116-255
+ *   modlin_linear_complexity  fp::linear_complexity          src/fp.rs:257-308
+ *   modlin_rank_batch         main.rs's loop of fp::rank over reps (main.rs:82-127),
+ *                             one launch sequence for all matrices
+ *   modlin_lc_batch           main.rs's loop of linear_complexity (main.rs:152-175)
+ *   modlin_corank_prob        stats::corank_prob             src/stats.rs:52-62
+ *   modlin_corank_tail_pvalue stats::corank_tail_pvalue      src/stats.rs:69-84
+ *   modlin_lc_left_tail_pvalue stats::lc_left_tail_pvalue    src/stats.rs:101-126
+ *   modlin_pretty_p_value     stats::pretty_p_value          src/stats.rs:129-137
+ *   modlin_prng_fill          Prng::try_new + next_u64       src/prng.rs:46-343
+ *
+ * Ownership: the caller owns every buffer.  modlin_rank leaves m unspecified
+ * (the reference also destroys it, holding partial factors).  A context owns
+ * device memory, the stream and cached plans; use one context per thread.
+ */
+#ifndef MODLIN_H
+#define MODLIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define MODLIN_OK 0
+#define MODLIN_EINVAL 1      /* bad argument: p outside [2, 2^63), n = 0 for BM, overflow */
+#define MODLIN_ENOMEM 2      /* host or device allocation failed */
+#define MODLIN_EHIP 3        /* HIP runtime error; see modlin_ctx_last_error */
+#define MODLIN_ENODEV 4      /* no usable gfx950 device */
+#define MODLIN_EUNSUPPORTED 5
+
+/* Field (src/fp.rs:34-47): p, Barrett reciprocal mu = floor((2^128-1)/p) split
+ * in two words, and safe_batch = number of (p-1)^2 products that fit a u128. */
+typedef struct modlin_field {
+    uint64_t p;
+    uint64_t mu_hi, mu_lo;
+    uint64_t safe_batch;
+} modlin_field_t;
+
+typedef struct modlin_ctx modlin_ctx_t;
+
+/* Progress callback: done / total units (columns for rank, steps for BM,
+ * items for batches).  NULL disables it. */
+typedef void (*modlin_progress_fn)(void *user, uint64_t done, uint64_t total);
+
+enum modlin_method {
+    MODLIN_METHOD_AUTO = 0,  /* MFMA limb update where it pays, else plain */
+    MODLIN_METHOD_PLAIN = 1, /* VALU Barrett trailing update */
+    MODLIN_METHOD_MFMA = 2   /* signed-digit int8 limb update on MFMA */
+};
+enum modlin_elem { MODLIN_ELEM_U64 = 0, MODLIN_ELEM_U32 = 1 };
+enum modlin_mem { MODLIN_MEM_HOST = 0, MODLIN_MEM_DEVICE = 1 };
+
+/* One matrix of a batch: n x n row-major, entries in [0, p). */
+typedef struct modlin_mat_desc {
+    const void *data;
+    uint64_t n;
+    uint32_t elem; /* modlin_elem */
+    uint32_t mem;  /* modlin_mem */
+} modlin_mat_desc_t;
+
+/* One sequence of a batch, elements in [0, p). */
+typedef struct modlin_seq_desc {
+    const void *data;
+    uint64_t n;
+    uint32_t elem;
+    uint32_t mem;
+} modlin_seq_desc_t;
+
+#define MODLIN_FLAG_PROFILE 1u /* time the dominant kernels with HIP events */
+
+typedef struct modlin_opts {
+    uint32_t method; /* modlin_method */
+    uint32_t panel;  /* panel width (0 = tuned default, else 1..64) */
+    uint32_t flags;  /* MODLIN_FLAG_* */
+    uint32_t reserved;
+} modlin_opts_t;
+
+/* Counters of the last rank / BM call on a context. */
+typedef struct modlin_stats {
+    double macs;            /* F_p multiply-accumulates in trailing updates */
+    double update_ms;       /* summed duration of trailing-update kernels (PROFILE) */
+    double update_launches; /* number of trailing-update launches (PROFILE) */
+    double limb_count;      /* int8 limbs per element of the MFMA update (0 = VALU) */
+    double iterations;      /* panel iterations */
+    double bm_bytes;        /* algorithmic BM bytes */
+    double bm_ms;           /* BM kernel time (PROFILE) */
+    double total_ms;        /* wall time of the device part of the call */
+} modlin_stats_t;
+
+/* --- Field (src/fp.rs:49-79) -------------------------------------------- */
+int modlin_field_init(uint64_t p, modlin_field_t *out);
+uint64_t modlin_reduce(const modlin_field_t *f, uint64_t x);
+uint64_t modlin_reduce_u128(const modlin_field_t *f, uint64_t hi, uint64_t lo);
+
+/* --- Context --------------------------------------------------------------- */
+int modlin_ctx_create(int device, modlin_ctx_t **out);
+void modlin_ctx_destroy(modlin_ctx_t *ctx);
+const char *modlin_ctx_last_error(const modlin_ctx_t *ctx);
+int modlin_ctx_last_stats(const modlin_ctx_t *ctx, modlin_stats_t *out);
+
+/* --- Hot paths ------------------------------------------------------------- */
+int modlin_rank(modlin_ctx_t *ctx, const modlin_field_t *f, uint64_t *m, uint64_t n, modlin_progress_fn progress,
+                void *user, uint64_t *rank);
+int modlin_linear_complexity(modlin_ctx_t *ctx, const modlin_field_t *f, const uint64_t *s, uint64_t n,
+                             modlin_progress_fn progress, void *user, uint64_t *lc);
+int modlin_rank_batch(modlin_ctx_t *ctx, const modlin_field_t *f, const modlin_mat_desc_t *mats, uint64_t count,
+                      const modlin_opts_t *opts, uint64_t *ranks);
+int modlin_lc_batch(modlin_ctx_t *ctx, const modlin_field_t *f, const modlin_seq_desc_t *seqs, uint64_t count,
+                    const modlin_opts_t *opts, uint64_t *lcs);
+
+/* --- Statistics (src/stats.rs) ------------------------------------------- */
+double modlin_corank_prob(uint64_t p, uint64_t n, uint64_t d);
+double modlin_corank_tail_pvalue(uint64_t p, uint64_t n, uint64_t c);
+double modlin_lc_left_tail_pvalue(uint64_t p, uint64_t n, uint64_t ell);
+/* Writes "0", "1" or Rust's {:?} of x; returns the length, or -1 if cap is too small. */
+int modlin_pretty_p_value(double x, char *buf, size_t cap);
+
+/* --- Generators (src/prng.rs) ---------------------------------------------- */
+enum modlin_prng {
+    MODLIN_PRNG_XOROSHIRO128PP = 0, /* prng.rs:46-84 */
+    MODLIN_PRNG_MIXMAX17 = 1,       /* prng.rs:303-322 */
+    MODLIN_PRNG_MIXMAX240 = 2,      /* prng.rs:282-301 */
+    MODLIN_PRNG_MIXMAX256 = 3,      /* prng.rs:324-343 */
+    MODLIN_PRNG_SPLITMIX64 = 4,     /* published SplitMix64 (workload generation) */
+    MODLIN_PRNG_XOSHIRO256SS = 5    /* published xoshiro256**, SplitMix64-seeded */
+};
+/* Fills out[0..count) with next_u64() after discarding `skip` outputs.
+ * Returns MODLIN_EINVAL for a zero MIXMAX seed (prng.rs:153-155). */
+int modlin_prng_fill(uint32_t kind, uint64_t seed, uint64_t skip, uint64_t *out, uint64_t count);
+const char *modlin_prng_name(uint32_t kind);
+
+/* --- Workload helpers (host; the fill side of main.rs:86-105) ---------------- */
+/* out[i][c] = sum_k coef[i][k] * rows[k][c] mod p, for i < nout, k < nsrc, c < n
+ * (row-major; rows and out may not overlap).  Builds planted-deficiency matrices. */
+int modlin_util_combine_rows(uint64_t p, const uint64_t *rows, uint64_t nsrc, uint64_t n, const uint64_t *coef,
+                             uint64_t nout, uint64_t *out);
+/* s[i] = sum_{j<L} coef[j] * s[i-1-j] mod p for i >= L, with s[0..L) = init:
+ * a sequence of linear complexity at most L. */
+int modlin_util_lfsr(uint64_t p, const uint64_t *coef, uint64_t L, const uint64_t *init, uint64_t n, uint64_t *out);
+
+/* --- Device buffers for zero-copy callers ---------------------------------- */
+int modlin_dev_alloc(modlin_ctx_t *ctx, uint64_t bytes, void **out);
+int modlin_dev_free(modlin_ctx_t *ctx, void *ptr);
+int modlin_memcpy_h2d(modlin_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+int modlin_memcpy_d2h(modlin_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+int modlin_synchronize(modlin_ctx_t *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MODLIN_H */

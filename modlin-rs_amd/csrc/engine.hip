@@ -1,0 +1,277 @@
+// Host orchestration of the rank engine: workspace layout in HBM, the panel
+// iteration loop and the context.  The reference's driver loop is
+// src/fp.rs:138-252 (one matrix); here every matrix of a batch advances one
+// panel per iteration, and each iteration is five launches on one stream.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "engine.hpp"
+#include "rank_common.hpp"
+
+namespace mdl {
+
+// rank_kernels.hip
+void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
+                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
+                           uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1);
+
+FieldParams make_params(uint64_t p) {
+    typedef unsigned __int128 u128;
+    FieldParams f{};
+    f.p = p;
+    u128 mu = (~(u128)0) / p;
+    f.mu_hi = (uint64_t)(mu >> 64);
+    f.mu_lo = (uint64_t)mu;
+    uint32_t k = 64 - __builtin_clzll(p);
+    f.k = k;
+    f.big = (k == 63) ? 1u : 0u;
+    u128 muk = (((u128)1) << (2 * k)) / p;
+    u128 mup = muk << (63 - k);
+    f.mup = (mup >> 64) ? ~(uint64_t)0 : (uint64_t)mup;
+    f.r64 = (uint64_t)((((u128)1) << 64) % p);
+    u128 maxp = (u128)(p - 1) * (u128)(p - 1);
+    u128 sb = (~(u128)0) / (maxp ? maxp : 1);
+    if (sb > (u128)UINT64_MAX) sb = UINT64_MAX;
+    if (sb < 1) sb = 1;
+    f.safe_batch = (uint64_t)sb;
+    return f;
+}
+
+int hip_fail(modlin_ctx *ctx, hipError_t e, const char *what) {
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return MODLIN_EHIP;
+}
+
+int arena_reserve(modlin_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->arena_bytes) return MODLIN_OK;
+    if (ctx->arena) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(ctx->arena);
+        ctx->arena = nullptr;
+        ctx->arena_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&ctx->arena, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->err = "device allocation of " + std::to_string(bytes) + " bytes failed";
+        return MODLIN_ENOMEM;
+    }
+    ctx->arena_bytes = bytes;
+    return MODLIN_OK;
+}
+
+static constexpr size_t PINNED_CHUNK = 64ull << 20;
+
+static int ensure_pinned(modlin_ctx *ctx) {
+    if (ctx->pinned[0]) return MODLIN_OK;
+    for (int i = 0; i < 2; i++) {
+        if (hipHostMalloc(&ctx->pinned[i], PINNED_CHUNK, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->pinned_ev[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->err = "pinned staging allocation failed";
+            return MODLIN_ENOMEM;
+        }
+    }
+    ctx->pinned_bytes = PINNED_CHUNK;
+    return MODLIN_OK;
+}
+
+void release_pinned(modlin_ctx *ctx) {
+    for (int i = 0; i < 2; i++) {
+        if (ctx->pinned_ev[i]) hipEventDestroy(ctx->pinned_ev[i]);
+        if (ctx->pinned[i]) hipHostFree(ctx->pinned[i]);
+        ctx->pinned_ev[i] = nullptr;
+        ctx->pinned[i] = nullptr;
+    }
+    ctx->pinned_bytes = 0;
+}
+
+int copy_h2d(modlin_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = ensure_pinned(ctx);
+    if (rc) return rc;
+    size_t off = 0;
+    int buf = 0;
+    bool used[2] = {false, false};
+    while (off < bytes) {
+        const size_t chunk = std::min(bytes - off, ctx->pinned_bytes);
+        if (used[buf]) MDL_HIP(hipEventSynchronize(ctx->pinned_ev[buf]));  // staging slot free again
+        memcpy(ctx->pinned[buf], (const char *)src + off, chunk);
+        MDL_HIP(hipMemcpyAsync((char *)dst + off, ctx->pinned[buf], chunk, hipMemcpyHostToDevice, ctx->stream));
+        MDL_HIP(hipEventRecord(ctx->pinned_ev[buf], ctx->stream));
+        used[buf] = true;
+        off += chunk;
+        buf ^= 1;
+    }
+    // both slots must be free before the caller's next staging use
+    for (int i = 0; i < 2; i++)
+        if (used[i]) MDL_HIP(hipEventSynchronize(ctx->pinned_ev[i]));
+    return MODLIN_OK;
+}
+
+int copy_d2h(modlin_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = ensure_pinned(ctx);
+    if (rc) return rc;
+    size_t off = 0;
+    while (off < bytes) {
+        const size_t chunk = std::min(bytes - off, ctx->pinned_bytes);
+        MDL_HIP(hipMemcpyAsync(ctx->pinned[0], (const char *)src + off, chunk, hipMemcpyDeviceToHost, ctx->stream));
+        MDL_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy((char *)dst + off, ctx->pinned[0], chunk);
+        off += chunk;
+    }
+    return MODLIN_OK;
+}
+
+__global__ void k_widen_u32(const uint32_t *src, uint64_t *dst, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_iota(uint32_t *perm, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) perm[i] = i;
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint64_t count, const modlin_opts_t *opts,
+               uint64_t *ranks, modlin_progress_fn progress, void *user) {
+    const auto t_start = std::chrono::steady_clock::now();
+    ctx->stats = modlin_stats_t{};
+    if (count == 0) return MODLIN_OK;
+    const FieldParams f = make_params(p);
+    uint32_t b = (opts && opts->panel) ? opts->panel : (uint32_t)BMAX;
+    if (b > (uint32_t)BMAX) b = BMAX;
+    const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
+
+    // Workspace layout (256-byte aligned regions).
+    std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offS(count);
+    size_t cur = 0;
+    const size_t A = 256;
+    cur = align_up(cur + count * sizeof(RankMat), A);
+    const size_t off_gx = cur;
+    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+    const size_t off_upd = cur;
+    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+    const size_t off_plan = cur;
+    cur = align_up(cur + sizeof(RankPlan), A);
+    const size_t off_active = cur;
+    cur = align_up(cur + sizeof(uint32_t), A);
+    for (uint64_t i = 0; i < count; i++) {
+        const uint64_t n = mats[i].n;
+        if (n > 0xffffffffull / 2 || (n && n > (((uint64_t)1) << 62) / n / 8)) {
+            ctx->err = "matrix side too large";
+            return MODLIN_EINVAL;
+        }
+        if (mats[i].elem != MODLIN_ELEM_U64 && mats[i].elem != MODLIN_ELEM_U32) {
+            ctx->err = "unknown element type";
+            return MODLIN_EINVAL;
+        }
+        if (n && !mats[i].data) {
+            ctx->err = "null matrix pointer";
+            return MODLIN_EINVAL;
+        }
+        offM[i] = cur; cur = align_up(cur + n * n * 8, A);
+        offP[i] = cur; cur = align_up(cur + n * 4, A);
+        offA[i] = cur; cur = align_up(cur + n * BMAX * 8, A);
+        offX[i] = cur; cur = align_up(cur + BMAX * n * 8, A);
+        offZ[i] = cur; cur = align_up(cur + BMAX * BMAX * 8, A);
+        offS[i] = 0;
+        if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
+    }
+    int rc = arena_reserve(ctx, cur);
+    if (rc) return rc;
+    char *base = (char *)ctx->arena;
+    hipStream_t st = ctx->stream;
+
+    std::vector<RankMat> hm(count);
+    uint32_t active = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        const uint64_t n = mats[i].n;
+        RankMat &S = hm[i];
+        memset(&S, 0, sizeof(S));
+        S.M = (uint64_t *)(base + offM[i]);
+        S.perm = (uint32_t *)(base + offP[i]);
+        S.Ab = (uint64_t *)(base + offA[i]);
+        S.X = (uint64_t *)(base + offX[i]);
+        S.Z = (uint64_t *)(base + offZ[i]);
+        S.n = (uint32_t)n;
+        S.ld = (uint32_t)n;
+        S.done = (n == 0) ? 1u : 0u;
+        if (n == 0) continue;
+        active++;
+        const size_t elems = n * n;
+        const bool dev = mats[i].mem == MODLIN_MEM_DEVICE;
+        const size_t esz = mats[i].elem == MODLIN_ELEM_U64 ? 8 : 4;
+        void *dst = mats[i].elem == MODLIN_ELEM_U64 ? (void *)S.M : (void *)(base + offS[i]);
+        if (dev) {
+            MDL_HIP(hipMemcpyAsync(dst, mats[i].data, elems * esz, hipMemcpyDeviceToDevice, st));
+        } else {
+            rc = copy_h2d(ctx, dst, mats[i].data, elems * esz);
+            if (rc) return rc;
+        }
+        if (mats[i].elem == MODLIN_ELEM_U32)
+            hipLaunchKernelGGL(k_widen_u32, dim3(1024), dim3(256), 0, st, (const uint32_t *)dst, S.M, elems);
+        hipLaunchKernelGGL(k_iota, dim3(std::max<uint32_t>(1, (uint32_t)((n + 255) / 256))), dim3(256), 0, st,
+                           S.perm, (uint32_t)n);
+    }
+    RankMat *dmats = (RankMat *)(base + 0);
+    uint32_t *gx_off = (uint32_t *)(base + off_gx);
+    uint32_t *upd_off = (uint32_t *)(base + off_upd);
+    RankPlan *plan = (RankPlan *)(base + off_plan);
+    uint32_t *dactive = (uint32_t *)(base + off_active);
+    rc = copy_h2d(ctx, dmats, hm.data(), count * sizeof(RankMat));
+    if (rc) return rc;
+    rc = copy_h2d(ctx, dactive, &active, sizeof(uint32_t));
+    if (rc) return rc;
+    MDL_HIP(hipGetLastError());
+
+    const uint32_t nmat = (uint32_t)count;
+    const uint32_t grid_gx = 1024, grid_upd = (uint32_t)ctx->cu_count * 8;
+    const int check_every = 8;
+    std::vector<hipEvent_t> evs;
+    if (profile) {
+        evs.resize(2 * check_every);
+        for (auto &e : evs) MDL_HIP(hipEventCreate(&e));
+    }
+    double upd_ms = 0, upd_launches = 0;
+    uint32_t h_active = active;
+    uint64_t iters = 0;
+    while (h_active > 0) {
+        for (int s = 0; s < check_every; s++) {
+            launch_rank_iteration(st, dmats, nmat, f, b, gx_off, upd_off, plan, dactive, grid_gx, grid_upd,
+                                  profile ? evs[2 * s] : nullptr, profile ? evs[2 * s + 1] : nullptr);
+        }
+        MDL_HIP(hipGetLastError());
+        iters += check_every;
+        rc = copy_d2h(ctx, &h_active, dactive, sizeof(uint32_t));
+        if (rc) return rc;
+        if (profile) {
+            for (int s = 0; s < check_every; s++) {
+                float ms = 0;
+                MDL_HIP(hipEventElapsedTime(&ms, evs[2 * s], evs[2 * s + 1]));
+                upd_ms += ms;
+                upd_launches += 1;
+            }
+        }
+        if (progress) progress(user, count - h_active, count);
+    }
+    for (auto &e : evs) hipEventDestroy(e);
+    rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
+    if (rc) return rc;
+    double macs = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        ranks[i] = hm[i].rank;
+        macs += (double)hm[i].macs;
+    }
+    ctx->stats.macs = macs;
+    ctx->stats.update_ms = upd_ms;
+    ctx->stats.update_launches = upd_launches;
+    ctx->stats.iterations = (double)iters;
+    ctx->stats.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return MODLIN_OK;
+}
+
+}  // namespace mdl

@@ -1,0 +1,58 @@
+// Internal host-side declarations shared by the engine translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/modlin.h"
+#include "fp_dev.hpp"
+
+struct modlin_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    modlin_stats_t stats{};
+    // grow-only device arena reused across calls
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
+    int cu_count = 256;
+    // pinned host staging buffers for host <-> device transfers (double-buffered)
+    void *pinned[2] = {nullptr, nullptr};
+    size_t pinned_bytes = 0;
+    hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+};
+
+namespace mdl {
+
+// Host set-up of the device field parameters from p (Field::new, fp.rs:49-61).
+FieldParams make_params(uint64_t p);
+
+// Records a HIP failure on the context and returns MODLIN_EHIP.
+int hip_fail(modlin_ctx *ctx, hipError_t e, const char *what);
+
+// Ensures the context arena holds at least `bytes`; returns MODLIN_OK or an error.
+int arena_reserve(modlin_ctx *ctx, size_t bytes);
+
+// Host -> device copy of `bytes` from pageable memory through the context's
+// pinned staging buffers, ordered on ctx->stream.  The source may be reused as
+// soon as the call returns.
+int copy_h2d(modlin_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Device -> host copy through the staging buffers; synchronous.
+int copy_d2h(modlin_ctx *ctx, void *dst, const void *src, size_t bytes);
+void release_pinned(modlin_ctx *ctx);
+
+#define MDL_HIP(call)                                              \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return mdl::hip_fail(ctx, e_, #call); \
+    } while (0)
+
+int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint64_t count, const modlin_opts_t *opts,
+               uint64_t *ranks, modlin_progress_fn progress, void *user);
+int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_t count, const modlin_opts_t *opts,
+             uint64_t *lcs);
+
+}  // namespace mdl

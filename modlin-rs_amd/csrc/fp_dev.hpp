@@ -1,0 +1,173 @@
+// Prime-field arithmetic for p < 2^63 on gfx950 (and on the host, for set-up).
+//
+// Contract: the reference's Field (src/fp.rs:34-114).  Every function returns the
+// canonical residue in [0, p), so results are bit-identical to the reference's
+// Barrett reduction whatever reduction is used internally.
+//
+//  * reduce128  -- the reference's own Barrett (fp.rs:71-79): q = mulhi128(t, mu)
+//                  with mu = floor((2^128-1)/p), r = t - q p < 3p.  Exact for any
+//                  128-bit t; used once per lazily accumulated batch.
+//  * mulmod     -- HAC 14.42 Barrett for t = a b < p^2 with k = bitlen(p):
+//                  q1 = t >> (k-1), q3 = (q1 mu_k) >> (k+1), mu_k = floor(2^2k / p),
+//                  0 <= t - q3 p < 3p.  mu_k is stored pre-shifted by (63-k) so q3
+//                  is a single high-word product.  For k <= 62 the remainder fits a
+//                  64-bit word; k = 63 takes a 128-bit correction path.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MDL_HD __host__ __device__ __forceinline__
+#else
+#define MDL_HD inline
+#endif
+
+namespace mdl {
+
+struct FieldParams {
+    uint64_t p;
+    uint64_t mu_hi, mu_lo;  // floor((2^128 - 1) / p)                     fp.rs:52
+    uint64_t mup;           // floor(2^2k / p) << (63 - k), at most 2^64-1
+    uint64_t r64;           // 2^64 mod p
+    uint64_t safe_batch;    // products of (p-1)^2 that fit in 128 bits    fp.rs:53-59
+    uint32_t k;             // bit length of p
+    uint32_t big;           // k == 63
+};
+
+MDL_HD uint64_t umulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// 64x64 -> 128
+MDL_HD void mul_wide(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo) {
+    lo = a * b;
+    hi = umulhi64(a, b);
+}
+
+// 128-bit add in place: (h, l) += (bh, bl)
+MDL_HD void add128(uint64_t &h, uint64_t &l, uint64_t bh, uint64_t bl) {
+    uint64_t nl = l + bl;
+    h += bh + (nl < l ? 1u : 0u);
+    l = nl;
+}
+
+MDL_HD uint64_t add_mod(uint64_t a, uint64_t b, uint64_t p) {  // fp.rs:82-85
+    uint64_t s = a + b;
+    return s >= p ? s - p : s;
+}
+MDL_HD uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t p) {  // fp.rs:88-90
+    return a >= b ? a - b : a + p - b;
+}
+
+// t mod p for any 128-bit t (fp.rs:71-79).
+MDL_HD uint64_t reduce128(const FieldParams &f, uint64_t th, uint64_t tl) {
+    // q = high 128 bits of t * mu  (fp.rs:12-24 with four 64x64 partials)
+    uint64_t a1 = th, a0 = tl, b1 = f.mu_hi, b0 = f.mu_lo;
+    uint64_t lo_hi = umulhi64(a0, b0);
+    uint64_t m1h, m1l, m2h, m2l, hh, hl;
+    mul_wide(a1, b0, m1h, m1l);
+    mul_wide(a0, b1, m2h, m2l);
+    mul_wide(a1, b1, hh, hl);
+    // carry = lo_hi + m1l + m2l (up to 66 bits)
+    uint64_t c = lo_hi + m1l;
+    uint64_t cc = (c < lo_hi) ? 1u : 0u;
+    uint64_t c2 = c + m2l;
+    cc += (c2 < c) ? 1u : 0u;
+    // q = (hh, hl) + m1h + m2h + cc
+    uint64_t qh = hh, ql = hl;
+    add128(qh, ql, 0, m1h);
+    add128(qh, ql, 0, m2h);
+    add128(qh, ql, 0, cc);
+    // r = t - q p   (low 128 bits suffice: 0 <= r < 3p)
+    uint64_t qph, qpl;
+    mul_wide(ql, f.p, qph, qpl);
+    qph += qh * f.p;
+    uint64_t rl = tl - qpl;
+    uint64_t rh = th - qph - (tl < qpl ? 1u : 0u);
+    // r < 3p < 2^65: subtract p while r >= p
+    while (rh != 0 || rl >= f.p) {
+        uint64_t nl = rl - f.p;
+        rh -= (rl < f.p) ? 1u : 0u;
+        rl = nl;
+    }
+    return rl;
+}
+
+// t mod p for t < 2^2k (in particular t = a b with a, b < p).
+MDL_HD uint64_t reduce_prod(const FieldParams &f, uint64_t th, uint64_t tl) {
+    const uint32_t k = f.k;
+    uint64_t q1 = (th << (65 - k)) | (tl >> (k - 1));
+    uint64_t q3 = umulhi64(q1, f.mup);
+    if (!f.big) {
+        uint64_t r = tl - q3 * f.p;  // exact: 0 <= r < 4p < 2^64
+        if (r >= 2 * f.p) r -= 2 * f.p;
+        if (r >= f.p) r -= f.p;
+        return r;
+    }
+    uint64_t qph, qpl;
+    mul_wide(q3, f.p, qph, qpl);
+    uint64_t rl = tl - qpl;
+    uint64_t rh = th - qph - (tl < qpl ? 1u : 0u);
+    while (rh != 0 || rl >= f.p) {
+        uint64_t nl = rl - f.p;
+        rh -= (rl < f.p) ? 1u : 0u;
+        rl = nl;
+    }
+    return rl;
+}
+
+MDL_HD uint64_t mul_mod(const FieldParams &f, uint64_t a, uint64_t b) {  // fp.rs:93-95
+    uint64_t h, l;
+    mul_wide(a, b, h, l);
+    return reduce_prod(f, h, l);
+}
+
+// (pi x - e y) mod p, the fraction-free elimination step.
+MDL_HD uint64_t ff_step(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
+    return sub_mod(mul_mod(f, pi, x), mul_mod(f, e, y), f.p);
+}
+
+MDL_HD uint64_t pow_mod(const FieldParams &f, uint64_t a, uint64_t e) {  // fp.rs:97-107
+    uint64_t r = 1 % f.p;
+    while (e) {
+        if (e & 1) r = mul_mod(f, r, a);
+        a = mul_mod(f, a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+MDL_HD uint64_t inv_mod(const FieldParams &f, uint64_t a) { return pow_mod(f, a, f.p - 2); }  // fp.rs:111-113
+
+// Inverse of a in [1, p) by the binary extended Euclidean algorithm (Knuth,
+// TAOCP vol. 2, 4.5.2): about 2 bitlen(p) short steps instead of the ~2 bitlen(p)
+// dependent Barrett products of Fermat.  Any exact inverse is bit-identical to
+// the reference's a^(p-2).  p = 2 has the single unit 1.
+MDL_HD uint64_t inv_mod_bin(uint64_t a, uint64_t p) {
+    if (p == 2) return 1;
+    uint64_t u = a, v = p, x1 = 1, x2 = 0;
+    while (u != 1 && v != 1) {
+        while ((u & 1) == 0) {
+            u >>= 1;
+            x1 = (x1 & 1) ? (x1 + p) >> 1 : x1 >> 1;  // x1 + p < 2^64 since x1, p < 2^63
+        }
+        while ((v & 1) == 0) {
+            v >>= 1;
+            x2 = (x2 & 1) ? (x2 + p) >> 1 : x2 >> 1;
+        }
+        if (u >= v) {
+            u -= v;
+            x1 = sub_mod(x1, x2, p);
+        } else {
+            v -= u;
+            x2 = sub_mod(x2, x1, p);
+        }
+    }
+    return u == 1 ? x1 : x2;
+}
+
+}  // namespace mdl

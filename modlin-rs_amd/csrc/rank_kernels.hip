@@ -1,0 +1,384 @@
+// Rank over F_p on gfx950: the blocked right-looking elimination of the
+// reference (src/fp.rs:116-255) restructured for the GPU.
+//
+// One iteration handles one column panel [col, jend) of every active matrix:
+//
+//   k_block_ge   one workgroup per matrix.  Loads the top block (the next BMAX
+//                logical rows) of the panel into LDS and eliminates it with the
+//                reference's first-nonzero pivot rule (fp.rs:150-163): a wave
+//                ballot + ffs finds the pivot row, rows are swapped in LDS, and the
+//                elimination is fraction-free (pi*x - e*y), so no modular inverse
+//                sits on the per-column critical path.  The elimination is
+//                Gauss-Jordan and tracks the row operations E, so that at the end
+//                E A = D (diagonal) for A = M[R, J], and Z = A^-1 = D^-1 E needs
+//                only pb independent inverses (Fermat, fp.rs:111-113).
+//   k_plan       prefix sums of the per-matrix tile counts (the work queue).
+//   k_gather_x   Ab = M[O, J] (gathered multipliers' source) and
+//                X = Z M[R, c0:] (the reference's U-block solve, fp.rs:180-200).
+//   k_update     S = M[O, c0:] - Ab X mod p, the trailing update (fp.rs:202-248),
+//                with lazy 128-bit accumulation and one Barrett reduction per
+//                safe_batch products, exactly as the reference accumulates.
+//   k_finish     advances the panel; handles a pivot the top block missed.
+//
+// Exactness.  For any pivot rows R and columns J with A = M[R, J] invertible,
+// rank(M) = |R| + rank(M[O,:] - M[O,J] A^-1 M[R,:]) (Schur complement).  The top
+// block is a prefix of the logical row order, so whenever it has a nonzero in a
+// column the reference's first nonzero row is inside it.  If the block has fewer
+// than b pivots and rows exist below it, the update also covers the panel columns
+// and any nonzero left there (a pivot the block could not see) sends the next
+// panel back to that column, with the first nonzero row swapped to the top --
+// the reference's own pivot for that column.
+#include "rank_common.hpp"
+
+namespace mdl {
+
+// ---------------------------------------------------------------------------
+// k_block_ge
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b) {
+    RankMat &S = mats[blockIdx.x];
+    if (S.done) return;
+    __shared__ uint64_t T[BMAX][BMAX + 1];   // panel part of the top block
+    __shared__ uint64_t E[BMAX][BMAX + 1];   // coefficients on pivot slots
+    __shared__ uint64_t own[BMAX];           // coefficient on the row's own original row
+    __shared__ uint64_t D[BMAX];             // diagonal of E A (pivot rows)
+    __shared__ uint32_t rowid[BMAX];
+    __shared__ uint32_t Jl[BMAX];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
+    const uint32_t jend = min(col + b, n);
+    const uint32_t w = jend - col;
+    const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
+    const uint64_t p = f.p;
+
+    for (uint32_t idx = tid; idx < Bt * w; idx += blockDim.x) {
+        uint32_t q = idx / w, c = idx - q * w;
+        uint32_t r = S.perm[rank0 + q];
+        T[q][c] = S.M[(size_t)r * ld + col + c];
+    }
+    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) E[idx / BMAX][idx % BMAX] = 0;
+    if (tid < Bt) {
+        rowid[tid] = S.perm[rank0 + tid];
+        own[tid] = 1 % p;
+        D[tid] = 0;
+    }
+    __syncthreads();
+
+    const uint32_t lane = tid & 63;
+    const uint32_t row = tid >> 4, sub = tid & 15;
+    uint32_t k = 0;
+    for (uint32_t j = 0; j < w && k < Bt; j++) {
+        // Pivot search: first block position q >= k with a nonzero in column j.
+        // Every wave evaluates the same ballot, so no barrier is needed.
+        bool nz = (lane >= k) && (lane < Bt) && (T[lane][j] != 0);
+        uint64_t mask = __ballot(nz);
+        if (mask == 0) continue;  // uniform: no pivot in this column
+        uint32_t q = __ffsll((unsigned long long)mask) - 1;
+        if (q != k) {  // swap block positions k and q (fp.rs:158-162)
+            __syncthreads();  // every wave has read column j for its ballot before rows move
+            if (tid < w) {
+                uint64_t t = T[k][tid]; T[k][tid] = T[q][tid]; T[q][tid] = t;
+            } else if (tid >= 64 && tid < 64 + k) {
+                uint32_t s = tid - 64;
+                uint64_t t = E[k][s]; E[k][s] = E[q][s]; E[q][s] = t;
+            } else if (tid == 128) {
+                uint64_t t = own[k]; own[k] = own[q]; own[q] = t;
+                uint32_t r = rowid[k]; rowid[k] = rowid[q]; rowid[q] = r;
+            }
+            __syncthreads();
+        }
+        const uint64_t pi = T[k][j];
+        uint64_t e = 0;
+        if (row < Bt && row != k) e = T[row][j];
+        __syncthreads();  // every e read before any row changes
+        if (row < Bt && row != k && e != 0) {
+            for (uint32_t c = j + 1 + sub; c < w; c += 16) T[row][c] = ff_step(f, pi, T[row][c], e, T[k][c]);
+            for (uint32_t s = sub; s < k; s += 16) E[row][s] = ff_step(f, pi, E[row][s], e, E[k][s]);
+            if (sub == 0) {
+                E[row][k] = sub_mod(0, mul_mod(f, e, own[k]), p);
+                own[row] = mul_mod(f, pi, own[row]);
+                if (row < k) D[row] = mul_mod(f, pi, D[row]);
+            }
+        }
+        if (tid == 0) {
+            D[k] = pi;
+            Jl[k] = j;
+        }
+        __syncthreads();
+        k++;
+    }
+    const uint32_t pb = k;
+
+    // Z = D^-1 E' with E'[r][s] = E[r][s] (s != r) and own[r] + E[r][r] (s == r):
+    // slot r of a pivot row and its own original row are the same row.
+    __shared__ uint64_t dinv[BMAX];
+    if (tid < pb) dinv[tid] = inv_mod_bin(D[tid], p);
+    __syncthreads();
+    for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) {
+        uint32_t r = idx / pb, s = idx - r * pb;
+        uint64_t v = (r == s) ? add_mod(own[r], E[r][r], p) : E[r][s];
+        S.Z[r * BMAX + s] = mul_mod(f, dinv[r], v);
+    }
+    if (tid < Bt) S.perm[rank0 + tid] = rowid[tid];
+    if (tid < pb) {
+        S.J[tid] = col + Jl[tid];
+        S.R[tid] = rowid[tid];
+    }
+    if (tid == 0) {
+        S.jend = jend;
+        S.pb = pb;
+        S.rank = rank0 + pb;
+        const bool below = (n - rank0) > Bt;
+        const uint32_t resid = (pb < w && below) ? 1u : 0u;
+        S.resid = resid;
+        if (pb > 0) {
+            S.c0 = resid ? col : jend;
+            S.cend = n;
+        } else {
+            S.c0 = col;
+            S.cend = resid ? jend : col;  // detection only, or nothing at all
+        }
+        S.first_nz_col = 0xffffffffu;
+        S.iters++;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan: tile counts and prefix sums (single workgroup)
+// ---------------------------------------------------------------------------
+constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task
+constexpr uint32_t XCOLS = 64;         // columns per X task
+constexpr uint32_t UT = 64;            // trailing-update tile edge
+
+__device__ __forceinline__ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+__global__ void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
+    if (threadIdx.x != 0) return;
+    uint32_t g = 0, u = 0;
+    for (uint32_t i = 0; i < nmat; i++) {
+        gx_off[i] = g;
+        upd_off[i] = u;
+        const RankMat &S = mats[i];
+        if (S.done) continue;
+        const uint32_t O = S.n - S.rank;
+        const uint32_t cols = S.cend - S.c0;
+        if (O == 0 || cols == 0) continue;
+        if (S.pb > 0) g += cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
+        u += cdiv(O, UT) * cdiv(cols, UT);
+    }
+    gx_off[nmat] = g;
+    upd_off[nmat] = u;
+    plan->gx_total = g;
+    plan->upd_total = u;
+}
+
+__device__ __forceinline__ uint32_t find_mat(const uint32_t *off, uint32_t nmat, uint32_t t) {
+    // largest i with off[i] <= t (off is non-decreasing, off[nmat] > t)
+    uint32_t lo = 0, hi = nmat;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    while (lo + 1 < nmat && off[lo + 1] <= t) lo++;
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// k_gather_x
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather_x(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
+                                                  const RankPlan *plan, FieldParams f) {
+    __shared__ uint64_t Zs[BMAX][BMAX + 1];
+    __shared__ uint64_t Ms[BMAX][XCOLS + 1];
+    const uint32_t total = plan->gx_total;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint32_t mi = find_mat(gx_off, nmat, t);
+        RankMat &S = mats[mi];
+        const uint32_t lt = t - gx_off[mi];
+        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
+        const uint32_t ng = cdiv(O, GATHER_ROWS);
+        if (lt < ng) {
+            // Ab[i][k] = M[perm[rank + i]][J[k]]
+            const uint32_t r0 = lt * GATHER_ROWS;
+            const uint32_t rows = min(GATHER_ROWS, O - r0);
+            for (uint32_t idx = tid; idx < rows * pb; idx += blockDim.x) {
+                uint32_t i = idx / pb, kk = idx - i * pb;
+                uint32_t r = S.perm[S.rank + r0 + i];
+                S.Ab[(size_t)(r0 + i) * BMAX + kk] = S.M[(size_t)r * ld + S.J[kk]];
+            }
+        } else {
+            // X[k][c] = sum_s Z[k][s] M[R[s]][c0 + c]
+            const uint32_t cb = (lt - ng) * XCOLS;
+            const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
+            __syncthreads();
+            for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) Zs[idx / pb][idx % pb] = S.Z[(idx / pb) * BMAX + idx % pb];
+            for (uint32_t idx = tid; idx < pb * XCOLS; idx += blockDim.x) {
+                uint32_t s = idx / XCOLS, c = idx - s * XCOLS;
+                Ms[s][c] = (c < cols) ? S.M[(size_t)S.R[s] * ld + S.c0 + cb + c] : 0;
+            }
+            __syncthreads();
+            const uint64_t sb = f.safe_batch;
+            for (uint32_t o = tid; o < pb * cols; o += blockDim.x) {
+                uint32_t kk = o / cols, c = o - kk * cols;
+                uint64_t ah = 0, al = 0, tot = 0, cnt = 0;
+                for (uint32_t s = 0; s < pb; s++) {
+                    uint64_t h, l;
+                    mul_wide(Zs[kk][s], Ms[s][c], h, l);
+                    add128(ah, al, h, l);
+                    if (++cnt == sb) {
+                        tot = add_mod(tot, reduce128(f, ah, al), f.p);
+                        ah = al = 0;
+                        cnt = 0;
+                    }
+                }
+                if (cnt) tot = add_mod(tot, reduce128(f, ah, al), f.p);
+                S.X[(size_t)kk * S.n + cb + c] = tot;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_update (VALU Barrett): S = M[O, c0:cend] - Ab X
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
+                                                     const RankPlan *plan, FieldParams f) {
+    __shared__ uint64_t As[UT][BMAX + 1];
+    __shared__ uint64_t Xs[BMAX][UT + 1];
+    const uint32_t total = plan->upd_total;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ty = tid >> 4, tx = tid & 15;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint32_t mi = find_mat(upd_off, nmat, t);
+        RankMat &S = mats[mi];
+        const uint32_t lt = t - upd_off[mi];
+        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
+        const uint32_t cols = S.cend - S.c0;
+        const uint32_t nct = cdiv(cols, UT);
+        const uint32_t tr = lt / nct, tc = lt - tr * nct;
+        const uint32_t r0 = tr * UT, cb = tc * UT;
+        const uint32_t nr = min(UT, O - r0), nc = min(UT, cols - cb);
+        __syncthreads();
+        for (uint32_t idx = tid; idx < UT * pb; idx += blockDim.x) {
+            uint32_t i = idx / pb, kk = idx - i * pb;
+            As[i][kk] = (i < nr) ? S.Ab[(size_t)(r0 + i) * BMAX + kk] : 0;
+        }
+        for (uint32_t idx = tid; idx < pb * UT; idx += blockDim.x) {
+            uint32_t kk = idx / UT, c = idx - kk * UT;
+            Xs[kk][c] = (c < nc) ? S.X[(size_t)kk * S.n + cb + c] : 0;
+        }
+        __syncthreads();
+        const uint64_t sb = f.safe_batch, p = f.p;
+        uint64_t ah[4][4], al[4][4], tot[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) ah[a][c] = al[a][c] = tot[a][c] = 0;
+        uint64_t cnt = 0;
+        for (uint32_t kk = 0; kk < pb; kk++) {
+            uint64_t av[4], xv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = As[ty * 4 + a][kk];
+#pragma unroll
+            for (int c = 0; c < 4; c++) xv[c] = Xs[kk][tx * 4 + c];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    uint64_t h, l;
+                    mul_wide(av[a], xv[c], h, l);
+                    add128(ah[a][c], al[a][c], h, l);
+                }
+            if (++cnt == sb) {
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        tot[a][c] = add_mod(tot[a][c], reduce128(f, ah[a][c], al[a][c]), p);
+                        ah[a][c] = al[a][c] = 0;
+                    }
+                cnt = 0;
+            }
+        }
+        if (cnt) {
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) tot[a][c] = add_mod(tot[a][c], reduce128(f, ah[a][c], al[a][c]), p);
+        }
+        uint32_t nzcol = 0xffffffffu;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const uint32_t i = ty * 4 + a;
+            if (i >= nr) continue;
+            const uint32_t r = S.perm[S.rank + r0 + i];
+            uint64_t *mrow = S.M + (size_t)r * ld + S.c0 + cb;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t cc = tx * 4 + c;
+                if (cc >= nc) continue;
+                uint64_t v = sub_mod(mrow[cc], tot[a][c], p);
+                if (pb > 0) mrow[cc] = v;
+                const uint32_t gcol = S.c0 + cb + cc;
+                if (v != 0 && gcol < S.jend) nzcol = min(nzcol, gcol);
+            }
+        }
+        if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
+        if (tid == 0 && pb > 0) atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_finish: advance the panel; bring a missed pivot to the top of the block
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active) {
+    RankMat &S = mats[blockIdx.x];
+    if (S.done) return;
+    __shared__ uint32_t best;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = S.n;
+    uint32_t jstar = S.first_nz_col;
+    const bool miss = S.resid && jstar < S.jend;
+    if (miss) {
+        if (tid == 0) best = 0xffffffffu;
+        __syncthreads();
+        uint32_t mine = 0xffffffffu;
+        for (uint32_t q = S.rank + tid; q < n; q += blockDim.x) {
+            if (S.M[(size_t)S.perm[q] * S.ld + jstar] != 0) { mine = q; break; }
+        }
+        if (mine != 0xffffffffu) atomicMin(&best, mine);
+        __syncthreads();
+        if (tid == 0 && best != 0xffffffffu && best != S.rank) {
+            uint32_t t = S.perm[S.rank];
+            S.perm[S.rank] = S.perm[best];
+            S.perm[best] = t;
+        }
+    }
+    if (tid == 0) {
+        S.col = miss ? jstar : S.jend;
+        if (S.col >= n || S.rank >= n) {
+            S.done = 1;
+            atomicSub(active, 1u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers (one panel iteration of every matrix in the batch)
+// ---------------------------------------------------------------------------
+void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
+                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
+                           uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1) {
+    hipLaunchKernelGGL(k_block_ge, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, dmats, nmat, gx_off, upd_off, plan);
+    hipLaunchKernelGGL(k_gather_x, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f);
+    if (ev_upd0) hipEventRecord(ev_upd0, st);
+    hipLaunchKernelGGL(k_update_valu, dim3(grid_upd), dim3(256), 0, st, dmats, nmat, upd_off, plan, f);
+    if (ev_upd1) hipEventRecord(ev_upd1, st);
+    hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active);
+}
+
+}  // namespace mdl

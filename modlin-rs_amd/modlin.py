@@ -1,0 +1,337 @@
+"""Python binding of libmodlin.so (the C ABI in include/modlin.h).
+
+Mirrors the reference crate's public surface (src/lib.rs:9-14):
+
+  Field(p)                       fp::Field::new            src/fp.rs:49-61
+  Field.reduce / reduce_u128     Field::reduce[_u128]      src/fp.rs:63-79
+  Context.rank(field, m)         fp::rank                  src/fp.rs:125
+  Context.linear_complexity      fp::linear_complexity     src/fp.rs:260
+  corank_prob, corank_tail_pvalue, lc_left_tail_pvalue, pretty_p_value, P_FLOOR
+                                 stats::*                  src/stats.rs:16-137
+  Prng                           prng::Prng                src/prng.rs
+
+plus the batched entry points (rank_batch, lc_batch).  Errors the reference
+turns into panics (p outside [2, 2^63), an empty BM sequence) raise ValueError
+here; a missing library or GPU raises RuntimeError -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmodlin.so")
+
+MODLIN_OK, MODLIN_EINVAL, MODLIN_ENOMEM, MODLIN_EHIP, MODLIN_ENODEV, MODLIN_EUNSUPPORTED = range(6)
+METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA = 0, 1, 2
+ELEM_U64, ELEM_U32 = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+FLAG_PROFILE = 1
+P_FLOOR = 1e-307  # stats.rs:16
+
+PRNG_XOROSHIRO128PP, PRNG_MIXMAX17, PRNG_MIXMAX240, PRNG_MIXMAX256, PRNG_SPLITMIX64, PRNG_XOSHIRO256SS = range(6)
+
+# Exported symbols, in the order of include/modlin.h.
+EXPORTS = [
+    "modlin_field_init", "modlin_reduce", "modlin_reduce_u128",
+    "modlin_ctx_create", "modlin_ctx_destroy", "modlin_ctx_last_error", "modlin_ctx_last_stats",
+    "modlin_rank", "modlin_linear_complexity", "modlin_rank_batch", "modlin_lc_batch",
+    "modlin_corank_prob", "modlin_corank_tail_pvalue", "modlin_lc_left_tail_pvalue", "modlin_pretty_p_value",
+    "modlin_prng_fill", "modlin_prng_name",
+    "modlin_util_combine_rows", "modlin_util_lfsr",
+    "modlin_dev_alloc", "modlin_dev_free", "modlin_memcpy_h2d", "modlin_memcpy_d2h", "modlin_synchronize",
+]
+
+
+class _FieldT(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint64), ("mu_hi", ctypes.c_uint64), ("mu_lo", ctypes.c_uint64),
+                ("safe_batch", ctypes.c_uint64)]
+
+
+class _MatDesc(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_uint64), ("elem", ctypes.c_uint32),
+                ("mem", ctypes.c_uint32)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_uint32), ("panel", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_double) for name in
+                ("macs", "update_ms", "update_launches", "limb_count", "iterations", "bm_bytes", "bm_ms",
+                 "total_ms")]
+
+
+_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libmodlin.so (built in-tree); raises RuntimeError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(LIB_PATH)
+    u64, sz, vp = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_void_p
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    lib.modlin_field_init.argtypes = [u64, ctypes.POINTER(_FieldT)]
+    lib.modlin_reduce.argtypes = [ctypes.POINTER(_FieldT), u64]
+    lib.modlin_reduce.restype = u64
+    lib.modlin_reduce_u128.argtypes = [ctypes.POINTER(_FieldT), u64, u64]
+    lib.modlin_reduce_u128.restype = u64
+    lib.modlin_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.modlin_ctx_destroy.argtypes = [vp]
+    lib.modlin_ctx_destroy.restype = None
+    lib.modlin_ctx_last_error.argtypes = [vp]
+    lib.modlin_ctx_last_error.restype = ctypes.c_char_p
+    lib.modlin_ctx_last_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
+    lib.modlin_rank.argtypes = [vp, ctypes.POINTER(_FieldT), P64, u64, _PROGRESS, vp, P64]
+    lib.modlin_linear_complexity.argtypes = [vp, ctypes.POINTER(_FieldT), P64, u64, _PROGRESS, vp, P64]
+    lib.modlin_rank_batch.argtypes = [vp, ctypes.POINTER(_FieldT), ctypes.POINTER(_MatDesc), u64,
+                                      ctypes.POINTER(_Opts), P64]
+    lib.modlin_lc_batch.argtypes = [vp, ctypes.POINTER(_FieldT), ctypes.POINTER(_MatDesc), u64,
+                                    ctypes.POINTER(_Opts), P64]
+    for name in ("modlin_corank_prob", "modlin_corank_tail_pvalue", "modlin_lc_left_tail_pvalue"):
+        getattr(lib, name).argtypes = [u64, u64, u64]
+        getattr(lib, name).restype = ctypes.c_double
+    lib.modlin_pretty_p_value.argtypes = [ctypes.c_double, ctypes.c_char_p, sz]
+    lib.modlin_prng_fill.argtypes = [ctypes.c_uint32, u64, u64, P64, u64]
+    lib.modlin_prng_name.argtypes = [ctypes.c_uint32]
+    lib.modlin_prng_name.restype = ctypes.c_char_p
+    lib.modlin_util_combine_rows.argtypes = [u64, P64, u64, u64, P64, u64, P64]
+    lib.modlin_util_lfsr.argtypes = [u64, P64, u64, P64, u64, P64]
+    lib.modlin_dev_alloc.argtypes = [vp, u64, ctypes.POINTER(vp)]
+    lib.modlin_dev_free.argtypes = [vp, vp]
+    lib.modlin_memcpy_h2d.argtypes = [vp, vp, vp, u64]
+    lib.modlin_memcpy_d2h.argtypes = [vp, vp, vp, u64]
+    lib.modlin_synchronize.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def _u64p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+class Field:
+    """A prime field F_p, p < 2^63 (src/fp.rs:34-61)."""
+
+    def __init__(self, p: int):
+        lib = load()
+        self._f = _FieldT()
+        if p < 2 or p >= 1 << 63 or lib.modlin_field_init(p, ctypes.byref(self._f)) != MODLIN_OK:
+            raise ValueError("modulus must be at least 2 and below 2^63")
+        self.p = p
+
+    @property
+    def mu(self) -> int:
+        return (self._f.mu_hi << 64) | self._f.mu_lo
+
+    @property
+    def safe_batch(self) -> int:
+        return self._f.safe_batch
+
+    def reduce(self, x: int) -> int:
+        return load().modlin_reduce(ctypes.byref(self._f), x)
+
+    def reduce_u128(self, t: int) -> int:
+        return load().modlin_reduce_u128(ctypes.byref(self._f), t >> 64, t & ((1 << 64) - 1))
+
+
+class ModlinError(RuntimeError):
+    pass
+
+
+class Context:
+    """A device context: stream, device arena, last-call statistics."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = ctypes.c_void_p()
+        rc = lib.modlin_ctx_create(device, ctypes.byref(h))
+        if rc != MODLIN_OK:
+            raise RuntimeError(f"modlin_ctx_create failed with status {rc} (no gfx950 device?)")
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.modlin_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc == MODLIN_OK:
+            return
+        msg = self._lib.modlin_ctx_last_error(self._h).decode(errors="replace")
+        if rc == MODLIN_EINVAL:
+            raise ValueError(f"{what}: {msg}")
+        raise ModlinError(f"{what}: status {rc}: {msg}")
+
+    def stats(self) -> dict:
+        s = _Stats()
+        self._lib.modlin_ctx_last_stats(self._h, ctypes.byref(s))
+        return {name: getattr(s, name) for name, _ in _Stats._fields_}
+
+    # --- single-item entry points (src/fp.rs:125, 260) -------------------
+    def rank(self, field: Field, m: np.ndarray, n: int | None = None) -> int:
+        a = np.ascontiguousarray(m, dtype=np.uint64)
+        if n is None:
+            n = int(round(a.size ** 0.5))
+        if a.size != n * n:
+            raise ValueError("matrix must hold n*n entries")
+        out = ctypes.c_uint64()
+        ptr = _u64p(a) if a.size else None
+        self._check(self._lib.modlin_rank(self._h, ctypes.byref(field._f), ptr, n, _PROGRESS(), None,
+                                          ctypes.byref(out)), "rank")
+        return out.value
+
+    def linear_complexity(self, field: Field, s: np.ndarray) -> int:
+        a = np.ascontiguousarray(s, dtype=np.uint64)
+        out = ctypes.c_uint64()
+        ptr = _u64p(a) if a.size else None
+        self._check(self._lib.modlin_linear_complexity(self._h, ctypes.byref(field._f), ptr, a.size, _PROGRESS(),
+                                                       None, ctypes.byref(out)), "linear_complexity")
+        return out.value
+
+    # --- batched entry points -------------------------------------------
+    @staticmethod
+    def _descs(items, square: bool):
+        keep, descs = [], []
+        for it in items:
+            if isinstance(it, DeviceBuffer):
+                descs.append(_MatDesc(it.ptr, it.n, it.elem, MEM_DEVICE))
+                continue
+            a = np.ascontiguousarray(it)
+            if a.dtype == np.uint32:
+                elem = ELEM_U32
+            else:
+                a = np.ascontiguousarray(a, dtype=np.uint64)
+                elem = ELEM_U64
+            n = int(round(a.size ** 0.5)) if square else a.size
+            if square and n * n != a.size:
+                raise ValueError("matrix must hold n*n entries")
+            keep.append(a)
+            descs.append(_MatDesc(a.ctypes.data if a.size else None, n, elem, MEM_HOST))
+        arr = (_MatDesc * max(1, len(descs)))(*descs)
+        return arr, keep
+
+    def rank_batch(self, field: Field, mats: Sequence, method: int = METHOD_AUTO, panel: int = 0,
+                   profile: bool = False) -> list[int]:
+        arr, keep = self._descs(mats, True)
+        out = np.zeros(max(1, len(mats)), dtype=np.uint64)
+        opts = _Opts(method, panel, FLAG_PROFILE if profile else 0, 0)
+        self._check(self._lib.modlin_rank_batch(self._h, ctypes.byref(field._f), arr, len(mats), ctypes.byref(opts),
+                                                _u64p(out)), "rank_batch")
+        return [int(x) for x in out[:len(mats)]]
+
+    def lc_batch(self, field: Field, seqs: Sequence, profile: bool = False) -> list[int]:
+        arr, keep = self._descs(seqs, False)
+        out = np.zeros(max(1, len(seqs)), dtype=np.uint64)
+        opts = _Opts(0, 0, FLAG_PROFILE if profile else 0, 0)
+        self._check(self._lib.modlin_lc_batch(self._h, ctypes.byref(field._f), arr, len(seqs), ctypes.byref(opts),
+                                              _u64p(out)), "lc_batch")
+        return [int(x) for x in out[:len(seqs)]]
+
+    # --- device buffers ---------------------------------------------------
+    def to_device(self, a: np.ndarray, square: bool = True) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        if a.dtype not in (np.uint32, np.uint64):
+            a = a.astype(np.uint64)
+        ptr = ctypes.c_void_p()
+        self._check(self._lib.modlin_dev_alloc(self._h, a.nbytes, ctypes.byref(ptr)), "dev_alloc")
+        self._check(self._lib.modlin_memcpy_h2d(self._h, ptr, a.ctypes.data, a.nbytes), "memcpy_h2d")
+        n = int(round(a.size ** 0.5)) if square else a.size
+        return DeviceBuffer(self, ptr.value, n, ELEM_U32 if a.dtype == np.uint32 else ELEM_U64, a.nbytes)
+
+    def synchronize(self):
+        self._check(self._lib.modlin_synchronize(self._h), "synchronize")
+
+
+class DeviceBuffer:
+    """A device allocation holding one matrix or sequence (zero-copy input)."""
+
+    def __init__(self, ctx: Context, ptr: int, n: int, elem: int, nbytes: int):
+        self.ctx, self.ptr, self.n, self.elem, self.nbytes = ctx, ptr, n, elem, nbytes
+
+    def free(self):
+        if self.ptr:
+            self.ctx._lib.modlin_dev_free(self.ctx._h, self.ptr)
+            self.ptr = 0
+
+
+# --- statistics (src/stats.rs) --------------------------------------------
+def corank_prob(p: int, n: int, d: int) -> float:
+    return load().modlin_corank_prob(p, n, d)
+
+
+def corank_tail_pvalue(p: int, n: int, c: int) -> float:
+    return load().modlin_corank_tail_pvalue(p, n, c)
+
+
+def lc_left_tail_pvalue(p: int, n: int, ell: int) -> float:
+    return load().modlin_lc_left_tail_pvalue(p, n, ell)
+
+
+def pretty_p_value(x: float) -> str:
+    buf = ctypes.create_string_buffer(64)
+    if load().modlin_pretty_p_value(x, buf, 64) < 0:
+        raise ValueError("buffer too small")
+    return buf.value.decode()
+
+
+# --- generators (src/prng.rs) ---------------------------------------------
+def prng_fill(kind: int, seed: int, count: int, skip: int = 0) -> np.ndarray:
+    out = np.zeros(max(1, count), dtype=np.uint64)
+    rc = load().modlin_prng_fill(kind, seed, skip, _u64p(out), count)
+    if rc != MODLIN_OK:
+        raise ValueError("the zero seed is not allowed for MIXMAX" if rc == MODLIN_EINVAL else f"status {rc}")
+    return out[:count]
+
+
+def prng_name(kind: int) -> str:
+    r = load().modlin_prng_name(kind)
+    if r is None:
+        raise ValueError("unknown generator")
+    return r.decode()
+
+
+# --- workload helpers (host) ----------------------------------------------
+def combine_rows(p: int, rows: np.ndarray, coef: np.ndarray) -> np.ndarray:
+    """out[i] = sum_k coef[i, k] * rows[k] mod p (exact)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    coef = np.ascontiguousarray(coef, dtype=np.uint64)
+    nsrc, n = rows.shape
+    nout = coef.shape[0]
+    out = np.zeros((max(1, nout), max(1, n)), dtype=np.uint64)
+    rc = load().modlin_util_combine_rows(p, _u64p(rows), nsrc, n, _u64p(coef), nout, _u64p(out))
+    if rc != MODLIN_OK:
+        raise ValueError("combine_rows: bad arguments")
+    return out[:nout, :n]
+
+
+def lfsr(p: int, coef: np.ndarray, init: np.ndarray, n: int) -> np.ndarray:
+    """s[i] = sum_j coef[j] s[i-1-j] mod p for i >= L = len(coef), s[:L] = init."""
+    coef = np.ascontiguousarray(coef, dtype=np.uint64)
+    init = np.ascontiguousarray(init, dtype=np.uint64)
+    out = np.zeros(max(1, n), dtype=np.uint64)
+    rc = load().modlin_util_lfsr(p, _u64p(coef), coef.size, _u64p(init), n, _u64p(out))
+    if rc != MODLIN_OK:
+        raise ValueError("lfsr: bad arguments")
+    return out[:n]

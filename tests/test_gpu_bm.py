@@ -1,0 +1,86 @@
+"""GPU parity of Berlekamp-Massey (libmodlin.so through its C ABI) against the oracle.
+
+Bit-exact bar: the linear complexity of every sequence equals the oracle's.
+Covers the reference's known answers and family (test_linear_complexity.rs), the
+README answers (272, 503, 5000, 65280), batches of uneven lengths, recurrences
+of known order, u32 inputs and the empty-sequence error.
+"""
+import numpy as np
+import pytest
+
+import families
+from conftest import modlin, oracle, workloads
+
+pytestmark = pytest.mark.gpu
+M61 = (1 << 61) - 1
+
+
+def test_known_answers(ctx):
+    p = M61
+    f = modlin.Field(p)
+    s = [1, 2, 3]
+    for i in range(3, 40):
+        s.append((3 * s[i - 1] + 5 * s[i - 2] + 7 * s[i - 3]) % p)
+    assert ctx.linear_complexity(f, np.array(s, dtype=np.uint64)) == 3
+    assert ctx.linear_complexity(f, np.full(20, 4, dtype=np.uint64)) == 1
+    assert ctx.linear_complexity(f, np.zeros(20, dtype=np.uint64)) == 0
+
+
+def test_empty_sequence_rejected(ctx):
+    with pytest.raises(ValueError):
+        ctx.linear_complexity(modlin.Field(M61), np.zeros(0, dtype=np.uint64))
+
+
+def test_reference_family(ctx):
+    by_p = {}
+    for p, label, s, k in families.lc_family():
+        by_p.setdefault(p, []).append((label, s, k))
+    for p, items in by_p.items():
+        got = ctx.lc_batch(modlin.Field(p), [s for _, s, _ in items])
+        for (label, s, k), g in zip(items, got):
+            assert g == oracle.lc_ref(p, s), (p, label)
+            if k is not None:
+                assert g <= k
+
+
+def test_readme_short(ctx):
+    s = (modlin.prng_fill(modlin.PRNG_MIXMAX17, 1, 1000) % np.uint64(M61)).astype(np.uint64)
+    assert ctx.linear_complexity(modlin.Field(M61), s) == 272
+    s = (modlin.prng_fill(modlin.PRNG_XOROSHIRO128PP, 0, 1000) % np.uint64(2)).astype(np.uint64)
+    assert ctx.linear_complexity(modlin.Field(2), s) == 503
+    s = (modlin.prng_fill(modlin.PRNG_XOROSHIRO128PP, 0, 10000) % np.uint64(M61)).astype(np.uint64)
+    assert ctx.linear_complexity(modlin.Field(M61), s) == 5000
+
+
+def test_c4_subset_against_oracle(ctx):
+    seqs, specs = workloads.c4(count=48)
+    seqs = [s[:3000] for s in seqs]
+    got = ctx.lc_batch(modlin.Field(M61), seqs)
+    for i, (s, g) in enumerate(zip(seqs, got)):
+        assert g == oracle.lc(M61, s), (i, specs[i])
+
+
+def test_recurrence_orders(ctx):
+    rng = np.random.default_rng(4)
+    for p in [65521, (1 << 31) - 1, M61, (1 << 63) - 25]:
+        seqs, orders = [], []
+        for L in [1, 16, 100, 400]:
+            coef = rng.integers(1, p, size=L, dtype=np.uint64)
+            init = rng.integers(0, p, size=L, dtype=np.uint64)
+            seqs.append(modlin.lfsr(p, coef, init, 3 * L + 50))
+            orders.append(L)
+        assert ctx.lc_batch(modlin.Field(p), seqs) == orders
+
+
+def test_u32_inputs(ctx):
+    p = 65521
+    rng = np.random.default_rng(2)
+    seqs = [rng.integers(0, p, size=n).astype(np.uint32) for n in [10, 333, 2000]]
+    got = ctx.lc_batch(modlin.Field(p), seqs)
+    assert got == [oracle.lc(p, s.astype(np.uint64)) for s in seqs]
+
+
+def test_readme_mixmax256_long(ctx):
+    s = workloads.c5_bm(200_000)
+    print("BM n=200000 MIXMAX-256", flush=True)
+    assert ctx.linear_complexity(modlin.Field(M61), s) == 65280   # README.md:107

@@ -1,0 +1,137 @@
+"""GPU parity of rank (libmodlin.so through its C ABI) against the oracle.
+
+Bit-exact bar: the rank (hence corank) of every matrix equals the oracle's.
+Covers the reference's known answers and family (test_rank.rs), the README
+cases, panel and top-block boundaries (sides 1..129, 63/64/65), pivots the top
+block cannot see (zero top rows, permutation matrices), sparse and zero
+matrices, the prime classes of the Barrett code paths (k <= 62, k = 63), u32
+inputs, batches of mixed sizes, and the planted-corank matrices of the large
+configurations (size-independent property: corank exactly 64).
+"""
+import numpy as np
+import pytest
+
+import families
+from conftest import modlin, oracle, workloads
+
+pytestmark = pytest.mark.gpu
+M61 = (1 << 61) - 1
+METHODS = [modlin.METHOD_PLAIN]
+
+
+def _rank_all(ctx, p, mats, method=modlin.METHOD_PLAIN, panel=0):
+    return ctx.rank_batch(modlin.Field(p), mats, method=method, panel=panel)
+
+
+def test_known_matrices(ctx):
+    f = modlin.Field(M61)
+    assert ctx.rank(f, np.eye(3, dtype=np.uint64)) == 3
+    assert ctx.rank(f, np.array([1, 2, 3, 4, 5, 6, 5, 7, 9], dtype=np.uint64)) == 2
+    assert ctx.rank(f, np.zeros(16, dtype=np.uint64)) == 0
+    assert ctx.rank(modlin.Field(2), np.array([1, 1, 0, 0, 1, 1, 1, 0, 1], dtype=np.uint64)) == 2
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_reference_family(ctx, method):
+    by_p = {}
+    for p, n, target, a in families.rank_family():
+        by_p.setdefault(p, []).append((n, target, a))
+    for p, items in by_p.items():
+        got = _rank_all(ctx, p, [a for _, _, a in items], method)
+        for (n, target, a), g in zip(items, got):
+            print(f"family p={p} n={n} target={target}", flush=True)
+            assert g == oracle.rank_ref(p, a, n), (p, n, target)
+
+
+def test_readme_c1(ctx):
+    mm, ctl = workloads.c1()
+    r = _rank_all(ctx, M61, [mm, ctl])
+    assert [500 - r[0], 500 - r[1]] == [432, 0]
+
+
+def _hidden_pivot_cases(p, rng):
+    cases = []
+    for n in [5, 70, 130]:
+        a = np.zeros((n, n), dtype=np.uint64)
+        a[np.arange(n), n - 1 - np.arange(n)] = 1        # anti-diagonal permutation
+        cases.append(a)
+        b = rng.integers(0, p, size=(n, n)).astype(np.uint64)
+        b[: min(n - 1, 64)] = 0                          # zero top block, pivots below
+        cases.append(b)
+        c = np.zeros((n, n), dtype=np.uint64)
+        c[n - 1, :] = rng.integers(1, p, size=n)         # a single nonzero row at the bottom
+        cases.append(c)
+        d = rng.integers(0, p, size=(n, n)).astype(np.uint64)
+        d[:, : n // 2] = 0                               # zero leading columns
+        cases.append(d)
+        e = (rng.random((n, n)) < 0.02).astype(np.uint64) * rng.integers(1, p, size=(n, n)).astype(np.uint64)
+        cases.append(e)                                  # sparse
+    return cases
+
+
+@pytest.mark.parametrize("p", [2, 97, M61])
+def test_hidden_pivots_and_sparse(ctx, p):
+    rng = np.random.default_rng(p % 1000)
+    cases = _hidden_pivot_cases(p, rng)
+    got = _rank_all(ctx, p, cases)
+    for a, g in zip(cases, got):
+        assert g == oracle.rank_ref(p, a, a.shape[0])
+
+
+@pytest.mark.parametrize("p", [2, 3, 127, 131, 32749, 65521, 8388593, (1 << 31) - 1, M61, (1 << 62) + 135,
+                               (1 << 63) - 25])
+def test_prime_classes(ctx, p):
+    rng = np.random.default_rng(7)
+    mats = []
+    for n in [1, 2, 63, 64, 65, 127, 128, 129, 200]:
+        mats.append(rng.integers(0, p, size=(n, n), dtype=np.uint64))
+        r = max(1, n // 3)
+        b = rng.integers(0, p, size=(r, n), dtype=np.uint64)
+        cf = rng.integers(0, p, size=(n, r), dtype=np.uint64)
+        mats.append(modlin.combine_rows(p, b, cf))          # rank <= n/3
+    got = _rank_all(ctx, p, mats)
+    for a, g in zip(mats, got):
+        assert g == oracle.rank(p, a, a.shape[0]), (p, a.shape)
+
+
+def test_panel_widths(ctx):
+    rng = np.random.default_rng(1)
+    p = M61
+    mats = [rng.integers(0, p, size=(n, n), dtype=np.uint64) for n in [50, 100, 150]]
+    mats.append(workloads.c3(300, p))
+    want = [oracle.rank(p, a, a.shape[0]) for a in mats]
+    for b in [1, 2, 7, 16, 32, 64]:
+        assert _rank_all(ctx, p, mats, panel=b) == want, b
+
+
+def test_u32_inputs(ctx):
+    p = 65521
+    mats = workloads.c5_rank(count=4, n=300, p=p)
+    got = _rank_all(ctx, p, mats)
+    for a, g in zip(mats, got):
+        assert g == oracle.rank(p, a.astype(np.uint64), 300)
+    # a u32 rank-deficient matrix
+    b = modlin.combine_rows(p, mats[0][:10].astype(np.uint64), mats[1][:, :10].astype(np.uint64))
+    assert _rank_all(ctx, p, [b.astype(np.uint32)]) == [10]
+
+
+def test_mixed_batch_c2_subset(ctx):
+    mats = workloads.c2(count=16, max_side=700)
+    got = _rank_all(ctx, M61, mats)
+    for i, (a, g) in enumerate(zip(mats, got)):
+        assert g == oracle.rank(M61, a, a.shape[0]), i
+
+
+def test_empty_and_tiny(ctx):
+    f = modlin.Field(97)
+    assert ctx.rank_batch(f, []) == []
+    assert ctx.rank_batch(f, [np.zeros(0, dtype=np.uint64)]) == [0]
+    assert ctx.rank_batch(f, [np.array([5], dtype=np.uint64), np.array([0], dtype=np.uint64)]) == [1, 0]
+
+
+@pytest.mark.parametrize("p", [M61, (1 << 31) - 1])
+def test_planted_corank_large(ctx, p):
+    n = 4096
+    a = workloads.c3(n, p)
+    print(f"planted n={n} p={p}", flush=True)
+    assert _rank_all(ctx, p, [a]) == [n - 64]
