@@ -1,0 +1,156 @@
+"""Benchmark: F_p rank columns/s of the batched rank on one MI355X.
+
+Workload (one "step"): rank of the 64-matrix batch of the batched-rank
+configuration -- square sides drawn log-uniform in [256, 4096] from
+SplitMix64 seed 7, p = 2^61-1, u64 entries; even matrices uniform from
+xoshiro256** seed 7, odd matrices consecutive MIXMAX-17 seed-7 stretches
+(modlin-rs_amd/workloads.py:c2).  Inputs are resident in HBM before timing;
+each step copies them into the library's workspace (the reference destroys its
+input too) and runs the whole batch through one launch sequence.
+
+Prints one JSON line.  value = total matrix columns ranked per second
+(sum of sides / step time).  `roofline` refers to the trailing-update kernel,
+timed with HIP events on the library's stream; `cpu_baseline` is the oracle's
+OpenMP port of fp.rs on a bounded sample of the same batch on this host.
+Extra keys report the plain (VALU) path on the same batch and the per-batch
+corank check against the CPU-computed fixtures.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "modlin-rs_amd"))
+
+import modlin as M  # noqa: E402
+import workloads as W  # noqa: E402
+
+M61 = (1 << 61) - 1
+PEAK_INT8_TOPS = 5000.0   # MI355X dense int8 MFMA, TOP/s
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E, GB/s
+
+
+def limb_count(p: int) -> int:
+    """Signed 8-bit digits needed for the balanced residues |x| <= (p-1)/2."""
+    half, L = (p - 1) // 2, 1
+    while 127 * (256 ** L - 1) // 255 < half:
+        L += 1
+    return L
+
+
+def cpu_baseline(mats, budget_n3=4.0e10):
+    """The oracle's port of fp::rank (OpenMP over rows, as rayon does) on the
+    first matrices of the batch whose sum of n^3 stays within the budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    sample, acc = [], 0.0
+    for m in mats:
+        n = m.shape[0]
+        if sample and acc + float(n) ** 3 > budget_n3:
+            break
+        sample.append(m)
+        acc += float(n) ** 3
+    t0 = time.perf_counter()
+    for m in sample:
+        O.rank(M61, m, m.shape[0])
+    dt = time.perf_counter() - t0
+    cols = sum(m.shape[0] for m in sample)
+    return {"value": cols / dt, "unit": "columns/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"first {len(sample)} matrices of the batch (sum n = {cols}, sum n^3 = {acc:.3g}), "
+                      f"{dt:.1f} s"}
+
+
+def run(ctx, field, dev, method, steps, warmup):
+    for _ in range(warmup):
+        ctx.rank_batch(field, dev, method=method)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ranks = None
+    for _ in range(steps):
+        ranks = ctx.rank_batch(field, dev, method=method)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    prof_ranks = ctx.rank_batch(field, dev, method=method, profile=True)
+    st = ctx.stats()
+    return ranks, dt / steps, st, prof_ranks
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--method", choices=["auto", "plain", "mfma"], default="auto")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+    method = {"auto": M.METHOD_AUTO, "plain": M.METHOD_PLAIN, "mfma": M.METHOD_MFMA}[args.method]
+
+    mats = W.c2()
+    sides = [m.shape[0] for m in mats]
+    cols = sum(sides)
+    field = M.Field(M61)
+    ctx = M.Context(0)
+    dev = [ctx.to_device(m) for m in mats]
+
+    ranks, t_step, st, _ = run(ctx, field, dev, method, args.steps, args.warmup)
+    coranks = [n - r for n, r in zip(sides, ranks)]
+    golden_path = os.path.join(ROOT, "tests", "golden", "c2_coranks.json")
+    check = None
+    if os.path.exists(golden_path):
+        check = json.load(open(golden_path))["coranks"] == coranks
+
+    L = limb_count(M61) if st["limb_count"] else 0
+    upd_s = st["update_ms"] / 1e3
+    if L:
+        ops = 2.0 * L * L * st["macs"]
+        achieved = ops / upd_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
+    else:
+        # VALU update: algorithmic HBM bytes = 16 per updated element (read + write)
+        # + 8 per operand element of each tile (counted by the kernel)
+        achieved = st["update_bytes"] / upd_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None}
+
+    out = {
+        "metric": "Fp rank cols/s & BM steps/s: batched limb-MFMA vs plain GPU path, bit-exact",
+        "value": cols / t_step,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (F_p, p=2^61-1)" + (f"; update int8 x{L} limbs, int32 accumulate" if L else ""),
+        "data": "synthetic (deterministic generators, see workloads.py)",
+        "config": {"workload": "batched rank: 64 square matrices, sides log-uniform in [256, 4096] (seed 7), "
+                               "p=2^61-1, half xoshiro256** uniform, half MIXMAX-17",
+                   "matrices": len(mats), "sum_sides": cols, "max_side": max(sides), "method": args.method},
+        "roofline": roof,
+        "update_ms_per_step": st["update_ms"],
+        "update_macs_per_step": st["macs"],
+        "coranks_match_fixture": check,
+    }
+    if not args.no_plain and method != M.METHOD_PLAIN:
+        _, t_plain, st_p, _ = run(ctx, field, dev, M.METHOD_PLAIN, max(1, args.steps // 2), 0)
+        out["plain_columns_per_s"] = cols / t_plain
+        out["speedup_vs_plain"] = t_plain / t_step
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mats)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,8 @@ typedef struct modlin_stats {
     double bm_bytes;        /* algorithmic BM bytes */
     double bm_ms;           /* BM kernel time (PROFILE) */
     double total_ms;        /* wall time of the device part of the call */
+    double update_bytes;    /* algorithmic HBM bytes of the trailing updates: 16 per updated
+                               element (read + write) + 8 per operand element per tile */
 } modlin_stats_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */

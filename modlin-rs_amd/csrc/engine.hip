@@ -12,10 +12,44 @@
 
 namespace mdl {
 
-// rank_kernels.hip
+// rank_kernels.hip, mfma_update.hip
 void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
                            uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
                            uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1);
+void launch_rank_iteration_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f,
+                                const LimbParams &lp, uint32_t b, uint32_t *gx_off, uint32_t *upd_off,
+                                RankPlan *plan, uint32_t *active, uint32_t grid_gx, uint32_t grid_upd,
+                                hipEvent_t e0, hipEvent_t e1);
+
+// Signed 8-bit digits for the balanced residues |x| <= (p-1)/2.
+uint32_t limb_count(uint64_t p) {
+    typedef unsigned __int128 u128;
+    const u128 half = (p - 1) / 2;
+    u128 cap = 127, pw = 1;
+    uint32_t L = 1;
+    while (cap < half) {
+        pw *= 256;
+        cap += 127 * pw;
+        L++;
+    }
+    return L;
+}
+
+LimbParams make_limb_params(uint64_t p) {
+    typedef unsigned __int128 u128;
+    LimbParams lp{};
+    lp.L = limb_count(p);
+    u128 x = 1 % p;
+    for (int l = 0; l < 8; l++) {
+        lp.pow256[l] = (uint64_t)x;
+        x = (x * 256) % p;
+    }
+    const u128 big = ((u128)1) << 85;
+    const u128 pbig = (big / p + 1) * p;  // >= 2^85
+    lp.pbig_hi = (uint64_t)(pbig >> 64);
+    lp.pbig_lo = (uint64_t)pbig;
+    return lp;
+}
 
 FieldParams make_params(uint64_t p) {
     typedef unsigned __int128 u128;
@@ -144,9 +178,14 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     uint32_t b = (opts && opts->panel) ? opts->panel : (uint32_t)BMAX;
     if (b > (uint32_t)BMAX) b = BMAX;
     const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
+    const uint32_t method = opts ? opts->method : (uint32_t)MODLIN_METHOD_AUTO;
+    const bool use_mfma = method != MODLIN_METHOD_PLAIN;
+    const LimbParams lp = make_limb_params(p);
+    const size_t KA = (size_t)lp.L * KSTEP;
 
     // Workspace layout (256-byte aligned regions).
-    std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offS(count);
+    std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offS(count), offAl(count),
+        offXl(count);
     size_t cur = 0;
     const size_t A = 256;
     cur = align_up(cur + count * sizeof(RankMat), A);
@@ -179,6 +218,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offZ[i] = cur; cur = align_up(cur + BMAX * BMAX * 8, A);
         offS[i] = 0;
         if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
+        offAl[i] = offXl[i] = 0;
+        if (use_mfma) {
+            const size_t npad = (n + UT - 1) / UT * UT;
+            offAl[i] = cur; cur = align_up(cur + npad * KA, A);
+            offXl[i] = cur; cur = align_up(cur + (size_t)lp.L * npad * KA, A);
+        }
     }
     int rc = arena_reserve(ctx, cur);
     if (rc) return rc;
@@ -196,6 +241,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.Ab = (uint64_t *)(base + offA[i]);
         S.X = (uint64_t *)(base + offX[i]);
         S.Z = (uint64_t *)(base + offZ[i]);
+        S.Al = use_mfma ? (int8_t *)(base + offAl[i]) : nullptr;
+        S.Xl = use_mfma ? (int8_t *)(base + offXl[i]) : nullptr;
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
         S.done = (n == 0) ? 1u : 0u;
@@ -240,8 +287,13 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     uint64_t iters = 0;
     while (h_active > 0) {
         for (int s = 0; s < check_every; s++) {
-            launch_rank_iteration(st, dmats, nmat, f, b, gx_off, upd_off, plan, dactive, grid_gx, grid_upd,
-                                  profile ? evs[2 * s] : nullptr, profile ? evs[2 * s + 1] : nullptr);
+            hipEvent_t e0 = profile ? evs[2 * s] : nullptr, e1 = profile ? evs[2 * s + 1] : nullptr;
+            if (use_mfma)
+                launch_rank_iteration_mfma(st, dmats, nmat, f, lp, b, gx_off, upd_off, plan, dactive, grid_gx,
+                                           grid_upd, e0, e1);
+            else
+                launch_rank_iteration(st, dmats, nmat, f, b, gx_off, upd_off, plan, dactive, grid_gx, grid_upd, e0,
+                                      e1);
         }
         MDL_HIP(hipGetLastError());
         iters += check_every;
@@ -260,15 +312,18 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     for (auto &e : evs) hipEventDestroy(e);
     rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
     if (rc) return rc;
-    double macs = 0;
+    double macs = 0, ubytes = 0;
     for (uint64_t i = 0; i < count; i++) {
         ranks[i] = hm[i].rank;
         macs += (double)hm[i].macs;
+        ubytes += (double)hm[i].upd_bytes;
     }
     ctx->stats.macs = macs;
+    ctx->stats.update_bytes = ubytes;
     ctx->stats.update_ms = upd_ms;
     ctx->stats.update_launches = upd_launches;
     ctx->stats.iterations = (double)iters;
+    ctx->stats.limb_count = use_mfma ? (double)lp.L : 0.0;
     ctx->stats.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return MODLIN_OK;

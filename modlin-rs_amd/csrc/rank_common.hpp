@@ -34,7 +34,7 @@ struct RankMat {
     uint32_t J[BMAX];        // pivot columns of the panel (absolute)
     uint32_t R[BMAX];        // physical pivot rows, in pivot order
     unsigned long long macs; // F_p multiply-accumulates of the trailing updates
-    unsigned long long upd_launch_macs;  // MACs of this iteration's update
+    unsigned long long upd_bytes;        // algorithmic HBM bytes of the trailing updates
 };
 
 // Work plan of one iteration: per-matrix tile offsets (prefix sums).
@@ -42,5 +42,38 @@ struct RankPlan {
     uint32_t gx_total, upd_total;
     uint32_t pad0, pad1;
 };
+
+// Constants of the signed-digit limb representation used by the MFMA update.
+// An element x in [0, p) is first mapped to its balanced residue
+// x^ = x - p [x > (p-1)/2], |x^| <= (p-1)/2, then written with L signed 8-bit
+// digits: x^ = sum_l d_l 256^l, d_l in [-128, 127].
+struct LimbParams {
+    uint64_t pow256[8];       // 256^l mod p
+    uint64_t pbig_hi, pbig_lo; // a multiple of p >= 2^85 (keeps C - v positive)
+    uint32_t L;                // digits per element
+    uint32_t pad;
+};
+
+constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task
+constexpr uint32_t XCOLS = 64;         // columns per X task
+constexpr uint32_t UT = 64;            // trailing-update tile edge
+constexpr uint32_t KSTEP = 64;         // int8 K per MFMA step (one digit plane of a panel)
+
+__device__ __forceinline__ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ uint32_t find_mat(const uint32_t *off, uint32_t nmat, uint32_t t) {
+    // largest i with off[i] <= t (off is non-decreasing, off[nmat] > t)
+    uint32_t lo = 0, hi = nmat;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Balanced signed digits of x in [0, p).
+__device__ __forceinline__ int64_t balanced(uint64_t x, uint64_t p) {
+    return x > (p - 1) / 2 ? (int64_t)x - (int64_t)p : (int64_t)x;
+}
 
 }  // namespace mdl

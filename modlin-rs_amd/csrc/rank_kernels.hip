@@ -147,11 +147,6 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // ---------------------------------------------------------------------------
 // k_plan: tile counts and prefix sums (single workgroup)
 // ---------------------------------------------------------------------------
-constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task
-constexpr uint32_t XCOLS = 64;         // columns per X task
-constexpr uint32_t UT = 64;            // trailing-update tile edge
-
-__device__ __forceinline__ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 __global__ void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
     if (threadIdx.x != 0) return;
@@ -173,16 +168,6 @@ __global__ void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t 
     plan->upd_total = u;
 }
 
-__device__ __forceinline__ uint32_t find_mat(const uint32_t *off, uint32_t nmat, uint32_t t) {
-    // largest i with off[i] <= t (off is non-decreasing, off[nmat] > t)
-    uint32_t lo = 0, hi = nmat;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= t) lo = mid; else hi = mid;
-    }
-    while (lo + 1 < nmat && off[lo + 1] <= t) lo++;
-    return lo;
-}
 
 // ---------------------------------------------------------------------------
 // k_gather_x
@@ -327,7 +312,10 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
             }
         }
         if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
-        if (tid == 0 && pb > 0) atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
+        if (tid == 0 && pb > 0) {
+            atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
+            atomicAdd(&S.upd_bytes, 16ull * nr * nc + 8ull * pb * (nr + nc));
+        }
     }
 }
 
@@ -369,16 +357,25 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 // ---------------------------------------------------------------------------
 // Host launchers (one panel iteration of every matrix in the batch)
 // ---------------------------------------------------------------------------
+void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
+                       uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
+    hipLaunchKernelGGL(k_block_ge, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, dmats, nmat, gx_off, upd_off, plan);
+}
+
+void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active) {
+    hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active);
+}
+
 void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
                            uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
                            uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1) {
-    hipLaunchKernelGGL(k_block_ge, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, dmats, nmat, gx_off, upd_off, plan);
+    launch_block_plan(st, dmats, nmat, f, b, gx_off, upd_off, plan);
     hipLaunchKernelGGL(k_gather_x, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f);
     if (ev_upd0) hipEventRecord(ev_upd0, st);
     hipLaunchKernelGGL(k_update_valu, dim3(grid_upd), dim3(256), 0, st, dmats, nmat, upd_off, plan, f);
     if (ev_upd1) hipEventRecord(ev_upd1, st);
-    hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active);
+    launch_finish(st, dmats, nmat, active);
 }
 
 }  // namespace mdl

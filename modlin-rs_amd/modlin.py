@@ -64,7 +64,7 @@ class _Opts(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("macs", "update_ms", "update_launches", "limb_count", "iterations", "bm_bytes", "bm_ms",
-                 "total_ms")]
+                 "total_ms", "update_bytes")]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)

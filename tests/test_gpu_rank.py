@@ -16,7 +16,7 @@ from conftest import modlin, oracle, workloads
 
 pytestmark = pytest.mark.gpu
 M61 = (1 << 61) - 1
-METHODS = [modlin.METHOD_PLAIN]
+METHODS = [modlin.METHOD_PLAIN, modlin.METHOD_MFMA]
 
 
 def _rank_all(ctx, p, mats, method=modlin.METHOD_PLAIN, panel=0):
@@ -69,18 +69,25 @@ def _hidden_pivot_cases(p, rng):
     return cases
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("p", [2, 97, M61])
-def test_hidden_pivots_and_sparse(ctx, p):
+def test_hidden_pivots_and_sparse(ctx, p, method):
     rng = np.random.default_rng(p % 1000)
     cases = _hidden_pivot_cases(p, rng)
-    got = _rank_all(ctx, p, cases)
+    got = _rank_all(ctx, p, cases, method)
     for a, g in zip(cases, got):
         assert g == oracle.rank_ref(p, a, a.shape[0])
 
 
-@pytest.mark.parametrize("p", [2, 3, 127, 131, 32749, 65521, 8388593, (1 << 31) - 1, M61, (1 << 62) + 135,
-                               (1 << 63) - 25])
-def test_prime_classes(ctx, p):
+# one or more primes for every limb count L = 1..8 of the MFMA update, and both
+# Barrett classes (bit length <= 62 and = 63)
+PRIMES = [2, 3, 127, 131, 251, 257, 32749, 32771, 65521, 8388593, (1 << 31) - 1, 68719476731, 17592186044399,
+          4503599627370449, M61, (1 << 62) + 135, (1 << 63) - 25]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("p", PRIMES)
+def test_prime_classes(ctx, p, method):
     rng = np.random.default_rng(7)
     mats = []
     for n in [1, 2, 63, 64, 65, 127, 128, 129, 200]:
@@ -89,35 +96,42 @@ def test_prime_classes(ctx, p):
         b = rng.integers(0, p, size=(r, n), dtype=np.uint64)
         cf = rng.integers(0, p, size=(n, r), dtype=np.uint64)
         mats.append(modlin.combine_rows(p, b, cf))          # rank <= n/3
-    got = _rank_all(ctx, p, mats)
+    # extreme residues: 0, 1, p-1 and (p-1)/2, (p+1)/2 (the balanced-digit edges)
+    edge = np.array([0, 1, p - 1, (p - 1) // 2, (p + 1) // 2 % p], dtype=np.uint64)
+    mats.append(edge[rng.integers(0, 5, size=(130, 130))])
+    mats.append(np.full((70, 70), p - 1, dtype=np.uint64) - np.eye(70, dtype=np.uint64) * (p > 2))
+    got = _rank_all(ctx, p, mats, method)
     for a, g in zip(mats, got):
         assert g == oracle.rank(p, a, a.shape[0]), (p, a.shape)
 
 
-def test_panel_widths(ctx):
+@pytest.mark.parametrize("method", METHODS)
+def test_panel_widths(ctx, method):
     rng = np.random.default_rng(1)
     p = M61
     mats = [rng.integers(0, p, size=(n, n), dtype=np.uint64) for n in [50, 100, 150]]
     mats.append(workloads.c3(300, p))
     want = [oracle.rank(p, a, a.shape[0]) for a in mats]
     for b in [1, 2, 7, 16, 32, 64]:
-        assert _rank_all(ctx, p, mats, panel=b) == want, b
+        assert _rank_all(ctx, p, mats, method, panel=b) == want, b
 
 
-def test_u32_inputs(ctx):
+@pytest.mark.parametrize("method", METHODS)
+def test_u32_inputs(ctx, method):
     p = 65521
     mats = workloads.c5_rank(count=4, n=300, p=p)
-    got = _rank_all(ctx, p, mats)
+    got = _rank_all(ctx, p, mats, method)
     for a, g in zip(mats, got):
         assert g == oracle.rank(p, a.astype(np.uint64), 300)
     # a u32 rank-deficient matrix
     b = modlin.combine_rows(p, mats[0][:10].astype(np.uint64), mats[1][:, :10].astype(np.uint64))
-    assert _rank_all(ctx, p, [b.astype(np.uint32)]) == [10]
+    assert _rank_all(ctx, p, [b.astype(np.uint32)], method) == [10]
 
 
-def test_mixed_batch_c2_subset(ctx):
+@pytest.mark.parametrize("method", METHODS)
+def test_mixed_batch_c2_subset(ctx, method):
     mats = workloads.c2(count=16, max_side=700)
-    got = _rank_all(ctx, M61, mats)
+    got = _rank_all(ctx, M61, mats, method)
     for i, (a, g) in enumerate(zip(mats, got)):
         assert g == oracle.rank(M61, a, a.shape[0]), i
 
@@ -129,9 +143,17 @@ def test_empty_and_tiny(ctx):
     assert ctx.rank_batch(f, [np.array([5], dtype=np.uint64), np.array([0], dtype=np.uint64)]) == [1, 0]
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("p", [M61, (1 << 31) - 1])
-def test_planted_corank_large(ctx, p):
+def test_planted_corank_large(ctx, p, method):
     n = 4096
     a = workloads.c3(n, p)
-    print(f"planted n={n} p={p}", flush=True)
-    assert _rank_all(ctx, p, [a]) == [n - 64]
+    print(f"planted n={n} p={p} method={method}", flush=True)
+    assert _rank_all(ctx, p, [a], method) == [n - 64]
+
+
+def test_readme_c1_both_methods(ctx):
+    mm, ctl = workloads.c1()
+    for method in METHODS:
+        r = _rank_all(ctx, M61, [mm, ctl], method)
+        assert [500 - r[0], 500 - r[1]] == [432, 0]
