@@ -40,14 +40,14 @@ LimbParams make_limb_params(uint64_t p) {
     LimbParams lp{};
     lp.L = limb_count(p);
     u128 x = 1 % p;
-    for (int l = 0; l < 8; l++) {
+    for (int l = 0; l < 16; l++) {
         lp.pow256[l] = (uint64_t)x;
         x = (x * 256) % p;
     }
-    const u128 big = ((u128)1) << 85;
-    const u128 pbig = (big / p + 1) * p;  // >= 2^85
-    lp.pbig_hi = (uint64_t)(pbig >> 64);
-    lp.pbig_lo = (uint64_t)pbig;
+    const int nc = 2 * (int)lp.L - 1;
+    u128 k0 = 0;
+    for (int q = 0; 4 * q < nc; q++) k0 = (k0 + lp.pow256[4 * q]) % p;  // sum of 2^(32 q) mod p
+    lp.k0 = (uint64_t)((k0 * ((((u128)1) << 49) % p)) % p);
     return lp;
 }
 
@@ -70,6 +70,7 @@ FieldParams make_params(uint64_t p) {
     if (sb > (u128)UINT64_MAX) sb = UINT64_MAX;
     if (sb < 1) sb = 1;
     f.safe_batch = (uint64_t)sb;
+    f.mode = p == P61 ? MODE_M61 : (p == P31 ? MODE_M31 : MODE_GENERIC);
     return f;
 }
 
@@ -222,7 +223,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         if (use_mfma) {
             const size_t npad = (n + UT - 1) / UT * UT;
             offAl[i] = cur; cur = align_up(cur + npad * KA, A);
-            offXl[i] = cur; cur = align_up(cur + (size_t)lp.L * npad * KA, A);
+            offXl[i] = cur; cur = align_up(cur + npad * KA, A);
         }
     }
     int rc = arena_reserve(ctx, cur);

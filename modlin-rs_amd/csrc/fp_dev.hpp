@@ -24,6 +24,13 @@
 
 namespace mdl {
 
+// Reduction mode: generic Barrett, or shift-and-add folding for the Mersenne
+// primes 2^61 - 1 and 2^31 - 1 (2^61 == 1, resp. 2^31 == 1).  All modes return
+// the same canonical residues.
+enum : uint32_t { MODE_GENERIC = 0, MODE_M61 = 1, MODE_M31 = 2 };
+constexpr uint64_t P61 = 2305843009213693951ull;  // 2^61 - 1
+constexpr uint64_t P31 = 2147483647ull;           // 2^31 - 1
+
 struct FieldParams {
     uint64_t p;
     uint64_t mu_hi, mu_lo;  // floor((2^128 - 1) / p)                     fp.rs:52
@@ -32,7 +39,26 @@ struct FieldParams {
     uint64_t safe_batch;    // products of (p-1)^2 that fit in 128 bits    fp.rs:53-59
     uint32_t k;             // bit length of p
     uint32_t big;           // k == 63
+    uint32_t mode;          // MODE_*
+    uint32_t pad;
 };
+
+// t = th 2^64 + tl mod 2^61 - 1 (any 128-bit t): 2^64 == 8.
+MDL_HD uint64_t fold_m61(uint64_t th, uint64_t tl) {
+    uint64_t x = (tl & P61) + (tl >> 61) + ((th << 3) & P61) + (th >> 58);  // < 2^63
+    x = (x & P61) + (x >> 61);                                               // <= p + 3
+    return x >= P61 ? x - P61 : x;
+}
+
+// t = th 2^64 + tl mod 2^31 - 1 (any 128-bit t): 2^31 == 1, 2^64 == 4.
+MDL_HD uint64_t fold_m31(uint64_t th, uint64_t tl) {
+    const uint64_t a = (tl & P31) + ((tl >> 31) & P31) + (tl >> 62);  // == tl, < 2^33
+    const uint64_t b = (th & P31) + ((th >> 31) & P31) + (th >> 62);  // == th, < 2^33
+    uint64_t x = a + 4 * b;                                             // < 2^36
+    x = (x & P31) + (x >> 31);                                          // < 2^31 + 32
+    x = (x & P31) + (x >> 31);                                          // <= p + 1
+    return x >= P31 ? x - P31 : x;
+}
 
 MDL_HD uint64_t umulhi64(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -65,6 +91,8 @@ MDL_HD uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t p) {  // fp.rs:88-90
 
 // t mod p for any 128-bit t (fp.rs:71-79).
 MDL_HD uint64_t reduce128(const FieldParams &f, uint64_t th, uint64_t tl) {
+    if (f.mode == MODE_M61) return fold_m61(th, tl);
+    if (f.mode == MODE_M31) return fold_m31(th, tl);
     // q = high 128 bits of t * mu  (fp.rs:12-24 with four 64x64 partials)
     uint64_t a1 = th, a0 = tl, b1 = f.mu_hi, b0 = f.mu_lo;
     uint64_t lo_hi = umulhi64(a0, b0);
@@ -99,6 +127,8 @@ MDL_HD uint64_t reduce128(const FieldParams &f, uint64_t th, uint64_t tl) {
 
 // t mod p for t < 2^2k (in particular t = a b with a, b < p).
 MDL_HD uint64_t reduce_prod(const FieldParams &f, uint64_t th, uint64_t tl) {
+    if (f.mode == MODE_M61) return fold_m61(th, tl);
+    if (f.mode == MODE_M31) return fold_m31(th, tl);
     const uint32_t k = f.k;
     uint64_t q1 = (th << (65 - k)) | (tl >> (k - 1));
     uint64_t q3 = umulhi64(q1, f.mup);
@@ -128,6 +158,14 @@ MDL_HD uint64_t mul_mod(const FieldParams &f, uint64_t a, uint64_t b) {  // fp.r
 
 // (pi x - e y) mod p, the fraction-free elimination step.
 MDL_HD uint64_t ff_step(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
+    if (f.mode != MODE_GENERIC) {
+        // pi x + (p - e) y < 2 p^2 < 2^128: one folding reduction
+        uint64_t h, l, h2, l2;
+        mul_wide(pi, x, h, l);
+        mul_wide(f.p - e, y, h2, l2);
+        add128(h, l, h2, l2);
+        return f.mode == MODE_M61 ? fold_m61(h, l) : fold_m31(h, l);
+    }
     return sub_mod(mul_mod(f, pi, x), mul_mod(f, e, y), f.p);
 }
 

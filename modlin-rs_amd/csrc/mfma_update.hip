@@ -3,26 +3,25 @@
 // The only dense contraction of the rank (fp.rs:202-248).  Operands are exact
 // integers below p < 2^63; the product is computed exactly with int8 MFMA:
 //
-//   * every element is written in balanced signed digits, x^ = x - p [x > (p-1)/2]
-//     = sum_l d_l 256^l with d_l in [-128, 127] and L = limbs(p) digits
-//     (1 for p < 2^8, 3 for 65521, 4 for 2^31-1, 8 for 2^61-1);
-//   * the digit shifts of A are folded into X:  A X = sum_la 256^la A_la X
-//     == sum_la A_la X'_la (mod p) with X'_la = 256^la X mod p, and X'_la is
-//     itself split into digits D_{la,lb}.  Hence
-//         A X == sum_lb 256^lb P_lb (mod p),  P_lb = sum_la A_la D_{la,lb},
-//     i.e. L GEMMs with K = L * 64 that share the A operand (A digits of all la
-//     are laid out along K).  |P_lb| <= L * 64 * 2^14 < 2^23: exact in int32;
-//   * the epilogue forms v = sum_lb 256^lb P_lb (|v| < 2^81), adds a multiple of
-//     p >= 2^85 and reduces the positive 128-bit value with the reference's
-//     Barrett reduction (fp.rs:71-79).  Every result is the canonical residue, so
-//     the update equals the VALU path and the CPU reference bit for bit.
+//   * every element x in [0, p) is written in balanced signed digits,
+//     x^ = x - p [x > (p-1)/2] = sum_l d_l 256^l, d_l in [-128, 127], with
+//     L = limbs(p) digits (1 for p < 2^8, 3 for 65521, 4 for 2^31-1, 8 for 2^61-1);
+//   * A X = sum_s 256^s P_s with P_s = sum_{la+lb=s} A_la X_lb, s < 2L-1: each
+//     digit pair is one int8 GEMM with K = 64 (the panel), accumulated exactly in
+//     int32 per shift class (|P_s| <= 8 * 64 * 2^14 = 2^23);
+//   * the epilogue folds the classes into four signed 49-bit words
+//     w_i = sum_{s in [4i, 4i+4)} P_s 256^(s-4i), maps them to residues with the
+//     constants 2^(32 i) mod p, and reduces once with the reference's Barrett
+//     reduction (fp.rs:71-79).  Every result is the canonical residue, so the
+//     update equals the VALU path and the CPU reference bit for bit.
 //
-// k_gather_split  produces the digit planes: A (rows O x L*64, K-contiguous) and
-//                 X' (L planes of columns x L*64, K-contiguous), fusing the X
-//                 product (the reference's U solve) with the digit split.
-// k_update_mfma   64 x 64 output tiles per workgroup, 4 waves of 32 x 32, built
-//                 from v_mfma_i32_16x16x64_i8 (lane l holds row / column l % 16 and
-//                 the 16 contiguous K bytes 16 (l / 16) .. +16 of a 64-deep step).
+// k_gather_split  digit planes of A = M[O, J] (rows x L*64, K-contiguous) and of
+//                 X = Z M[R, c0:] (columns x L*64, K-contiguous), fusing the X
+//                 product (the reference's U-block solve) with the digit split.
+// k_update_mfma   one 64 x 64 output tile per workgroup iteration: both digit
+//                 tiles are staged in LDS with 16-byte loads, 8 waves compute
+//                 16 x 32 each with v_mfma_i32_16x16x64_i8 (lane l holds row or
+//                 column l % 16 and K bytes 16 (l / 16) .. +16 of a 64-byte step).
 #include "rank_common.hpp"
 
 namespace mdl {
@@ -31,7 +30,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 template <int L>
 __global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
-                                                      const RankPlan *plan, FieldParams f, LimbParams lp) {
+                                                      const RankPlan *plan, FieldParams f) {
     __shared__ uint64_t Zs[BMAX][BMAX + 1];
     __shared__ uint64_t Ms[BMAX][XCOLS + 1];
     constexpr uint32_t KA = L * KSTEP;
@@ -62,7 +61,7 @@ __global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nm
                 }
             }
         } else {
-            // X'[la][k][c] digits from X[k][c] = sum_s Z[k][s] M[R[s]][c0 + c]
+            // X digits: Xl[c][lb*64 + k] = digit lb of X[k][c], X[k][c] = sum_s Z[k][s] M[R[s]][c0 + c]
             const uint32_t cb = (lt - ng) * XCOLS;
             const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
             __syncthreads();
@@ -73,7 +72,6 @@ __global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nm
             }
             __syncthreads();
             const uint64_t sb = f.safe_batch;
-            const size_t plane = (size_t)cdiv(S.n, UT) * UT * KA;
             for (uint32_t o = tid; o < pb * cols; o += blockDim.x) {
                 const uint32_t c = o / pb, kk = o - c * pb;
                 uint64_t ah = 0, al = 0, tot = 0, cnt = 0;
@@ -88,17 +86,13 @@ __global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nm
                     }
                 }
                 if (cnt) tot = add_mod(tot, reduce128(f, ah, al), p);
+                int64_t v = balanced(tot, p);
                 int8_t *dst = S.Xl + (size_t)(cb + c) * KA + kk;
 #pragma unroll
-                for (int la = 0; la < L; la++) {
-                    const uint64_t xl = la ? mul_mod(f, tot, lp.pow256[la]) : tot;
-                    int64_t v = balanced(xl, p);
-#pragma unroll
-                    for (int lb = 0; lb < L; lb++) {
-                        const int8_t d = (int8_t)(v & 0xff);
-                        dst[lb * plane + la * KSTEP] = d;
-                        v = (v - d) >> 8;
-                    }
+                for (int lb = 0; lb < L; lb++) {
+                    const int8_t d = (int8_t)(v & 0xff);
+                    dst[lb * KSTEP] = d;
+                    v = (v - d) >> 8;
                 }
             }
             __syncthreads();
@@ -106,98 +100,207 @@ __global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nm
     }
 }
 
+// 128-bit product of u (< 2^50) and c (< 2^63), accumulated into (h, l).
+__device__ __forceinline__ void mac128(uint64_t &h, uint64_t &l, uint64_t u, uint64_t c) {
+    uint64_t ph, pl;
+    mul_wide(u, c, ph, pl);
+    add128(h, l, ph, pl);
+}
+
+// Geometry of one 64 x 64 update tile.
+struct TileMeta {
+    uint32_t mi, r0, cb, nr, nc, pb;
+};
+
+__device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, const uint32_t *upd_off, uint32_t t) {
+    TileMeta m;
+    m.mi = find_mat(upd_off, nmat, t);
+    const RankMat &S = mats[m.mi];
+    const uint32_t lt = t - upd_off[m.mi];
+    const uint32_t O = S.n - S.rank;
+    const uint32_t cols = S.cend - S.c0;
+    const uint32_t nct = cdiv(cols, UT);
+    const uint32_t tr = lt / nct, tc = lt - tr * nct;
+    m.r0 = tr * UT;
+    m.cb = tc * UT;
+    m.nr = min(UT, O - m.r0);
+    m.nc = min(UT, cols - m.cb);
+    m.pb = S.pb;
+    return m;
+}
+
+// sum_i w_i 2^(32 i) mod p for signed |w_i| < 2^48 (i < NW), returned as the
+// residue of sum_i (w_i + 2^49) 2^(32 i); the caller subtracts lp.k0.
+template <int NW>
+__device__ __forceinline__ uint64_t fold_words(const FieldParams &f, const LimbParams &lp, const int64_t *w) {
+    constexpr uint64_t OFF = 1ull << 49;
+    if (f.mode == MODE_M61) {
+        // 2^32, 2^64 == 8, 2^96 == 2^35 (mod 2^61 - 1)
+        uint64_t s = (uint64_t)(w[0] + (int64_t)OFF);
+        if (NW > 1) {
+            const uint64_t u = (uint64_t)(w[1] + (int64_t)OFF);
+            s += ((u & ((1ull << 29) - 1)) << 32) + (u >> 29);
+        }
+        if (NW > 2) s += (uint64_t)(w[2] + (int64_t)OFF) << 3;
+        if (NW > 3) {
+            const uint64_t u = (uint64_t)(w[3] + (int64_t)OFF);
+            s += ((u & ((1ull << 26) - 1)) << 35) + (u >> 26);
+        }
+        return fold_m61(0, s);
+    }
+    if (f.mode == MODE_M31) {
+        // 2^(32 i) == 2^i (mod 2^31 - 1)
+        uint64_t s = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) s += (uint64_t)(w[q] + (int64_t)OFF) << q;
+        return fold_m31(0, s);
+    }
+    uint64_t h = 0, l = (uint64_t)(w[0] + (int64_t)OFF);
+#pragma unroll
+    for (int q = 1; q < NW; q++) mac128(h, l, (uint64_t)(w[q] + (int64_t)OFF), lp.pow256[4 * q]);
+    // the sum is below 2^115: the single-word Barrett applies once 2k >= 116
+    return f.k >= 58 ? reduce_prod(f, h, l) : reduce128(f, h, l);
+}
+
 template <int L>
-__global__ __launch_bounds__(256) void k_update_mfma(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
+__global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
                                                      const RankPlan *plan, FieldParams f, LimbParams lp) {
     constexpr uint32_t KA = L * KSTEP;
+    constexpr uint32_t KAP = KA + 16;    // LDS row stride: consecutive rows start 4 banks apart
+    constexpr int NC = 2 * L - 1;        // shift classes
+    constexpr int NW = (NC + 3) / 4;     // 32-bit-aligned words of the recombined value
+    constexpr uint32_t CH = KA / 16;     // 16-byte chunks per digit row
+    constexpr uint32_t NCH = UT * CH;              // chunks per operand tile
+    constexpr uint32_t NV = (NCH + 511) / 512;     // chunks per thread per operand tile
+    __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = plan->upd_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
     const uint64_t p = f.p;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const uint32_t mi = find_mat(upd_off, nmat, t);
-        RankMat &S = mats[mi];
-        const uint32_t lt = t - upd_off[mi];
-        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
-        const uint32_t cols = S.cend - S.c0;
-        const uint32_t nct = cdiv(cols, UT);
-        const uint32_t tr = lt / nct, tc = lt - tr * nct;
-        const uint32_t r0 = tr * UT, cb = tc * UT;
-        const uint32_t nr = min(UT, O - r0), nc = min(UT, cols - cb);
+
+    // operand prefetch registers: the next tile's digit chunks
+    v4i ra[NV], rb[NV];
+    uint32_t t = blockIdx.x;
+    TileMeta cur;
+    if (t < total) {
+        cur = tile_meta(mats, nmat, upd_off, t);
+        if (cur.pb) {
+            const RankMat &S = mats[cur.mi];
+            const v4i *ga = (const v4i *)(S.Al + (size_t)cur.r0 * KA);
+            const v4i *gb = (const v4i *)(S.Xl + (size_t)cur.cb * KA);
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++) {
+                if (NCH % 512 == 0 || tid + j * 512 < NCH) {
+                    ra[j] = ga[tid + j * 512];
+                    rb[j] = gb[tid + j * 512];
+                }
+            }
+        }
+    }
+    for (; t < total; t += gridDim.x) {
+        RankMat &S = mats[cur.mi];
+        const uint32_t r0 = cur.r0, cb = cur.cb, nr = cur.nr, nc = cur.nc, pb = cur.pb, ld = S.ld;
+        const uint32_t rank = S.rank, c0 = S.c0, jend = S.jend;
         uint32_t nzcol = 0xffffffffu;
         if (pb == 0) {
             // detection only (no pivots): scan the tile for a nonzero panel column
             for (uint32_t e = tid; e < nr * nc; e += blockDim.x) {
                 const uint32_t i = e / nc, c = e - i * nc;
-                const uint32_t gcol = S.c0 + cb + c;
-                if (gcol < S.jend && S.M[(size_t)S.perm[S.rank + r0 + i] * ld + gcol] != 0) nzcol = min(nzcol, gcol);
+                const uint32_t gcol = c0 + cb + c;
+                if (gcol < jend && S.M[(size_t)S.perm[rank + r0 + i] * ld + gcol] != 0) nzcol = min(nzcol, gcol);
             }
-            if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
-            continue;
+        } else {
+            // stage this tile's digits (already in registers) in LDS
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++) {
+                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
+                if (NCH % 512 == 0 || v < NCH) {
+                    *(v4i *)(As + row * KAP + ch * 16) = ra[j];
+                    *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
+                }
+            }
+            __syncthreads();
         }
-        v4i acc[L][2][2];
+        // prefetch the next tile's digits while this one is computed
+        const uint32_t tn = t + gridDim.x;
+        TileMeta nxt;
+        if (tn < total) {
+            nxt = tile_meta(mats, nmat, upd_off, tn);
+            if (nxt.pb) {
+                const RankMat &Sn = mats[nxt.mi];
+                const v4i *ga = (const v4i *)(Sn.Al + (size_t)nxt.r0 * KA);
+                const v4i *gb = (const v4i *)(Sn.Xl + (size_t)nxt.cb * KA);
 #pragma unroll
-        for (int lb = 0; lb < L; lb++)
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) acc[lb][a][b] = v4i{0, 0, 0, 0};
-        const size_t plane = (size_t)cdiv(S.n, UT) * UT * KA;
-        const int8_t *Ap = S.Al + (size_t)(r0 + wr + (lane & 15)) * KA + 16 * (lane >> 4);
-        const int8_t *Bp = S.Xl + (size_t)(cb + wc + (lane & 15)) * KA + 16 * (lane >> 4);
-#pragma unroll
-        for (int s = 0; s < L; s++) {
-            const v4i a0 = *(const v4i *)(Ap + s * KSTEP);
-            const v4i a1 = *(const v4i *)(Ap + 16 * KA + s * KSTEP);
-#pragma unroll
-            for (int lb = 0; lb < L; lb++) {
-                const v4i b0 = *(const v4i *)(Bp + lb * plane + s * KSTEP);
-                const v4i b1 = *(const v4i *)(Bp + lb * plane + 16 * KA + s * KSTEP);
-                acc[lb][0][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b0, acc[lb][0][0], 0, 0, 0);
-                acc[lb][0][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b1, acc[lb][0][1], 0, 0, 0);
-                acc[lb][1][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b0, acc[lb][1][0], 0, 0, 0);
-                acc[lb][1][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b1, acc[lb][1][1], 0, 0, 0);
+                for (uint32_t j = 0; j < NV; j++) {
+                    if (NCH % 512 == 0 || tid + j * 512 < NCH) {
+                        ra[j] = ga[tid + j * 512];
+                        rb[j] = gb[tid + j * 512];
+                    }
+                }
             }
         }
-        // epilogue: C - sum_lb 256^lb P_lb, reduced
+        if (pb) {
+            // rows of this lane's outputs and their current values (loaded early)
+            uint64_t *mrow[4];
+            uint64_t cv[2][4];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+            for (int r = 0; r < 4; r++) {
+                const uint32_t i = wr + 4 * (lane >> 4) + r;
+                mrow[r] = (i < nr) ? S.M + (size_t)S.perm[rank + r0 + i] * ld + c0 + cb : nullptr;
+            }
 #pragma unroll
             for (int b = 0; b < 2; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const uint32_t i = wr + 16 * a + 4 * (lane >> 4) + r;
                     const uint32_t c = wc + 16 * b + (lane & 15);
-                    if (i >= nr || c >= nc) continue;
-                    int64_t lo = 0, hi = 0;
-#pragma unroll
-                    for (int lb = 0; lb < L; lb++) {
-                        const int64_t pv = (int64_t)acc[lb][a][b][r];
-                        if (lb < 4) lo += pv << (8 * lb);
-                        else hi += pv << (8 * (lb - 4));
-                    }
-                    // V = hi * 2^32 + lo as a two's-complement 128-bit value
-                    const uint64_t hl = (uint64_t)hi << 32, hh = (uint64_t)(hi >> 32);
-                    uint64_t vl = hl + (uint64_t)lo;
-                    uint64_t vh = hh + (lo < 0 ? ~0ull : 0ull) + (vl < hl ? 1ull : 0ull);
-                    uint64_t *mp = S.M + (size_t)S.perm[S.rank + r0 + i] * ld + S.c0 + cb + c;
-                    const uint64_t cv = *mp;
-                    // t = pbig + C - V  (positive, below 2^87)
-                    uint64_t tl = lp.pbig_lo + cv;
-                    uint64_t th = lp.pbig_hi + (tl < cv ? 1ull : 0ull);
-                    const uint64_t nl = tl - vl;
-                    th = th - vh - (tl < vl ? 1ull : 0ull);
-                    tl = nl;
-                    const uint64_t res = reduce128(f, th, tl);
-                    *mp = res;
-                    const uint32_t gcol = S.c0 + cb + c;
-                    if (res != 0 && gcol < S.jend) nzcol = min(nzcol, gcol);
+                    cv[b][r] = (mrow[r] && c < nc) ? mrow[r][c] : 0;
                 }
-        if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
-        if (tid == 0) {
-            atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
-            atomicAdd(&S.upd_bytes, 16ull * nr * nc + 8ull * pb * (nr + nc));
+            v4i acc[NC][2];
+#pragma unroll
+            for (int s = 0; s < NC; s++) acc[s][0] = acc[s][1] = v4i{0, 0, 0, 0};
+            v4i bf[L][2];
+            const uint32_t koff = 16 * (lane >> 4);
+#pragma unroll
+            for (int lb = 0; lb < L; lb++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    bf[lb][b] = *(const v4i *)(Bs + (wc + 16 * b + (lane & 15)) * KAP + lb * KSTEP + koff);
+#pragma unroll
+            for (int la = 0; la < L; la++) {
+                const v4i af = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
+#pragma unroll
+                for (int lb = 0; lb < L; lb++) {
+                    acc[la + lb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][0], acc[la + lb][0], 0, 0, 0);
+                    acc[la + lb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][1], acc[la + lb][1], 0, 0, 0);
+                }
+            }
+            // epilogue: C - sum_s 256^s P_s (mod p)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t c = wc + 16 * b + (lane & 15);
+                    if (!mrow[r] || c >= nc) continue;
+                    int64_t w[NW];
+#pragma unroll
+                    for (int q = 0; q < NW; q++) w[q] = 0;
+#pragma unroll
+                    for (int s = 0; s < NC; s++) w[s >> 2] += (int64_t)acc[s][b][r] << (8 * (s & 3));
+                    const uint64_t vr = fold_words<NW>(f, lp, w);
+                    const uint64_t res = sub_mod(add_mod(cv[b][r], lp.k0, p), vr, p);
+                    mrow[r][c] = res;
+                    const uint32_t gcol = c0 + cb + c;
+                    if (res != 0 && gcol < jend) nzcol = min(nzcol, gcol);
+                }
+            if (tid == 0) {
+                atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
+                atomicAdd(&S.upd_bytes, 16ull * nr * nc + 8ull * pb * (nr + nc));
+            }
         }
+        if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
+        cur = nxt;
     }
 }
 
@@ -211,9 +314,9 @@ template <int L>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t grid_gx, uint32_t grid_upd,
                           hipEvent_t e0, hipEvent_t e1) {
-    hipLaunchKernelGGL(k_gather_split<L>, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f, lp);
+    hipLaunchKernelGGL(k_gather_split<L>, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f);
     if (e0) hipEventRecord(e0, st);
-    hipLaunchKernelGGL(k_update_mfma<L>, dim3(grid_upd), dim3(256), 0, st, dmats, nmat, upd_off, plan, f, lp);
+    hipLaunchKernelGGL(k_update_mfma<L>, dim3(grid_upd), dim3(512), 0, st, dmats, nmat, upd_off, plan, f, lp);
     if (e1) hipEventRecord(e1, st);
 }
 

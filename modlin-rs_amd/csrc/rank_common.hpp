@@ -48,9 +48,9 @@ struct RankPlan {
 // x^ = x - p [x > (p-1)/2], |x^| <= (p-1)/2, then written with L signed 8-bit
 // digits: x^ = sum_l d_l 256^l, d_l in [-128, 127].
 struct LimbParams {
-    uint64_t pow256[8];       // 256^l mod p
-    uint64_t pbig_hi, pbig_lo; // a multiple of p >= 2^85 (keeps C - v positive)
-    uint32_t L;                // digits per element
+    uint64_t pow256[16];  // 256^l mod p
+    uint64_t k0;          // 2^49 * sum_{i : 4i < 2L-1} 256^(4i) mod p (epilogue offset)
+    uint32_t L;           // digits per element
     uint32_t pad;
 };
 
