@@ -28,78 +28,6 @@ namespace mdl {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-template <int L>
-__global__ __launch_bounds__(256) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
-                                                      const RankPlan *plan, FieldParams f) {
-    __shared__ uint64_t Zs[BMAX][BMAX + 1];
-    __shared__ uint64_t Ms[BMAX][XCOLS + 1];
-    constexpr uint32_t KA = L * KSTEP;
-    const uint32_t total = plan->gx_total;
-    const uint32_t tid = threadIdx.x;
-    const uint64_t p = f.p;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        const uint32_t mi = find_mat(gx_off, nmat, t);
-        RankMat &S = mats[mi];
-        const uint32_t lt = t - gx_off[mi];
-        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
-        const uint32_t ng = cdiv(O, GATHER_ROWS);
-        if (lt < ng) {
-            // A digits: Al[i][la*64 + k] = digit la of M[perm[rank + i]][J[k]] (zero for k >= pb)
-            const uint32_t r0 = lt * GATHER_ROWS;
-            const uint32_t rows = min(GATHER_ROWS, O - r0);
-            for (uint32_t idx = tid; idx < rows * KSTEP; idx += blockDim.x) {
-                const uint32_t i = idx / KSTEP, kk = idx - i * KSTEP;
-                uint64_t x = 0;
-                if (kk < pb) x = S.M[(size_t)S.perm[S.rank + r0 + i] * ld + S.J[kk]];
-                int64_t v = balanced(x, p);
-                int8_t *dst = S.Al + (size_t)(r0 + i) * KA + kk;
-#pragma unroll
-                for (int la = 0; la < L; la++) {
-                    const int8_t d = (int8_t)(v & 0xff);
-                    dst[la * KSTEP] = d;
-                    v = (v - d) >> 8;
-                }
-            }
-        } else {
-            // X digits: Xl[c][lb*64 + k] = digit lb of X[k][c], X[k][c] = sum_s Z[k][s] M[R[s]][c0 + c]
-            const uint32_t cb = (lt - ng) * XCOLS;
-            const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
-            __syncthreads();
-            for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) Zs[idx / pb][idx % pb] = S.Z[(idx / pb) * BMAX + idx % pb];
-            for (uint32_t idx = tid; idx < pb * XCOLS; idx += blockDim.x) {
-                const uint32_t s = idx / XCOLS, c = idx - s * XCOLS;
-                Ms[s][c] = (c < cols) ? S.M[(size_t)S.R[s] * ld + S.c0 + cb + c] : 0;
-            }
-            __syncthreads();
-            const uint64_t sb = f.safe_batch;
-            for (uint32_t o = tid; o < pb * cols; o += blockDim.x) {
-                const uint32_t c = o / pb, kk = o - c * pb;
-                uint64_t ah = 0, al = 0, tot = 0, cnt = 0;
-                for (uint32_t s = 0; s < pb; s++) {
-                    uint64_t h, l;
-                    mul_wide(Zs[kk][s], Ms[s][c], h, l);
-                    add128(ah, al, h, l);
-                    if (++cnt == sb) {
-                        tot = add_mod(tot, reduce128(f, ah, al), p);
-                        ah = al = 0;
-                        cnt = 0;
-                    }
-                }
-                if (cnt) tot = add_mod(tot, reduce128(f, ah, al), p);
-                int64_t v = balanced(tot, p);
-                int8_t *dst = S.Xl + (size_t)(cb + c) * KA + kk;
-#pragma unroll
-                for (int lb = 0; lb < L; lb++) {
-                    const int8_t d = (int8_t)(v & 0xff);
-                    dst[lb * KSTEP] = d;
-                    v = (v - d) >> 8;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 // 128-bit product of u (< 2^50) and c (< 2^63), accumulated into (h, l).
 __device__ __forceinline__ void mac128(uint64_t &h, uint64_t &l, uint64_t u, uint64_t c) {
     uint64_t ph, pl;
@@ -160,6 +88,144 @@ __device__ __forceinline__ uint64_t fold_words(const FieldParams &f, const LimbP
     for (int q = 1; q < NW; q++) mac128(h, l, (uint64_t)(w[q] + (int64_t)OFF), lp.pow256[4 * q]);
     // the sum is below 2^115: the single-word Barrett applies once 2k >= 116
     return f.k >= 58 ? reduce_prod(f, h, l) : reduce128(f, h, l);
+}
+
+// Balanced signed digits of four consecutive elements, packed per digit plane:
+// out[l] holds digit l of x[0..3] in its bytes 0..3.
+template <int L>
+__device__ __forceinline__ void digits4(const uint64_t *x, uint64_t p, uint32_t *out) {
+    int64_t v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = balanced(x[j], p);
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int8_t d = (int8_t)(v[j] & 0xff);
+            w |= (uint32_t)(uint8_t)d << (8 * j);
+            v[j] = (v[j] - d) >> 8;
+        }
+        out[l] = w;
+    }
+}
+
+// One 64 x 64 x 64 limb product on the matrix cores: acc[s][b] += sum_{la+lb=s}
+// A_la B_lb for this wave's 16 x 32 block (rows wr.., columns wc..).  As / Bs hold
+// the digit rows (K-contiguous, L planes of 64 bytes, row stride KAP).
+template <int L, uint32_t KAP>
+__device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs, uint32_t lane, uint32_t wr,
+                                              uint32_t wc, v4i (*acc)[2]) {
+    v4i bf[L][2];
+    const uint32_t koff = 16 * (lane >> 4);
+#pragma unroll
+    for (int lb = 0; lb < L; lb++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) bf[lb][b] = *(const v4i *)(Bs + (wc + 16 * b + (lane & 15)) * KAP + lb * KSTEP + koff);
+#pragma unroll
+    for (int la = 0; la < L; la++) {
+        const v4i af = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
+#pragma unroll
+        for (int lb = 0; lb < L; lb++) {
+            acc[la + lb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][0], acc[la + lb][0], 0, 0, 0);
+            acc[la + lb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][1], acc[la + lb][1], 0, 0, 0);
+        }
+    }
+}
+
+// Digit planes of the two update operands.
+//  gather task: Al[i][la*64 + k] = digit la of M[perm[rank + i]][J[k]] (zero for k >= pb)
+//  X task:      Xl[c][lb*64 + k] = digit lb of X[k][c], X = Z M[R, c0:] mod p (the
+//               reference's U-block solve, fp.rs:180-200), computed as a limb product
+//               on the matrix cores: Z and the pivot rows are split into digits in LDS.
+template <int L>
+__global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
+                                                      const RankPlan *plan, FieldParams f, LimbParams lp) {
+    constexpr uint32_t KA = L * KSTEP;
+    constexpr uint32_t KAP = KA + 16;
+    constexpr int NC = 2 * L - 1, NW = (NC + 3) / 4;
+    __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
+    const uint32_t total = plan->gx_total;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
+    const uint64_t p = f.p;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint32_t mi = find_mat(gx_off, nmat, t);
+        RankMat &S = mats[mi];
+        const uint32_t lt = t - gx_off[mi];
+        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
+        const uint32_t ng = cdiv(O, GATHER_ROWS);
+        if (lt < ng) {
+            const uint32_t r0 = lt * GATHER_ROWS;
+            const uint32_t rows = min(GATHER_ROWS, O - r0);
+            for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
+                const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
+                const uint64_t *mrow = S.M + (size_t)S.perm[S.rank + r0 + i] * ld;
+                uint64_t x[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) x[j] = (k4 + j < pb) ? mrow[S.J[k4 + j]] : 0;
+                uint32_t w[L];
+                digits4<L>(x, p, w);
+                uint32_t *dst = (uint32_t *)(S.Al + (size_t)(r0 + i) * KA + k4);
+#pragma unroll
+                for (int la = 0; la < L; la++) dst[la * (KSTEP / 4)] = w[la];
+            }
+            continue;
+        }
+        const uint32_t cb = (lt - ng) * XCOLS;
+        const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
+        __syncthreads();
+        // As[k][la*64 + s] = digit la of Z[k][s];  Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
+        for (uint32_t idx = tid; idx < UT * (KSTEP / 4); idx += blockDim.x) {
+            const uint32_t k = idx / (KSTEP / 4), s4 = (idx - k * (KSTEP / 4)) * 4;
+            uint64_t x[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) x[j] = (k < pb && s4 + j < pb) ? S.Z[k * BMAX + s4 + j] : 0;
+            uint32_t w[L];
+            digits4<L>(x, p, w);
+#pragma unroll
+            for (int la = 0; la < L; la++) *(uint32_t *)(As + k * KAP + la * KSTEP + s4) = w[la];
+        }
+        for (uint32_t idx = tid; idx < XCOLS * (KSTEP / 4); idx += blockDim.x) {
+            const uint32_t c = idx % XCOLS, s4 = (idx / XCOLS) * 4;
+            uint64_t x[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)S.R[s4 + j] * ld + S.c0 + cb + c] : 0;
+            uint32_t w[L];
+            digits4<L>(x, p, w);
+#pragma unroll
+            for (int lb = 0; lb < L; lb++) *(uint32_t *)(Bs + c * KAP + lb * KSTEP + s4) = w[lb];
+        }
+        __syncthreads();
+        v4i acc[NC][2];
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc[q][0] = acc[q][1] = v4i{0, 0, 0, 0};
+        limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+        // X = sum_s 256^s P_s mod p, then its digits (4 consecutive k per lane and column)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const uint32_t c = wc + 16 * b + (lane & 15);
+            uint64_t x[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int64_t w[NW];
+#pragma unroll
+                for (int q = 0; q < NW; q++) w[q] = 0;
+#pragma unroll
+                for (int q = 0; q < NC; q++) w[q >> 2] += (int64_t)acc[q][b][r] << (8 * (q & 3));
+                x[r] = sub_mod(fold_words<NW>(f, lp, w), lp.k0, p);
+            }
+            if (c < cols) {
+                uint32_t dw[L];
+                digits4<L>(x, p, dw);
+                const uint32_t k4 = wr + 4 * (lane >> 4);
+#pragma unroll
+                for (int lb = 0; lb < L; lb++) *(uint32_t *)(S.Xl + (size_t)(cb + c) * KA + lb * KSTEP + k4) = dw[lb];
+            }
+        }
+    }
 }
 
 template <int L>
@@ -260,22 +326,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
             v4i acc[NC][2];
 #pragma unroll
             for (int s = 0; s < NC; s++) acc[s][0] = acc[s][1] = v4i{0, 0, 0, 0};
-            v4i bf[L][2];
-            const uint32_t koff = 16 * (lane >> 4);
-#pragma unroll
-            for (int lb = 0; lb < L; lb++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    bf[lb][b] = *(const v4i *)(Bs + (wc + 16 * b + (lane & 15)) * KAP + lb * KSTEP + koff);
-#pragma unroll
-            for (int la = 0; la < L; la++) {
-                const v4i af = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
-#pragma unroll
-                for (int lb = 0; lb < L; lb++) {
-                    acc[la + lb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][0], acc[la + lb][0], 0, 0, 0);
-                    acc[la + lb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][1], acc[la + lb][1], 0, 0, 0);
-                }
-            }
+            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
             // epilogue: C - sum_s 256^s P_s (mod p)
 #pragma unroll
             for (int b = 0; b < 2; b++)
@@ -314,7 +365,7 @@ template <int L>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t grid_gx, uint32_t grid_upd,
                           hipEvent_t e0, hipEvent_t e1) {
-    hipLaunchKernelGGL(k_gather_split<L>, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f);
+    hipLaunchKernelGGL(k_gather_split<L>, dim3(grid_gx), dim3(512), 0, st, dmats, nmat, gx_off, plan, f, lp);
     if (e0) hipEventRecord(e0, st);
     hipLaunchKernelGGL(k_update_mfma<L>, dim3(grid_upd), dim3(512), 0, st, dmats, nmat, upd_off, plan, f, lp);
     if (e1) hipEventRecord(e1, st);

@@ -148,26 +148,56 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // k_plan: tile counts and prefix sums (single workgroup)
 // ---------------------------------------------------------------------------
 
-__global__ void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
-    if (threadIdx.x != 0) return;
-    uint32_t g = 0, u = 0;
-    for (uint32_t i = 0; i < nmat; i++) {
-        gx_off[i] = g;
-        upd_off[i] = u;
-        const RankMat &S = mats[i];
-        if (S.done) continue;
-        const uint32_t O = S.n - S.rank;
-        const uint32_t cols = S.cend - S.c0;
-        if (O == 0 || cols == 0) continue;
-        if (S.pb > 0) g += cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
-        u += cdiv(O, UT) * cdiv(cols, UT);
+__global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *upd_off,
+                                              RankPlan *plan) {
+    __shared__ uint32_t sg[256], su[256];
+    __shared__ uint32_t base_g, base_u;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) base_g = base_u = 0;
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < nmat; i0 += 256) {
+        const uint32_t i = i0 + tid;
+        uint32_t g = 0, u = 0;
+        if (i < nmat) {
+            const RankMat &S = mats[i];
+            if (!S.done) {
+                const uint32_t O = S.n - S.rank;
+                const uint32_t cols = S.cend - S.c0;
+                if (O > 0 && cols > 0) {
+                    if (S.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
+                    u = cdiv(O, UT) * cdiv(cols, UT);
+                }
+            }
+        }
+        sg[tid] = g;
+        su[tid] = u;
+        __syncthreads();
+        // inclusive Hillis-Steele scan
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            const uint32_t ag = tid >= off ? sg[tid - off] : 0, au = tid >= off ? su[tid - off] : 0;
+            __syncthreads();
+            sg[tid] += ag;
+            su[tid] += au;
+            __syncthreads();
+        }
+        if (i < nmat) {
+            gx_off[i] = base_g + sg[tid] - g;
+            upd_off[i] = base_u + su[tid] - u;
+        }
+        __syncthreads();
+        if (tid == 255) {
+            base_g += sg[255];
+            base_u += su[255];
+        }
+        __syncthreads();
     }
-    gx_off[nmat] = g;
-    upd_off[nmat] = u;
-    plan->gx_total = g;
-    plan->upd_total = u;
+    if (tid == 0) {
+        gx_off[nmat] = base_g;
+        upd_off[nmat] = base_u;
+        plan->gx_total = base_g;
+        plan->upd_total = base_u;
+    }
 }
-
 
 // ---------------------------------------------------------------------------
 // k_gather_x
@@ -360,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
                        uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
     hipLaunchKernelGGL(k_block_ge, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, dmats, nmat, gx_off, upd_off, plan);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, gx_off, upd_off, plan);
 }
 
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active) {
