@@ -169,6 +169,27 @@ MDL_HD uint64_t ff_step(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t 
     return sub_mod(mul_mod(f, pi, x), mul_mod(f, e, y), f.p);
 }
 
+// Compile-time-mode variants (MODE = MODE_*; MODE_GENERIC defers to f.mode).
+template <int MODE>
+MDL_HD uint64_t mul_mod_t(const FieldParams &f, uint64_t a, uint64_t b) {
+    uint64_t h, l;
+    mul_wide(a, b, h, l);
+    if (MODE == MODE_M61) return fold_m61(h, l);
+    if (MODE == MODE_M31) return fold_m31(h, l);
+    return reduce_prod(f, h, l);
+}
+template <int MODE>
+MDL_HD uint64_t ff_step_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
+    if (MODE == MODE_M61 || MODE == MODE_M31) {
+        uint64_t h, l, h2, l2;
+        mul_wide(pi, x, h, l);
+        mul_wide(f.p - e, y, h2, l2);
+        add128(h, l, h2, l2);
+        return MODE == MODE_M61 ? fold_m61(h, l) : fold_m31(h, l);
+    }
+    return ff_step(f, pi, x, e, y);
+}
+
 MDL_HD uint64_t pow_mod(const FieldParams &f, uint64_t a, uint64_t e) {  // fp.rs:97-107
     uint64_t r = 1 % f.p;
     while (e) {

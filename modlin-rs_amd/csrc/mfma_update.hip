@@ -59,10 +59,10 @@ __device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, cons
 
 // sum_i w_i 2^(32 i) mod p for signed |w_i| < 2^48 (i < NW), returned as the
 // residue of sum_i (w_i + 2^49) 2^(32 i); the caller subtracts lp.k0.
-template <int NW>
+template <int MODE, int NW>
 __device__ __forceinline__ uint64_t fold_words(const FieldParams &f, const LimbParams &lp, const int64_t *w) {
     constexpr uint64_t OFF = 1ull << 49;
-    if (f.mode == MODE_M61) {
+    if (MODE == MODE_M61) {
         // 2^32, 2^64 == 8, 2^96 == 2^35 (mod 2^61 - 1)
         uint64_t s = (uint64_t)(w[0] + (int64_t)OFF);
         if (NW > 1) {
@@ -76,7 +76,7 @@ __device__ __forceinline__ uint64_t fold_words(const FieldParams &f, const LimbP
         }
         return fold_m61(0, s);
     }
-    if (f.mode == MODE_M31) {
+    if (MODE == MODE_M31) {
         // 2^(32 i) == 2^i (mod 2^31 - 1)
         uint64_t s = 0;
 #pragma unroll
@@ -138,7 +138,7 @@ __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs
 //  X task:      Xl[c][lb*64 + k] = digit lb of X[k][c], X = Z M[R, c0:] mod p (the
 //               reference's U-block solve, fp.rs:180-200), computed as a limb product
 //               on the matrix cores: Z and the pivot rows are split into digits in LDS.
-template <int L>
+template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
                                                       const RankPlan *plan, FieldParams f, LimbParams lp) {
     constexpr uint32_t KA = L * KSTEP;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 for (int q = 0; q < NW; q++) w[q] = 0;
 #pragma unroll
                 for (int q = 0; q < NC; q++) w[q >> 2] += (int64_t)acc[q][b][r] << (8 * (q & 3));
-                x[r] = sub_mod(fold_words<NW>(f, lp, w), lp.k0, p);
+                x[r] = sub_mod(fold_words<MODE, NW>(f, lp, w), lp.k0, p);
             }
             if (c < cols) {
                 uint32_t dw[L];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
     }
 }
 
-template <int L>
+template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
                                                      const RankPlan *plan, FieldParams f, LimbParams lp) {
     constexpr uint32_t KA = L * KSTEP;
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                     for (int q = 0; q < NW; q++) w[q] = 0;
 #pragma unroll
                     for (int s = 0; s < NC; s++) w[s >> 2] += (int64_t)acc[s][b][r] << (8 * (s & 3));
-                    const uint64_t vr = fold_words<NW>(f, lp, w);
+                    const uint64_t vr = fold_words<MODE, NW>(f, lp, w);
                     const uint64_t res = sub_mod(add_mod(cv[b][r], lp.k0, p), vr, p);
                     mrow[r][c] = res;
                     const uint32_t gcol = c0 + cb + c;
@@ -361,13 +361,13 @@ void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const Fiel
                        uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active);
 
-template <int L>
+template <int L, int MODE>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t grid_gx, uint32_t grid_upd,
                           hipEvent_t e0, hipEvent_t e1) {
-    hipLaunchKernelGGL(k_gather_split<L>, dim3(grid_gx), dim3(512), 0, st, dmats, nmat, gx_off, plan, f, lp);
+    hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(grid_gx), dim3(512), 0, st, dmats, nmat, gx_off, plan, f, lp);
     if (e0) hipEventRecord(e0, st);
-    hipLaunchKernelGGL(k_update_mfma<L>, dim3(grid_upd), dim3(512), 0, st, dmats, nmat, upd_off, plan, f, lp);
+    hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(grid_upd), dim3(512), 0, st, dmats, nmat, upd_off, plan, f, lp);
     if (e1) hipEventRecord(e1, st);
 }
 
@@ -376,16 +376,24 @@ void launch_rank_iteration_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, c
                                 RankPlan *plan, uint32_t *active, uint32_t grid_gx, uint32_t grid_upd,
                                 hipEvent_t e0, hipEvent_t e1) {
     launch_block_plan(st, dmats, nmat, f, b, gx_off, upd_off, plan);
-    switch (lp.L) {
-        case 1: launch_mfma_L<1>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 2: launch_mfma_L<2>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 3: launch_mfma_L<3>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 4: launch_mfma_L<4>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 5: launch_mfma_L<5>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 6: launch_mfma_L<6>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        case 7: launch_mfma_L<7>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
-        default: launch_mfma_L<8>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1); break;
+#define MDL_L(LL) launch_mfma_L<LL, MODE_GENERIC>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1)
+    if (f.mode == MODE_M61)
+        launch_mfma_L<8, MODE_M61>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1);
+    else if (f.mode == MODE_M31)
+        launch_mfma_L<4, MODE_M31>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1);
+    else {
+        switch (lp.L) {
+            case 1: MDL_L(1); break;
+            case 2: MDL_L(2); break;
+            case 3: MDL_L(3); break;
+            case 4: MDL_L(4); break;
+            case 5: MDL_L(5); break;
+            case 6: MDL_L(6); break;
+            case 7: MDL_L(7); break;
+            default: MDL_L(8); break;
+        }
     }
+#undef MDL_L
     launch_finish(st, dmats, nmat, active);
 }
 

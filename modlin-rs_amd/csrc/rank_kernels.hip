@@ -35,6 +35,7 @@ namespace mdl {
 // ---------------------------------------------------------------------------
 // k_block_ge
 // ---------------------------------------------------------------------------
+template <int MODE>
 __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b) {
     RankMat &S = mats[blockIdx.x];
     if (S.done) return;
@@ -93,12 +94,12 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         if (row < Bt && row != k) e = T[row][j];
         __syncthreads();  // every e read before any row changes
         if (row < Bt && row != k && e != 0) {
-            for (uint32_t c = j + 1 + sub; c < w; c += 16) T[row][c] = ff_step(f, pi, T[row][c], e, T[k][c]);
-            for (uint32_t s = sub; s < k; s += 16) E[row][s] = ff_step(f, pi, E[row][s], e, E[k][s]);
+            for (uint32_t c = j + 1 + sub; c < w; c += 16) T[row][c] = ff_step_t<MODE>(f, pi, T[row][c], e, T[k][c]);
+            for (uint32_t s = sub; s < k; s += 16) E[row][s] = ff_step_t<MODE>(f, pi, E[row][s], e, E[k][s]);
             if (sub == 0) {
-                E[row][k] = sub_mod(0, mul_mod(f, e, own[k]), p);
-                own[row] = mul_mod(f, pi, own[row]);
-                if (row < k) D[row] = mul_mod(f, pi, D[row]);
+                E[row][k] = sub_mod(0, mul_mod_t<MODE>(f, e, own[k]), p);
+                own[row] = mul_mod_t<MODE>(f, pi, own[row]);
+                if (row < k) D[row] = mul_mod_t<MODE>(f, pi, D[row]);
             }
         }
         if (tid == 0) {
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) {
         uint32_t r = idx / pb, s = idx - r * pb;
         uint64_t v = (r == s) ? add_mod(own[r], E[r][r], p) : E[r][s];
-        S.Z[r * BMAX + s] = mul_mod(f, dinv[r], v);
+        S.Z[r * BMAX + s] = mul_mod_t<MODE>(f, dinv[r], v);
     }
     if (tid < Bt) S.perm[rank0 + tid] = rowid[tid];
     if (tid < pb) {
@@ -389,7 +390,12 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 // ---------------------------------------------------------------------------
 void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
                        uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
-    hipLaunchKernelGGL(k_block_ge, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+    if (f.mode == MODE_M61)
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+    else if (f.mode == MODE_M31)
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+    else
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, gx_off, upd_off, plan);
 }
 
