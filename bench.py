@@ -10,10 +10,18 @@ input too) and runs the whole batch through one launch sequence.
 
 Prints one JSON line.  value = total matrix columns ranked per second
 (sum of sides / step time).  `roofline` refers to the trailing-update kernel,
-timed with HIP events on the library's stream; `cpu_baseline` is the oracle's
-OpenMP port of fp.rs on a bounded sample of the same batch on this host.
-Extra keys report the plain (VALU) path on the same batch and the per-batch
-corank check against the CPU-computed fixtures.
+timed with HIP events on the library's stream; `traffic` is its HBM bytes per
+launch from the committed rocprofv3 counter summary; `cpu_baseline` is the
+oracle's OpenMP port of fp.rs on a bounded sample of the same batch on this
+host.  Extra keys report the plain (VALU) path on the same batch, the single
+8192 x 8192 planted-corank matrix over 2^61-1 on both paths, the batched
+Berlekamp-Massey workload (1024 sequences, steps/s) and the per-item checks
+against the CPU-computed fixtures in tests/golden/.
+
+--copy-calibration additionally ranks one all-zero 4096 x 4096 u32 matrix held
+in device memory: its staging copy (64 MiB read, 64 MiB written) and widening
+kernel k_widen_u32 (64 MiB read, 128 MiB written) are copy kernels of known
+size that a counter run uses to scale FETCH_SIZE / WRITE_SIZE to bytes.
 """
 import argparse
 import json
@@ -64,6 +72,74 @@ def cpu_baseline(mats, budget_n3=4.0e10):
                       f"{dt:.1f} s"}
 
 
+def traffic_per_launch():
+    """HBM bytes per launch of the MFMA trailing update on this workload, from
+    the committed counter summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    same counters on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "k_update_mfma_batched_rank_64_p61_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration(ctx, n=4096):
+    dev = ctx.to_device(np.zeros((n, n), dtype=np.uint32))
+    r = ctx.rank_batch(M.Field(M61), [dev])
+    ctx.synchronize()
+    dev.free()
+    return {"copy_kernel": "k_widen_u32", "elements": n * n, "bytes_read": 4 * n * n, "bytes_written": 8 * n * n,
+            "rank": r[0]}
+
+
+def fixture(name):
+    path = os.path.join(ROOT, "tests", "golden", name)
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def single_large(ctx, n, p, steps):
+    """One n x n planted-corank matrix: MFMA and plain paths, columns/s."""
+    a = W.c3(n, p)
+    field = M.Field(p)
+    dev = [ctx.to_device(a)]
+    out = {"n": n, "p": p}
+    for name, method in (("mfma", M.METHOD_MFMA), ("plain", M.METHOD_PLAIN)):
+        ranks, t, st, _ = run(ctx, field, dev, method, steps, 1)
+        out[f"{name}_columns_per_s"] = n / t
+        out[f"{name}_ms"] = t * 1e3
+        out[f"{name}_corank"] = n - ranks[0]
+        out[f"{name}_update_ms"] = st["update_ms"]
+    out["speedup_mfma_vs_plain"] = out["mfma_columns_per_s"] / out["plain_columns_per_s"]
+    dev[0].free()
+    return out
+
+
+def bm_batch(ctx, steps):
+    """Batched Berlekamp-Massey: 1024 sequences, lengths uniform in [1000, 20000]."""
+    seqs, _ = W.c4()
+    field = M.Field(M61)
+    dev = [ctx.to_device(s, square=False) for s in seqs]
+    total = sum(s.size for s in seqs)
+    ctx.lc_batch(field, dev)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lcs = ctx.lc_batch(field, dev)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.lc_batch(field, dev, profile=True)
+    st = ctx.stats()
+    fx = fixture("c4_lc.json")
+    out = {"sequences": len(seqs), "steps": total, "steps_per_s": total / dt, "ms": dt * 1e3,
+           "kernel_ms": st["bm_ms"], "hbm_GBps": st["bm_bytes"] / (st["bm_ms"] / 1e3) / 1e9 if st["bm_ms"] else None,
+           "lc_match_fixture": (fx["lc"] == lcs) if fx else None}
+    for d in dev:
+        d.free()
+    return out
+
+
 def run(ctx, field, dev, method, steps, warmup):
     for _ in range(warmup):
         ctx.rank_batch(field, dev, method=method)
@@ -87,6 +163,8 @@ def main():
     ap.add_argument("--method", choices=["auto", "plain", "mfma"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 8192^2 and Berlekamp-Massey lines")
+    ap.add_argument("--copy-calibration", action="store_true", help="also run a 256 MiB copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -103,10 +181,8 @@ def main():
 
     ranks, t_step, st, _ = run(ctx, field, dev, method, args.steps, args.warmup)
     coranks = [n - r for n, r in zip(sides, ranks)]
-    golden_path = os.path.join(ROOT, "tests", "golden", "c2_coranks.json")
-    check = None
-    if os.path.exists(golden_path):
-        check = json.load(open(golden_path))["coranks"] == coranks
+    fx = fixture("c2_coranks.json")
+    check = (fx["coranks"] == coranks) if fx else None
 
     L = limb_count(M61) if st["limb_count"] else 0
     upd_s = st["update_ms"] / 1e3
@@ -114,7 +190,7 @@ def main():
         ops = 2.0 * L * L * st["macs"]
         achieved = ops / upd_s / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                "frac": achieved / PEAK_INT8_TOPS, "traffic": None}
+                "frac": achieved / PEAK_INT8_TOPS, "traffic": traffic_per_launch()}
     else:
         # VALU update: algorithmic HBM bytes = 16 per updated element (read + write)
         # + 8 per operand element of each tile (counted by the kernel)
@@ -140,6 +216,8 @@ def main():
                    "matrices": len(mats), "sum_sides": cols, "max_side": max(sides), "method": args.method},
         "roofline": roof,
         "update_ms_per_step": st["update_ms"],
+        "update_launches_per_step": st["update_launches"],
+        "update_avg_us": st["update_ms"] * 1e3 / max(1, st["update_launches"]),
         "update_macs_per_step": st["macs"],
         "coranks_match_fixture": check,
     }
@@ -147,6 +225,13 @@ def main():
         _, t_plain, st_p, _ = run(ctx, field, dev, M.METHOD_PLAIN, max(1, args.steps // 2), 0)
         out["plain_columns_per_s"] = cols / t_plain
         out["speedup_vs_plain"] = t_plain / t_step
+    for d in dev:
+        d.free()
+    if not args.no_extras:
+        out["single_8192_p61"] = single_large(ctx, 8192, M61, 2)
+        out["bm_batch_1024"] = bm_batch(ctx, 2)
+    if args.copy_calibration:
+        out["copy_calibration"] = copy_calibration(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mats)
     print(json.dumps(out))

@@ -185,8 +185,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     const size_t KA = (size_t)lp.L * KSTEP;
 
     // Workspace layout (256-byte aligned regions).
-    std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offS(count), offAl(count),
-        offXl(count);
+    std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offE(count), offS(count),
+        offAl(count), offXl(count);
     size_t cur = 0;
     const size_t A = 256;
     cur = align_up(cur + count * sizeof(RankMat), A);
@@ -217,6 +217,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offA[i] = cur; cur = align_up(cur + n * BMAX * 8, A);
         offX[i] = cur; cur = align_up(cur + BMAX * n * 8, A);
         offZ[i] = cur; cur = align_up(cur + BMAX * BMAX * 8, A);
+        offE[i] = cur; cur = align_up(cur + ETA_WORDS * 8, A);
         offS[i] = 0;
         if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
         offAl[i] = offXl[i] = 0;
@@ -242,6 +243,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.Ab = (uint64_t *)(base + offA[i]);
         S.X = (uint64_t *)(base + offX[i]);
         S.Z = (uint64_t *)(base + offZ[i]);
+        S.Eta = (uint64_t *)(base + offE[i]);
         S.Al = use_mfma ? (int8_t *)(base + offAl[i]) : nullptr;
         S.Xl = use_mfma ? (int8_t *)(base + offXl[i]) : nullptr;
         S.n = (uint32_t)n;

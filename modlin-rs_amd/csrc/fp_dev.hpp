@@ -169,23 +169,49 @@ MDL_HD uint64_t ff_step(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t 
     return sub_mod(mul_mod(f, pi, x), mul_mod(f, e, y), f.p);
 }
 
+// a b mod 2^61 - 1 up to a few multiples of p (result < 2^63 + 2^33), for
+// a, b < 2^61, from 31-bit halves: with a = a1 2^31 + a0, b = b1 2^31 + b0,
+//   a b = a1 b1 2^62 + (a0 b1 + a1 b0) 2^31 + a0 b0,   2^62 == 2,
+//   t 2^31 = (t >> 30) 2^61 + (t mod 2^30) 2^31 == (t >> 30) + (t mod 2^30) 2^31,
+// so four 32 x 32 -> 64 products and no 128-bit carries.  b is given split:
+// b0 = b mod 2^31, b1 = b >> 31.
+MDL_HD uint64_t mul_m61_lazy(uint64_t a, uint32_t b0, uint32_t b1) {
+    const uint32_t a0 = (uint32_t)a & 0x7fffffffu, a1 = (uint32_t)(a >> 31);
+    const uint64_t s = (uint64_t)a0 * b0 + (uint64_t)a1 * (2u * b1);  // < 2^62 + 2^61
+    const uint64_t t = (uint64_t)a0 * b1 + (uint64_t)a1 * b0;         // < 2^62
+    return s + (t >> 30) + ((t & 0x3fffffffull) << 31);
+}
+MDL_HD uint64_t fold_m61_64(uint64_t x) {  // x < 2^64 -> [0, 2^61 + 8)
+    return (x & P61) + (x >> 61);
+}
+MDL_HD uint64_t canon_m61(uint64_t x) {    // x < 2^63 -> [0, p)
+    x = (x & P61) + (x >> 61);               // <= p + 3
+    return x >= P61 ? x - P61 : x;
+}
+
 // Compile-time-mode variants (MODE = MODE_*; MODE_GENERIC defers to f.mode).
 template <int MODE>
 MDL_HD uint64_t mul_mod_t(const FieldParams &f, uint64_t a, uint64_t b) {
+    if (MODE == MODE_M61) return canon_m61(fold_m61_64(mul_m61_lazy(a, (uint32_t)b & 0x7fffffffu, (uint32_t)(b >> 31))));
     uint64_t h, l;
     mul_wide(a, b, h, l);
-    if (MODE == MODE_M61) return fold_m61(h, l);
     if (MODE == MODE_M31) return fold_m31(h, l);
     return reduce_prod(f, h, l);
 }
 template <int MODE>
 MDL_HD uint64_t ff_step_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
-    if (MODE == MODE_M61 || MODE == MODE_M31) {
+    if (MODE == MODE_M61) {
+        const uint64_t ne = P61 - e;
+        const uint64_t u = fold_m61_64(mul_m61_lazy(x, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31)));
+        const uint64_t v = fold_m61_64(mul_m61_lazy(y, (uint32_t)ne & 0x7fffffffu, (uint32_t)(ne >> 31)));
+        return canon_m61(u + v);
+    }
+    if (MODE == MODE_M31) {
         uint64_t h, l, h2, l2;
         mul_wide(pi, x, h, l);
         mul_wide(f.p - e, y, h2, l2);
         add128(h, l, h2, l2);
-        return MODE == MODE_M61 ? fold_m61(h, l) : fold_m31(h, l);
+        return fold_m31(h, l);
     }
     return ff_step(f, pi, x, e, y);
 }

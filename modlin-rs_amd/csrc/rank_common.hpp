@@ -21,6 +21,7 @@ struct RankMat {
     uint64_t *Ab;      // gathered A operand M[O, J]: n x BMAX (row stride BMAX)
     uint64_t *X;       // A^-1 M[R, c0:], BMAX x n (row stride n)
     uint64_t *Z;       // A^-1, BMAX x BMAX
+    uint64_t *Eta;     // row operations of the last block elimination (ETA_* layout)
     int8_t *Al;        // int8 limb planes of Ab (MFMA path)
     int8_t *Xl;        // int8 limb planes of the shifted X (MFMA path)
     uint32_t n, ld;
@@ -36,6 +37,19 @@ struct RankMat {
     unsigned long long macs; // F_p multiply-accumulates of the trailing updates
     unsigned long long upd_bytes;        // algorithmic HBM bytes of the trailing updates
 };
+
+// Record of one block elimination, written by k_block_ge and replayed by
+// k_block_z (u64 offsets into RankMat::Eta): the multiplier e of every block row
+// at every step, and per step the pivot, the pivot row's own-row coefficient and
+// the swapped position; per final row its own-row coefficient and 1 / D.
+constexpr uint32_t ETA_E = 0;                          // [step][row]
+constexpr uint32_t ETA_PI = BMAX * BMAX;               // [step]
+constexpr uint32_t ETA_OWNK = ETA_PI + BMAX;           // [step]
+constexpr uint32_t ETA_OWNF = ETA_OWNK + BMAX;         // [row]
+constexpr uint32_t ETA_DINV = ETA_OWNF + BMAX;         // [row]
+constexpr uint32_t ETA_SWQ = ETA_DINV + BMAX;          // [step]
+constexpr uint32_t ETA_WORDS = ETA_SWQ + BMAX;
+constexpr uint32_t ZG = 4;                             // workgroups per matrix in k_block_z
 
 // Work plan of one iteration: per-matrix tile offsets (prefix sums).
 struct RankPlan {

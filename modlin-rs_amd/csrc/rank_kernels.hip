@@ -33,18 +33,20 @@
 namespace mdl {
 
 // ---------------------------------------------------------------------------
-// k_block_ge
+// k_block_ge: pivot search and elimination of the panel part of the top block
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b) {
     RankMat &S = mats[blockIdx.x];
     if (S.done) return;
-    __shared__ uint64_t T[BMAX][BMAX + 1];   // panel part of the top block
-    __shared__ uint64_t E[BMAX][BMAX + 1];   // coefficients on pivot slots
-    __shared__ uint64_t own[BMAX];           // coefficient on the row's own original row
-    __shared__ uint64_t D[BMAX];             // diagonal of E A (pivot rows)
-    __shared__ uint32_t rowid[BMAX];
-    __shared__ uint32_t Jl[BMAX];
+    // One LDS row per block row: [0, BMAX) panel entries T, then own (coefficient
+    // on the row's own original row) and D (its pivot once the row is a pivot
+    // row, scaled like the row afterwards).
+    constexpr uint32_t COWN = BMAX, CD = BMAX + 1, UW = BMAX + 2;
+    __shared__ uint64_t U[BMAX][UW];
+    __shared__ uint64_t Et[BMAX][BMAX];     // multiplier e of each block row at each step
+    __shared__ uint64_t piK[BMAX], ownK[BMAX];
+    __shared__ uint32_t rowid[BMAX], Jl[BMAX], swq[BMAX];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
@@ -53,17 +55,17 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
     const uint64_t p = f.p;
 
-    for (uint32_t idx = tid; idx < Bt * w; idx += blockDim.x) {
-        uint32_t q = idx / w, c = idx - q * w;
-        uint32_t r = S.perm[rank0 + q];
-        T[q][c] = S.M[(size_t)r * ld + col + c];
+    for (uint32_t idx = tid; idx < BMAX * UW; idx += blockDim.x) {
+        const uint32_t q = idx / UW, c = idx - q * UW;
+        uint64_t v = 0;
+        if (q < Bt) {
+            if (c < w) v = S.M[(size_t)S.perm[rank0 + q] * ld + col + c];
+            else if (c == COWN) v = 1 % p;
+        }
+        U[q][c] = v;
     }
-    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) E[idx / BMAX][idx % BMAX] = 0;
-    if (tid < Bt) {
-        rowid[tid] = S.perm[rank0 + tid];
-        own[tid] = 1 % p;
-        D[tid] = 0;
-    }
+    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
+    if (tid < Bt) rowid[tid] = S.perm[rank0 + tid];
     __syncthreads();
 
     const uint32_t lane = tid & 63;
@@ -72,54 +74,58 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     for (uint32_t j = 0; j < w && k < Bt; j++) {
         // Pivot search: first block position q >= k with a nonzero in column j.
         // Every wave evaluates the same ballot, so no barrier is needed.
-        bool nz = (lane >= k) && (lane < Bt) && (T[lane][j] != 0);
+        bool nz = (lane >= k) && (lane < Bt) && (U[lane][j] != 0);
         uint64_t mask = __ballot(nz);
         if (mask == 0) continue;  // uniform: no pivot in this column
         uint32_t q = __ffsll((unsigned long long)mask) - 1;
         if (q != k) {  // swap block positions k and q (fp.rs:158-162)
             __syncthreads();  // every wave has read column j for its ballot before rows move
-            if (tid < w) {
-                uint64_t t = T[k][tid]; T[k][tid] = T[q][tid]; T[q][tid] = t;
-            } else if (tid >= 64 && tid < 64 + k) {
-                uint32_t s = tid - 64;
-                uint64_t t = E[k][s]; E[k][s] = E[q][s]; E[q][s] = t;
-            } else if (tid == 128) {
-                uint64_t t = own[k]; own[k] = own[q]; own[q] = t;
+            for (uint32_t c = tid; c < UW; c += blockDim.x) {
+                uint64_t t = U[k][c]; U[k][c] = U[q][c]; U[q][c] = t;
+            }
+            if (tid == 0) {
                 uint32_t r = rowid[k]; rowid[k] = rowid[q]; rowid[q] = r;
             }
             __syncthreads();
         }
-        const uint64_t pi = T[k][j];
-        uint64_t e = 0;
-        if (row < Bt && row != k) e = T[row][j];
-        __syncthreads();  // every e read before any row changes
-        if (row < Bt && row != k && e != 0) {
-            for (uint32_t c = j + 1 + sub; c < w; c += 16) T[row][c] = ff_step_t<MODE>(f, pi, T[row][c], e, T[k][c]);
-            for (uint32_t s = sub; s < k; s += 16) E[row][s] = ff_step_t<MODE>(f, pi, E[row][s], e, E[k][s]);
-            if (sub == 0) {
-                E[row][k] = sub_mod(0, mul_mod_t<MODE>(f, e, own[k]), p);
-                own[row] = mul_mod_t<MODE>(f, pi, own[row]);
-                if (row < k) D[row] = mul_mod_t<MODE>(f, pi, D[row]);
+        // Fraction-free Gauss-Jordan step on every other row with e = U[row][j] != 0:
+        // x <- pi x - e y on the panel columns after j; own and D scale by pi.
+        // Row k is only read and column j is never written, so the step needs no
+        // barrier between reading e and writing.
+        const uint64_t pi = U[k][j];
+        if (row < Bt && row != k) {
+            const uint64_t e = U[row][j];
+            if (sub == 0) Et[k][row] = e;
+            if (e != 0) {
+                const uint32_t nT = w - j - 1;
+                for (uint32_t it = sub; it < nT + 2; it += 16) {
+                    const uint32_t c = it < nT ? j + 1 + it : (it == nT ? COWN : CD);
+                    const uint64_t y = it < nT ? U[k][c] : 0;
+                    U[row][c] = ff_step_t<MODE>(f, pi, U[row][c], e, y);
+                }
             }
         }
         if (tid == 0) {
-            D[k] = pi;
+            U[k][CD] = pi;
             Jl[k] = j;
+            piK[k] = pi;
+            ownK[k] = U[k][COWN];
+            swq[k] = q;
         }
         __syncthreads();
         k++;
     }
     const uint32_t pb = k;
 
-    // Z = D^-1 E' with E'[r][s] = E[r][s] (s != r) and own[r] + E[r][r] (s == r):
-    // slot r of a pivot row and its own original row are the same row.
-    __shared__ uint64_t dinv[BMAX];
-    if (tid < pb) dinv[tid] = inv_mod_bin(D[tid], p);
-    __syncthreads();
-    for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) {
-        uint32_t r = idx / pb, s = idx - r * pb;
-        uint64_t v = (r == s) ? add_mod(own[r], E[r][r], p) : E[r][s];
-        S.Z[r * BMAX + s] = mul_mod_t<MODE>(f, dinv[r], v);
+    // The record for k_block_z (which builds A^-1 from it) and 1 / D.
+    uint64_t *eta = S.Eta;
+    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) eta[ETA_E + idx] = Et[idx / BMAX][idx % BMAX];
+    if (tid < pb) {
+        eta[ETA_PI + tid] = piK[tid];
+        eta[ETA_OWNK + tid] = ownK[tid];
+        eta[ETA_OWNF + tid] = U[tid][COWN];
+        eta[ETA_DINV + tid] = inv_mod_bin(U[tid][CD], p);
+        eta[ETA_SWQ + tid] = swq[tid];
     }
     if (tid < Bt) S.perm[rank0 + tid] = rowid[tid];
     if (tid < pb) {
@@ -142,6 +148,63 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         }
         S.first_nz_col = 0xffffffffu;
         S.iters++;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_block_z: Z = A^-1 for the pivot block A = M[R, J] of the last elimination.
+//
+// The elimination applied row operations to the block; the same operations on
+// the identity give rows E with E A = D (diagonal).  Only pivot slots < k are
+// nonzero in a row's coefficients at step k, so E is kept per pivot slot s:
+// slot s starts at step s as -e own[s] and is then carried through the later
+// steps (x <- pi x - e y, with the recorded swaps).  Slots are independent, so
+// ZG workgroups per matrix replay the record on BMAX / ZG slots each.  Finally
+// Z = D^-1 E' with E'[r][s] = E[r][s] (s != r) and own[r] + E[r][r] (s == r):
+// slot r of a pivot row and its own original row are the same row.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_block_z(RankMat *mats, FieldParams f) {
+    constexpr uint32_t SG = BMAX / ZG;  // slots per workgroup
+    RankMat &S = mats[blockIdx.x / ZG];
+    if (S.done) return;
+    const uint32_t pb = S.pb;
+    const uint32_t s0 = (blockIdx.x % ZG) * SG;
+    if (s0 >= pb) return;
+    __shared__ uint64_t Ec[BMAX][SG + 1];
+    __shared__ uint64_t Ek[BMAX][BMAX];          // the record's multipliers, steps s0..pb-1
+    __shared__ uint64_t piK[BMAX], ownK[BMAX];
+    __shared__ uint32_t swq[BMAX];
+    const uint32_t tid = threadIdx.x, row = tid / SG, sl = tid % SG, s = s0 + sl;
+    const uint64_t *eta = S.Eta;
+    const uint64_t p = f.p;
+    Ec[row][sl] = 0;
+    for (uint32_t idx = s0 * BMAX + tid; idx < pb * BMAX; idx += blockDim.x) Ek[idx / BMAX][idx % BMAX] = eta[ETA_E + idx];
+    if (tid < pb) {
+        piK[tid] = eta[ETA_PI + tid];
+        ownK[tid] = eta[ETA_OWNK + tid];
+        swq[tid] = (uint32_t)eta[ETA_SWQ + tid];
+    }
+    __syncthreads();
+    for (uint32_t k = s0; k < pb; k++) {
+        const uint32_t q = swq[k];
+        if (q != k) {
+            if (row == k) {
+                const uint64_t t = Ec[k][sl]; Ec[k][sl] = Ec[q][sl]; Ec[q][sl] = t;
+            }
+            __syncthreads();
+        }
+        const uint64_t e = Ek[k][row];
+        if (s <= k && s < pb && row != k && e != 0) {
+            const uint64_t x = (s == k) ? 0 : Ec[row][sl];
+            const uint64_t y = (s == k) ? ownK[k] : Ec[k][sl];
+            Ec[row][sl] = ff_step_t<MODE>(f, piK[k], x, e, y);
+        }
+        __syncthreads();
+    }
+    if (row < pb && s < pb) {
+        const uint64_t v = (row == s) ? add_mod(eta[ETA_OWNF + row], Ec[row][sl], p) : Ec[row][sl];
+        S.Z[row * BMAX + s] = mul_mod_t<MODE>(f, eta[ETA_DINV + row], v);
     }
 }
 
@@ -390,12 +453,16 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 // ---------------------------------------------------------------------------
 void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
                        uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
-    if (f.mode == MODE_M61)
+    if (f.mode == MODE_M61) {
         hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-    else if (f.mode == MODE_M31)
+        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+    } else if (f.mode == MODE_M31) {
         hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-    else
+        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+    } else {
         hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
+        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+    }
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, gx_off, upd_off, plan);
 }
 
