@@ -68,7 +68,10 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
     ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreate(&ctx->evA) != hipSuccess || hipEventCreate(&ctx->evB) != hipSuccess) {
+        hipEventCreate(&ctx->evA) != hipSuccess || hipEventCreate(&ctx->evB) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_ge, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fin, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return MODLIN_EHIP;
     }
@@ -80,12 +83,16 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) hipStreamSynchronize(ctx->aux);
     if (ctx->arena) hipFree(ctx->arena);
     mdl::release_pinned(ctx);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->evA) hipEventDestroy(ctx->evA);
     if (ctx->evB) hipEventDestroy(ctx->evB);
+    if (ctx->ev_ge) hipEventDestroy(ctx->ev_ge);
+    if (ctx->ev_fin) hipEventDestroy(ctx->ev_fin);
+    if (ctx->aux) hipStreamDestroy(ctx->aux);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

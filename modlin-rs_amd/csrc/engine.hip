@@ -1,7 +1,10 @@
 // Host orchestration of the rank engine: workspace layout in HBM, the panel
 // iteration loop and the context.  The reference's driver loop is
 // src/fp.rs:138-252 (one matrix); here every matrix of a batch advances one
-// panel per iteration, and each iteration is five launches on one stream.
+// panel per iteration.  An iteration is: block elimination and inverse (aux
+// stream), plan, gather, near update, finish (main stream), then the far update
+// (main stream) -- which overlaps the next iteration's block elimination: the
+// next panel's columns lie in the near part, already updated.
 #include <string.h>
 
 #include <algorithm>
@@ -13,13 +16,15 @@
 namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
-void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
-                           uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1);
-void launch_rank_iteration_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f,
-                                const LimbParams &lp, uint32_t b, uint32_t *gx_off, uint32_t *upd_off,
-                                RankPlan *plan, uint32_t *active, uint32_t grid_gx, uint32_t grid_upd,
-                                hipEvent_t e0, hipEvent_t e1);
+void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par);
+void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
+void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
+void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
+                         uint32_t par);
+void launch_update_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
+                         uint32_t par, uint32_t far);
+void launch_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
+                 const RankLaunch &rl, uint32_t par, int what);
 
 // Signed 8-bit digits for the balanced residues |x| <= (p-1)/2.
 uint32_t limb_count(uint64_t p) {
@@ -194,6 +199,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_upd = cur;
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+    const size_t off_far = cur;
+    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_plan = cur;
     cur = align_up(cur + sizeof(RankPlan), A);
     const size_t off_active = cur;
@@ -249,6 +256,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
         S.done = (n == 0) ? 1u : 0u;
+        S.pend = 0xffffffffu;
         if (n == 0) continue;
         active++;
         const size_t elems = n * n;
@@ -267,9 +275,11 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
                            S.perm, (uint32_t)n);
     }
     RankMat *dmats = (RankMat *)(base + 0);
-    uint32_t *gx_off = (uint32_t *)(base + off_gx);
-    uint32_t *upd_off = (uint32_t *)(base + off_upd);
-    RankPlan *plan = (RankPlan *)(base + off_plan);
+    RankLaunch rl;
+    rl.gx_off = (uint32_t *)(base + off_gx);
+    rl.near_off = (uint32_t *)(base + off_upd);
+    rl.far_off = (uint32_t *)(base + off_far);
+    rl.plan = (RankPlan *)(base + off_plan);
     uint32_t *dactive = (uint32_t *)(base + off_active);
     rc = copy_h2d(ctx, dmats, hm.data(), count * sizeof(RankMat));
     if (rc) return rc;
@@ -278,25 +288,47 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     MDL_HIP(hipGetLastError());
 
     const uint32_t nmat = (uint32_t)count;
-    const uint32_t grid_gx = 1024, grid_upd = (uint32_t)ctx->cu_count * 8;
+    rl.grid_gx = 1024;
+    rl.grid_near = (uint32_t)ctx->cu_count * 2;
+    rl.grid_far = (uint32_t)ctx->cu_count * 8;
     const int check_every = 8;
     std::vector<hipEvent_t> evs;
     if (profile) {
-        evs.resize(2 * check_every);
+        evs.resize(4 * check_every);
         for (auto &e : evs) MDL_HIP(hipEventCreate(&e));
     }
     double upd_ms = 0, upd_launches = 0;
     uint32_t h_active = active;
     uint64_t iters = 0;
+    hipStream_t aux = ctx->aux;
+    MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
     while (h_active > 0) {
         for (int s = 0; s < check_every; s++) {
-            hipEvent_t e0 = profile ? evs[2 * s] : nullptr, e1 = profile ? evs[2 * s + 1] : nullptr;
+            const uint32_t par = (uint32_t)((iters + s) & 1);
+            // block elimination and inverse once the previous finish has run
+            MDL_HIP(hipStreamWaitEvent(aux, ctx->ev_fin, 0));
+            launch_block(aux, dmats, nmat, f, b, par);
+            MDL_HIP(hipEventRecord(ctx->ev_ge, aux));
+            MDL_HIP(hipStreamWaitEvent(st, ctx->ev_ge, 0));
+            launch_plan(st, dmats, nmat, rl, par);
             if (use_mfma)
-                launch_rank_iteration_mfma(st, dmats, nmat, f, lp, b, gx_off, upd_off, plan, dactive, grid_gx,
-                                           grid_upd, e0, e1);
+                launch_mfma(st, dmats, nmat, f, lp, rl, par, -1);
             else
-                launch_rank_iteration(st, dmats, nmat, f, b, gx_off, upd_off, plan, dactive, grid_gx, grid_upd, e0,
-                                      e1);
+                launch_gather_plain(st, dmats, nmat, f, rl, par);
+            if (profile) MDL_HIP(hipEventRecord(evs[4 * s], st));
+            if (use_mfma)
+                launch_mfma(st, dmats, nmat, f, lp, rl, par, 0);
+            else
+                launch_update_plain(st, dmats, nmat, f, rl, par, 0);
+            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 1], st));
+            launch_finish(st, dmats, nmat, dactive, par);
+            MDL_HIP(hipEventRecord(ctx->ev_fin, st));
+            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 2], st));
+            if (use_mfma)
+                launch_mfma(st, dmats, nmat, f, lp, rl, par, 1);
+            else
+                launch_update_plain(st, dmats, nmat, f, rl, par, 1);
+            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 3], st));
         }
         MDL_HIP(hipGetLastError());
         iters += check_every;
@@ -304,14 +336,16 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         if (rc) return rc;
         if (profile) {
             for (int s = 0; s < check_every; s++) {
-                float ms = 0;
-                MDL_HIP(hipEventElapsedTime(&ms, evs[2 * s], evs[2 * s + 1]));
-                upd_ms += ms;
-                upd_launches += 1;
+                float ms0 = 0, ms1 = 0;
+                MDL_HIP(hipEventElapsedTime(&ms0, evs[4 * s], evs[4 * s + 1]));
+                MDL_HIP(hipEventElapsedTime(&ms1, evs[4 * s + 2], evs[4 * s + 3]));
+                upd_ms += ms0 + ms1;
+                upd_launches += 2;
             }
         }
         if (progress) progress(user, count - h_active, count);
     }
+    MDL_HIP(hipStreamSynchronize(aux));
     for (auto &e : evs) hipEventDestroy(e);
     rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
     if (rc) return rc;

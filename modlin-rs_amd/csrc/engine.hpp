@@ -18,6 +18,9 @@ struct modlin_ctx {
     void *arena = nullptr;
     size_t arena_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
+    // second stream for the look-ahead panel elimination, and its handshake events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_ge = nullptr, ev_fin = nullptr;
     int cu_count = 256;
     // pinned host staging buffers for host <-> device transfers (double-buffered)
     void *pinned[2] = {nullptr, nullptr};

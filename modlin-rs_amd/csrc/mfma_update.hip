@@ -40,20 +40,21 @@ struct TileMeta {
     uint32_t mi, r0, cb, nr, nc, pb;
 };
 
-__device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, const uint32_t *upd_off, uint32_t t) {
+__device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, const uint32_t *upd_off, uint32_t t,
+                                              uint32_t par, uint32_t far) {
     TileMeta m;
     m.mi = find_mat(upd_off, nmat, t);
     const RankMat &S = mats[m.mi];
+    const IterState &I = S.it[par];
     const uint32_t lt = t - upd_off[m.mi];
-    const uint32_t O = S.n - S.rank;
-    const uint32_t cols = S.cend - S.c0;
-    const uint32_t nct = cdiv(cols, UT);
-    const uint32_t tr = lt / nct, tc = lt - tr * nct;
-    m.r0 = tr * UT;
-    m.cb = tc * UT;
-    m.nr = min(UT, O - m.r0);
-    m.nc = min(UT, cols - m.cb);
-    m.pb = S.pb;
+    const uint32_t O = S.n - I.rank;
+    uint32_t cbase, cols;
+    update_range(I, far, cbase, cols);
+    m.pb = I.pb;
+    if (!tile_geom(O, cbase, cols, far, lt, m.r0, m.cb, m.nr, m.nc)) {
+        m.r0 = m.cb = m.nr = m.nc = 0;  // padding tile: no work
+        m.pb = 0;
+    }
     return m;
 }
 
@@ -140,7 +141,8 @@ __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs
 //               on the matrix cores: Z and the pivot rows are split into digits in LDS.
 template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
-                                                      const RankPlan *plan, FieldParams f, LimbParams lp) {
+                                                      const RankPlan *plan, FieldParams f, LimbParams lp,
+                                                      uint32_t par) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;
     constexpr int NC = 2 * L - 1, NW = (NC + 3) / 4;
@@ -153,18 +155,19 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint32_t mi = find_mat(gx_off, nmat, t);
         RankMat &S = mats[mi];
+        const IterState &I = S.it[par];
         const uint32_t lt = t - gx_off[mi];
-        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
+        const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld;
         const uint32_t ng = cdiv(O, GATHER_ROWS);
         if (lt < ng) {
             const uint32_t r0 = lt * GATHER_ROWS;
             const uint32_t rows = min(GATHER_ROWS, O - r0);
             for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
                 const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
-                const uint64_t *mrow = S.M + (size_t)S.perm[S.rank + r0 + i] * ld;
+                const uint64_t *mrow = S.M + (size_t)S.perm[I.rank + r0 + i] * ld;
                 uint64_t x[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) x[j] = (k4 + j < pb) ? mrow[S.J[k4 + j]] : 0;
+                for (int j = 0; j < 4; j++) x[j] = (k4 + j < pb) ? mrow[I.J[k4 + j]] : 0;
                 uint32_t w[L];
                 digits4<L>(x, p, w);
                 uint32_t *dst = (uint32_t *)(S.Al + (size_t)(r0 + i) * KA + k4);
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             continue;
         }
         const uint32_t cb = (lt - ng) * XCOLS;
-        const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
+        const uint32_t cols = min(XCOLS, I.cend - I.c0 - cb);
         __syncthreads();
         // As[k][la*64 + s] = digit la of Z[k][s];  Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
         for (uint32_t idx = tid; idx < UT * (KSTEP / 4); idx += blockDim.x) {
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             uint64_t x[4];
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)S.R[s4 + j] * ld + S.c0 + cb + c] : 0;
+                x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)I.R[s4 + j] * ld + I.c0 + cb + c] : 0;
             uint32_t w[L];
             digits4<L>(x, p, w);
 #pragma unroll
@@ -230,7 +233,8 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
 
 template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
-                                                     const RankPlan *plan, FieldParams f, LimbParams lp) {
+                                                     const RankPlan *plan, FieldParams f, LimbParams lp, uint32_t par,
+                                                     uint32_t far) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;    // LDS row stride: consecutive rows start 4 banks apart
     constexpr int NC = 2 * L - 1;        // shift classes
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     constexpr uint32_t NV = (NCH + 511) / 512;     // chunks per thread per operand tile
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
-    const uint32_t total = plan->upd_total;
+    const uint32_t total = far ? plan->far_total : plan->near_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
     const uint64_t p = f.p;
@@ -250,7 +254,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     uint32_t t = blockIdx.x;
     TileMeta cur;
     if (t < total) {
-        cur = tile_meta(mats, nmat, upd_off, t);
+        cur = tile_meta(mats, nmat, upd_off, t, par, far);
         if (cur.pb) {
             const RankMat &S = mats[cur.mi];
             const v4i *ga = (const v4i *)(S.Al + (size_t)cur.r0 * KA);
@@ -266,8 +270,9 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     }
     for (; t < total; t += gridDim.x) {
         RankMat &S = mats[cur.mi];
+        const IterState &I = S.it[par];
         const uint32_t r0 = cur.r0, cb = cur.cb, nr = cur.nr, nc = cur.nc, pb = cur.pb, ld = S.ld;
-        const uint32_t rank = S.rank, c0 = S.c0, jend = S.jend;
+        const uint32_t rank = I.rank, c0 = I.c0, jend = I.jend;
         uint32_t nzcol = 0xffffffffu;
         if (pb == 0) {
             // detection only (no pivots): scan the tile for a nonzero panel column
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
         const uint32_t tn = t + gridDim.x;
         TileMeta nxt;
         if (tn < total) {
-            nxt = tile_meta(mats, nmat, upd_off, tn);
+            nxt = tile_meta(mats, nmat, upd_off, tn, par, far);
             if (nxt.pb) {
                 const RankMat &Sn = mats[nxt.mi];
                 const v4i *ga = (const v4i *)(Sn.Al + (size_t)nxt.r0 * KA);
@@ -345,42 +350,32 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                     const uint32_t gcol = c0 + cb + c;
                     if (res != 0 && gcol < jend) nzcol = min(nzcol, gcol);
                 }
-            if (tid == 0) {
-                atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
-                atomicAdd(&S.upd_bytes, 16ull * nr * nc + 8ull * pb * (nr + nc));
-            }
         }
-        if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
+        if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
         cur = nxt;
     }
 }
 
-// Launchers: one panel iteration with the MFMA update (the block, plan and
-// finish kernels are those of the plain path).
-void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                       uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan);
-void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active);
-
+// Launchers: the digit gather and the two parts of the MFMA update.
 template <int L, int MODE>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
-                          uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t grid_gx, uint32_t grid_upd,
-                          hipEvent_t e0, hipEvent_t e1) {
-    hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(grid_gx), dim3(512), 0, st, dmats, nmat, gx_off, plan, f, lp);
-    if (e0) hipEventRecord(e0, st);
-    hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(grid_upd), dim3(512), 0, st, dmats, nmat, upd_off, plan, f, lp);
-    if (e1) hipEventRecord(e1, st);
+                          const RankLaunch &rl, uint32_t par, int what) {
+    if (what < 0)
+        hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(rl.grid_gx), dim3(512), 0, st, dmats, nmat, rl.gx_off,
+                           rl.plan, f, lp, par);
+    else
+        hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(what ? rl.grid_far : rl.grid_near), dim3(512), 0, st, dmats,
+                           nmat, what ? rl.far_off : rl.near_off, rl.plan, f, lp, par, (uint32_t)what);
 }
 
-void launch_rank_iteration_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f,
-                                const LimbParams &lp, uint32_t b, uint32_t *gx_off, uint32_t *upd_off,
-                                RankPlan *plan, uint32_t *active, uint32_t grid_gx, uint32_t grid_upd,
-                                hipEvent_t e0, hipEvent_t e1) {
-    launch_block_plan(st, dmats, nmat, f, b, gx_off, upd_off, plan);
-#define MDL_L(LL) launch_mfma_L<LL, MODE_GENERIC>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1)
+// what = -1: digit gather and X product; 0: near update; 1: far update.
+void launch_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
+                 const RankLaunch &rl, uint32_t par, int what) {
+#define MDL_L(LL) launch_mfma_L<LL, MODE_GENERIC>(st, dmats, nmat, f, lp, rl, par, what)
     if (f.mode == MODE_M61)
-        launch_mfma_L<8, MODE_M61>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1);
+        launch_mfma_L<8, MODE_M61>(st, dmats, nmat, f, lp, rl, par, what);
     else if (f.mode == MODE_M31)
-        launch_mfma_L<4, MODE_M31>(st, dmats, nmat, f, lp, gx_off, upd_off, plan, grid_gx, grid_upd, e0, e1);
+        launch_mfma_L<4, MODE_M31>(st, dmats, nmat, f, lp, rl, par, what);
     else {
         switch (lp.L) {
             case 1: MDL_L(1); break;
@@ -394,7 +389,6 @@ void launch_rank_iteration_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, c
         }
     }
 #undef MDL_L
-    launch_finish(st, dmats, nmat, active);
 }
 
 }  // namespace mdl

@@ -11,6 +11,24 @@ namespace mdl {
 // rank-pb update with pb <= BMAX.
 constexpr int BMAX = 64;
 
+// Parameters of one panel iteration, written by k_block_ge and read by the
+// kernels of the same iteration.  RankMat keeps two, selected by iteration
+// parity, so that the next panel's elimination can run while the far columns
+// of this panel's update are still being written.
+struct IterState {
+    uint32_t rank0;          // pivots before this panel: the top block starts here
+    uint32_t rank;           // pivots after it: the rows O still to update start here
+    uint32_t Bt;             // rows in the top block
+    uint32_t jend, pb;       // panel end; pivots found in the panel
+    uint32_t c0, cend;       // trailing-update column range
+    uint32_t split;          // first column of the far part of the update (jend + BMAX)
+    uint32_t resid;          // the panel needs a residual (missed-pivot) check
+    uint32_t pad;
+    uint32_t J[BMAX];        // pivot columns of the panel (absolute)
+    uint32_t R[BMAX];        // physical pivot rows, in pivot order
+    uint32_t top[BMAX];      // physical rows of the top block in their new order
+};
+
 // One matrix of a rank batch, resident in device memory.  The matrix is the
 // library's working copy (row-major, leading dimension ld); rows are addressed
 // through perm (logical position -> physical row) so that pivoting swaps
@@ -26,35 +44,45 @@ struct RankMat {
     int8_t *Xl;        // int8 limb planes of the shifted X (MFMA path)
     uint32_t n, ld;
     uint32_t rank, col;      // pivots found; first column not yet eliminated
-    uint32_t jend, pb;       // current panel end; pivots in the current panel
-    uint32_t c0, cend;       // trailing-update column range of this iteration
-    uint32_t resid;          // the panel needs a residual (missed-pivot) check
     uint32_t first_nz_col;   // smallest panel column with a nonzero residual
     uint32_t done;
     uint32_t iters;
-    uint32_t J[BMAX];        // pivot columns of the panel (absolute)
-    uint32_t R[BMAX];        // physical pivot rows, in pivot order
-    unsigned long long macs; // F_p multiply-accumulates of the trailing updates
-    unsigned long long upd_bytes;        // algorithmic HBM bytes of the trailing updates
+    uint32_t pend;           // missed pivot: position whose row moves to `rank`, applied by
+                             // the next k_plan (0xffffffff: none); see k_finish
+    IterState it[2];
+    // Counters, on a cache line of their own, kept by k_plan (no atomics in the
+    // update kernels).
+    alignas(128) unsigned long long macs;  // F_p multiply-accumulates of the trailing updates
+    unsigned long long upd_bytes;          // algorithmic HBM bytes of the trailing updates
+};
+
+// Work plan of one iteration: per-matrix tile offsets (prefix sums) of the
+// gather tasks and of the near (columns [c0, split)) and far ([split, cend))
+// update tiles.
+struct RankPlan {
+    uint32_t gx_total, near_total, far_total, pad;
 };
 
 // Record of one block elimination, written by k_block_ge and replayed by
 // k_block_z (u64 offsets into RankMat::Eta): the multiplier e of every block row
-// at every step, and per step the pivot, the pivot row's own-row coefficient and
-// the swapped position; per final row its own-row coefficient and 1 / D.
+// at every step, the pivot rows' entries U[r][s] in the pivot columns (upper
+// triangle), and per step the pivot, 1 / pivot, the pivot row's own-row
+// coefficient and the swapped position.
 constexpr uint32_t ETA_E = 0;                          // [step][row]
-constexpr uint32_t ETA_PI = BMAX * BMAX;               // [step]
-constexpr uint32_t ETA_OWNK = ETA_PI + BMAX;           // [step]
-constexpr uint32_t ETA_OWNF = ETA_OWNK + BMAX;         // [row]
-constexpr uint32_t ETA_DINV = ETA_OWNF + BMAX;         // [row]
-constexpr uint32_t ETA_SWQ = ETA_DINV + BMAX;          // [step]
+constexpr uint32_t ETA_U = BMAX * BMAX;                // [pivot row][pivot slot]
+constexpr uint32_t ETA_PI = 2 * BMAX * BMAX;           // [step]
+constexpr uint32_t ETA_DINV = ETA_PI + BMAX;           // [step]
+constexpr uint32_t ETA_OWNK = ETA_DINV + BMAX;         // [step]
+constexpr uint32_t ETA_SWQ = ETA_OWNK + BMAX;          // [step]
 constexpr uint32_t ETA_WORDS = ETA_SWQ + BMAX;
-constexpr uint32_t ZG = 4;                             // workgroups per matrix in k_block_z
+constexpr uint32_t ZG = 8;                             // workgroups (of 8 waves) per matrix in k_block_z
 
-// Work plan of one iteration: per-matrix tile offsets (prefix sums).
-struct RankPlan {
-    uint32_t gx_total, upd_total;
-    uint32_t pad0, pad1;
+
+// Device buffers and grid sizes of the per-iteration launches.
+struct RankLaunch {
+    uint32_t *gx_off, *near_off, *far_off;  // nmat + 1 offsets each
+    RankPlan *plan;
+    uint32_t grid_gx, grid_near, grid_far;
 };
 
 // Constants of the signed-digit limb representation used by the MFMA update.
@@ -74,6 +102,52 @@ constexpr uint32_t UT = 64;            // trailing-update tile edge
 constexpr uint32_t KSTEP = 64;         // int8 K per MFMA step (one digit plane of a panel)
 
 __device__ __forceinline__ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ uint32_t pad8(uint32_t x) { return (x + 7) & ~7u; }
+
+// Update tiles of one part of a matrix (O rows, `width` columns).  Near tiles are
+// numbered row-major.  Far tiles are numbered for the XCDs: a persistent grid of
+// a multiple of 8 workgroups walks tile t from workgroup t mod grid, and
+// workgroups are dispatched round-robin over the 8 XCDs, so tile t runs on XCD
+// t mod 8.  Column band tc goes to XCD tc mod 8 (bands padded to a multiple of 8,
+// padding tiles are empty) and each XCD walks its bands fastest, row band by row
+// band: an XCD's L2 then holds its own digit column tiles and the current digit
+// row tiles instead of all of them.
+__device__ __forceinline__ uint32_t part_tiles(uint32_t O, uint32_t width, uint32_t far) {
+    const uint32_t C = cdiv(width, UT);
+    return (C == 0 || O == 0) ? 0 : cdiv(O, UT) * (far ? pad8(C) : C);
+}
+
+// Geometry of tile lt of a part starting at column cbase (relative to c0);
+// false for a padding tile.
+__device__ __forceinline__ bool tile_geom(uint32_t O, uint32_t cbase, uint32_t width, uint32_t far, uint32_t lt,
+                                          uint32_t &r0, uint32_t &cb, uint32_t &nr, uint32_t &nc) {
+    const uint32_t C = cdiv(width, UT);
+    uint32_t tr, tc;
+    if (far) {
+        const uint32_t Cg = pad8(C) >> 3, x = lt & 7, j = lt >> 3;
+        tr = j / Cg;
+        tc = x + 8 * (j - tr * Cg);
+        if (tc >= C) return false;
+    } else {
+        tr = lt / C;
+        tc = lt - tr * C;
+    }
+    r0 = tr * UT;
+    cb = cbase + tc * UT;
+    nr = min(UT, O - r0);
+    nc = min(UT, cbase + width - cb);
+    return true;
+}
+
+// Column range of the near (far = 0) or far (far = 1) part of the update,
+// relative to c0: [base, base + width).
+__device__ __forceinline__ void update_range(const IterState &I, uint32_t far, uint32_t &base, uint32_t &width) {
+    const uint32_t cols = I.cend - I.c0;
+    const uint32_t nw = min(cols, I.split - I.c0);
+    base = far ? nw : 0;
+    width = far ? cols - nw : nw;
+}
 
 __device__ __forceinline__ uint32_t find_mat(const uint32_t *off, uint32_t nmat, uint32_t t) {
     // largest i with off[i] <= t (off is non-decreasing, off[nmat] > t)

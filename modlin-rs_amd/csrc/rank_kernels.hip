@@ -33,16 +33,15 @@
 namespace mdl {
 
 // ---------------------------------------------------------------------------
-// k_block_ge: pivot search and elimination of the panel part of the top block
+// k_block_ge: pivot search and fraction-free LU of the panel part of the top block
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b) {
+__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par) {
     RankMat &S = mats[blockIdx.x];
     if (S.done) return;
-    // One LDS row per block row: [0, BMAX) panel entries T, then own (coefficient
-    // on the row's own original row) and D (its pivot once the row is a pivot
-    // row, scaled like the row afterwards).
-    constexpr uint32_t COWN = BMAX, CD = BMAX + 1, UW = BMAX + 2;
+    // One LDS row per block row: [0, BMAX) panel entries T, then own (the
+    // coefficient on the row's own original row).
+    constexpr uint32_t COWN = BMAX, UW = BMAX + 1;
     __shared__ uint64_t U[BMAX][UW];
     __shared__ uint64_t Et[BMAX][BMAX];     // multiplier e of each block row at each step
     __shared__ uint64_t piK[BMAX], ownK[BMAX];
@@ -55,17 +54,25 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
     const uint64_t p = f.p;
 
+    // Rows of the top block; a missed pivot found by the last k_finish moves to
+    // position rank0 (the swap itself is committed to perm by k_plan).
+    const uint32_t pend = S.pend;
+    if (tid < Bt) {
+        const uint32_t pos = rank0 + tid;
+        const uint32_t src = (pend == 0xffffffffu) ? pos : (pos == rank0 ? pend : (pos == pend ? rank0 : pos));
+        rowid[tid] = S.perm[src];
+    }
+    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
+    __syncthreads();
     for (uint32_t idx = tid; idx < BMAX * UW; idx += blockDim.x) {
         const uint32_t q = idx / UW, c = idx - q * UW;
         uint64_t v = 0;
         if (q < Bt) {
-            if (c < w) v = S.M[(size_t)S.perm[rank0 + q] * ld + col + c];
+            if (c < w) v = S.M[(size_t)rowid[q] * ld + col + c];
             else if (c == COWN) v = 1 % p;
         }
         U[q][c] = v;
     }
-    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
-    if (tid < Bt) rowid[tid] = S.perm[rank0 + tid];
     __syncthreads();
 
     const uint32_t lane = tid & 63;
@@ -88,25 +95,24 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
             }
             __syncthreads();
         }
-        // Fraction-free Gauss-Jordan step on every other row with e = U[row][j] != 0:
-        // x <- pi x - e y on the panel columns after j; own and D scale by pi.
-        // Row k is only read and column j is never written, so the step needs no
-        // barrier between reading e and writing.
+        // Fraction-free elimination below the pivot (fp.rs:166-174 without the
+        // division): x <- pi x - e y on the panel columns after j for every row
+        // under k with e = U[row][j] != 0; own scales by pi.  Row k is only read
+        // and column j is never written, so no barrier separates reads and writes.
         const uint64_t pi = U[k][j];
-        if (row < Bt && row != k) {
+        if (row > k && row < Bt) {
             const uint64_t e = U[row][j];
             if (sub == 0) Et[k][row] = e;
             if (e != 0) {
                 const uint32_t nT = w - j - 1;
-                for (uint32_t it = sub; it < nT + 2; it += 16) {
-                    const uint32_t c = it < nT ? j + 1 + it : (it == nT ? COWN : CD);
+                for (uint32_t it = sub; it <= nT; it += 16) {
+                    const uint32_t c = it < nT ? j + 1 + it : COWN;
                     const uint64_t y = it < nT ? U[k][c] : 0;
                     U[row][c] = ff_step_t<MODE>(f, pi, U[row][c], e, y);
                 }
             }
         }
         if (tid == 0) {
-            U[k][CD] = pi;
             Jl[k] = j;
             piK[k] = pi;
             ownK[k] = U[k][COWN];
@@ -117,35 +123,46 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     }
     const uint32_t pb = k;
 
-    // The record for k_block_z (which builds A^-1 from it) and 1 / D.
+    // The record for k_block_z, which builds A^-1 from it.  A pivot row is final
+    // once it is the pivot, so U[r][s] = U[r][J_s] for s > r.
     uint64_t *eta = S.Eta;
-    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) eta[ETA_E + idx] = Et[idx / BMAX][idx % BMAX];
+    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) {
+        const uint32_t r = idx / BMAX, s = idx % BMAX;
+        eta[ETA_E + idx] = Et[r][s];
+        eta[ETA_U + idx] = (s > r && s < pb) ? U[r][Jl[s]] : 0;
+    }
     if (tid < pb) {
         eta[ETA_PI + tid] = piK[tid];
+        eta[ETA_DINV + tid] = inv_mod_bin(piK[tid], p);
         eta[ETA_OWNK + tid] = ownK[tid];
-        eta[ETA_OWNF + tid] = U[tid][COWN];
-        eta[ETA_DINV + tid] = inv_mod_bin(U[tid][CD], p);
         eta[ETA_SWQ + tid] = swq[tid];
     }
-    if (tid < Bt) S.perm[rank0 + tid] = rowid[tid];
+    // The new order of the block is applied to perm by k_plan, once the previous
+    // iteration's update (which reads perm) has finished.
+    IterState &I = S.it[par];
+    if (tid < Bt) I.top[tid] = rowid[tid];
     if (tid < pb) {
-        S.J[tid] = col + Jl[tid];
-        S.R[tid] = rowid[tid];
+        I.J[tid] = col + Jl[tid];
+        I.R[tid] = rowid[tid];
     }
     if (tid == 0) {
-        S.jend = jend;
-        S.pb = pb;
-        S.rank = rank0 + pb;
+        I.rank0 = rank0;
+        I.rank = rank0 + pb;
+        I.Bt = Bt;
+        I.jend = jend;
+        I.pb = pb;
+        I.split = jend + BMAX;
         const bool below = (n - rank0) > Bt;
         const uint32_t resid = (pb < w && below) ? 1u : 0u;
-        S.resid = resid;
+        I.resid = resid;
         if (pb > 0) {
-            S.c0 = resid ? col : jend;
-            S.cend = n;
+            I.c0 = resid ? col : jend;
+            I.cend = n;
         } else {
-            S.c0 = col;
-            S.cend = resid ? jend : col;  // detection only, or nothing at all
+            I.c0 = col;
+            I.cend = resid ? jend : col;  // detection only, or nothing at all
         }
+        S.rank = rank0 + pb;
         S.first_nz_col = 0xffffffffu;
         S.iters++;
     }
@@ -154,112 +171,171 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // ---------------------------------------------------------------------------
 // k_block_z: Z = A^-1 for the pivot block A = M[R, J] of the last elimination.
 //
-// The elimination applied row operations to the block; the same operations on
-// the identity give rows E with E A = D (diagonal).  Only pivot slots < k are
-// nonzero in a row's coefficients at step k, so E is kept per pivot slot s:
-// slot s starts at step s as -e own[s] and is then carried through the later
-// steps (x <- pi x - e y, with the recorded swaps).  Slots are independent, so
-// ZG workgroups per matrix replay the record on BMAX / ZG slots each.  Finally
-// Z = D^-1 E' with E'[r][s] = E[r][s] (s != r) and own[r] + E[r][r] (s == r):
-// slot r of a pivot row and its own original row are the same row.
+// The elimination's row operations, applied to the identity, give F with
+// F A = U (upper triangular, diagonal = the pivots).  In pivot-slot coordinates
+// F is lower triangular: F[r][r] = own[r] and F[r][s] (s < r) starts at step s as
+// -e own[s] and is then carried through the later steps (x <- pi x - e y, with
+// the recorded swaps).  Then Z = U^-1 F by back substitution.  Every slot s
+// (a column of F and of Z) is independent: one wave per slot, lane = block row,
+// with the step's pivot row broadcast from its lane -- no barriers.
 // ---------------------------------------------------------------------------
+// The value of lane `src` (wave-uniform) in every lane.
+__device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_block_z(RankMat *mats, FieldParams f) {
-    constexpr uint32_t SG = BMAX / ZG;  // slots per workgroup
+__global__ __launch_bounds__(512) void k_block_z(RankMat *mats, FieldParams f, uint32_t par) {
+    constexpr uint32_t WPG = 8;  // waves (slots) per workgroup; ZG * WPG = BMAX
     RankMat &S = mats[blockIdx.x / ZG];
     if (S.done) return;
-    const uint32_t pb = S.pb;
-    const uint32_t s0 = (blockIdx.x % ZG) * SG;
+    const uint32_t pb = S.it[par].pb;
+    const uint32_t s0 = (blockIdx.x % ZG) * WPG;
     if (s0 >= pb) return;
-    __shared__ uint64_t Ec[BMAX][SG + 1];
-    __shared__ uint64_t Ek[BMAX][BMAX];          // the record's multipliers, steps s0..pb-1
-    __shared__ uint64_t piK[BMAX], ownK[BMAX];
+    __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
+    __shared__ uint64_t Ur[BMAX][BMAX];  // U[r][t], t > r
+    __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
     __shared__ uint32_t swq[BMAX];
-    const uint32_t tid = threadIdx.x, row = tid / SG, sl = tid % SG, s = s0 + sl;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t *eta = S.Eta;
     const uint64_t p = f.p;
-    Ec[row][sl] = 0;
     for (uint32_t idx = s0 * BMAX + tid; idx < pb * BMAX; idx += blockDim.x) Ek[idx / BMAX][idx % BMAX] = eta[ETA_E + idx];
+    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) Ur[idx / BMAX][idx % BMAX] = eta[ETA_U + idx];
     if (tid < pb) {
         piK[tid] = eta[ETA_PI + tid];
+        dinv[tid] = eta[ETA_DINV + tid];
         ownK[tid] = eta[ETA_OWNK + tid];
         swq[tid] = (uint32_t)eta[ETA_SWQ + tid];
     }
     __syncthreads();
-    for (uint32_t k = s0; k < pb; k++) {
+    const uint32_t s = s0 + wave;
+    if (s >= pb) return;  // no barrier follows
+
+    // Forward: ev = F[lane][s].
+    uint64_t ev = 0;
+    uint64_t e_next = Ek[s][lane];
+    for (uint32_t k = s; k < pb; k++) {
+        const uint64_t e = e_next;
+        if (k + 1 < pb) e_next = Ek[k + 1][lane];
         const uint32_t q = swq[k];
         if (q != k) {
-            if (row == k) {
-                const uint64_t t = Ec[k][sl]; Ec[k][sl] = Ec[q][sl]; Ec[q][sl] = t;
-            }
-            __syncthreads();
+            const uint64_t vk = lane_bcast(ev, k), vq = lane_bcast(ev, q);
+            ev = (lane == k) ? vq : (lane == q) ? vk : ev;
         }
-        const uint64_t e = Ek[k][row];
-        if (s <= k && s < pb && row != k && e != 0) {
-            const uint64_t x = (s == k) ? 0 : Ec[row][sl];
-            const uint64_t y = (s == k) ? ownK[k] : Ec[k][sl];
-            Ec[row][sl] = ff_step_t<MODE>(f, piK[k], x, e, y);
-        }
-        __syncthreads();
+        const uint64_t yk = (k == s) ? ownK[k] : lane_bcast(ev, k);
+        const uint64_t x = (k == s) ? 0 : ev;
+        if (lane > k && e != 0) ev = ff_step_t<MODE>(f, piK[k], x, e, yk);
     }
-    if (row < pb && s < pb) {
-        const uint64_t v = (row == s) ? add_mod(eta[ETA_OWNF + row], Ec[row][sl], p) : Ec[row][sl];
-        S.Z[row * BMAX + s] = mul_mod_t<MODE>(f, eta[ETA_DINV + row], v);
+    const uint64_t fv = lane > s ? ev : (lane == s ? ownK[s] : 0);
+
+    // Back substitution, column-oriented: once z[t] is known every lane l < t adds
+    // U[l][t] z[t] to its sum; lane t-1 is then complete.
+    uint64_t acc = 0, z = 0;
+    for (int t = (int)pb - 1; t >= 0; t--) {
+        const uint64_t u = Ur[lane][t];
+        if (lane == (uint32_t)t) z = mul_mod_t<MODE>(f, dinv[t], sub_mod(fv, acc, p));
+        const uint64_t zt = lane_bcast(z, (uint32_t)t);
+        if (lane < (uint32_t)t) acc = add_mod(acc, mul_mod_t<MODE>(f, u, zt), p);
     }
+    if (lane < pb) S.Z[lane * BMAX + s] = z;
 }
 
 // ---------------------------------------------------------------------------
 // k_plan: tile counts and prefix sums (single workgroup)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *upd_off,
-                                              RankPlan *plan) {
-    __shared__ uint32_t sg[256], su[256];
-    __shared__ uint32_t base_g, base_u;
+__global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *near_off,
+                                              uint32_t *far_off, RankPlan *plan, uint32_t par) {
+    __shared__ uint32_t sg[256], sn[256], sf[256];
+    __shared__ uint32_t base_g, base_n, base_f;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) base_g = base_u = 0;
+    // Commit the last iteration's missed-pivot swap, then the top blocks' new row
+    // order (written by k_block_ge of this iteration, which saw the swap).  The
+    // previous iteration's far update, which read perm, has finished.
+    for (uint32_t i = tid; i < nmat; i += blockDim.x) {
+        RankMat &S = mats[i];
+        const uint32_t pend = S.pend;
+        if (pend != 0xffffffffu && !S.done) {
+            const uint32_t r = S.it[par].rank0;
+            const uint32_t t = S.perm[r];
+            S.perm[r] = S.perm[pend];
+            S.perm[pend] = t;
+            S.pend = 0xffffffffu;
+        }
+    }
+    __syncthreads();
+    for (uint32_t idx = tid; idx < nmat * BMAX; idx += blockDim.x) {
+        RankMat &S = mats[idx / BMAX];
+        const IterState &I = S.it[par];
+        const uint32_t q = idx % BMAX;
+        if (!S.done && q < I.Bt) S.perm[I.rank0 + q] = I.top[q];
+    }
+    if (tid == 0) base_g = base_n = base_f = 0;
     __syncthreads();
     for (uint32_t i0 = 0; i0 < nmat; i0 += 256) {
         const uint32_t i = i0 + tid;
-        uint32_t g = 0, u = 0;
+        uint32_t g = 0, un = 0, uf = 0;
         if (i < nmat) {
             const RankMat &S = mats[i];
             if (!S.done) {
-                const uint32_t O = S.n - S.rank;
-                const uint32_t cols = S.cend - S.c0;
+                const IterState &I = S.it[par];
+                const uint32_t O = S.n - I.rank;
+                const uint32_t cols = I.cend - I.c0;
                 if (O > 0 && cols > 0) {
-                    if (S.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
-                    u = cdiv(O, UT) * cdiv(cols, UT);
+                    if (I.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
+                    uint32_t base, wn, wf;
+                    update_range(I, 0, base, wn);
+                    un = part_tiles(O, wn, 0);
+                    update_range(I, 1, base, wf);
+                    uf = part_tiles(O, wf, 1);
+                    if (I.pb > 0) {
+                        // The updates' work: O x cols x pb products; per tile 16 B per
+                        // element and 8 B per operand element, summed over the tiles of
+                        // both parts (R row tiles, C column tiles).
+                        RankMat &Sw = mats[i];
+                        const unsigned long long R = cdiv(O, UT), C = cdiv(wn, UT) + cdiv(wf, UT);
+                        Sw.macs += (unsigned long long)O * cols * I.pb;
+                        Sw.upd_bytes += 16ull * O * cols + 8ull * I.pb * (O * C + (unsigned long long)cols * R);
+                    }
                 }
             }
         }
         sg[tid] = g;
-        su[tid] = u;
+        sn[tid] = un;
+        sf[tid] = uf;
         __syncthreads();
-        // inclusive Hillis-Steele scan
+        // inclusive Hillis-Steele scans
         for (uint32_t off = 1; off < 256; off <<= 1) {
-            const uint32_t ag = tid >= off ? sg[tid - off] : 0, au = tid >= off ? su[tid - off] : 0;
+            const uint32_t ag = tid >= off ? sg[tid - off] : 0, an = tid >= off ? sn[tid - off] : 0,
+                           af = tid >= off ? sf[tid - off] : 0;
             __syncthreads();
             sg[tid] += ag;
-            su[tid] += au;
+            sn[tid] += an;
+            sf[tid] += af;
             __syncthreads();
         }
         if (i < nmat) {
             gx_off[i] = base_g + sg[tid] - g;
-            upd_off[i] = base_u + su[tid] - u;
+            near_off[i] = base_n + sn[tid] - un;
+            far_off[i] = base_f + sf[tid] - uf;
         }
         __syncthreads();
         if (tid == 255) {
             base_g += sg[255];
-            base_u += su[255];
+            base_n += sn[255];
+            base_f += sf[255];
         }
         __syncthreads();
     }
     if (tid == 0) {
         gx_off[nmat] = base_g;
-        upd_off[nmat] = base_u;
+        near_off[nmat] = base_n;
+        far_off[nmat] = base_f;
         plan->gx_total = base_g;
-        plan->upd_total = base_u;
+        plan->near_total = base_n;
+        plan->far_total = base_f;
     }
 }
 
@@ -267,7 +343,7 @@ __global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint
 // k_gather_x
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gather_x(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
-                                                  const RankPlan *plan, FieldParams f) {
+                                                  const RankPlan *plan, FieldParams f, uint32_t par) {
     __shared__ uint64_t Zs[BMAX][BMAX + 1];
     __shared__ uint64_t Ms[BMAX][XCOLS + 1];
     const uint32_t total = plan->gx_total;
@@ -275,8 +351,9 @@ __global__ __launch_bounds__(256) void k_gather_x(RankMat *mats, uint32_t nmat, 
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint32_t mi = find_mat(gx_off, nmat, t);
         RankMat &S = mats[mi];
+        const IterState &I = S.it[par];
         const uint32_t lt = t - gx_off[mi];
-        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
+        const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld;
         const uint32_t ng = cdiv(O, GATHER_ROWS);
         if (lt < ng) {
             // Ab[i][k] = M[perm[rank + i]][J[k]]
@@ -284,18 +361,18 @@ __global__ __launch_bounds__(256) void k_gather_x(RankMat *mats, uint32_t nmat, 
             const uint32_t rows = min(GATHER_ROWS, O - r0);
             for (uint32_t idx = tid; idx < rows * pb; idx += blockDim.x) {
                 uint32_t i = idx / pb, kk = idx - i * pb;
-                uint32_t r = S.perm[S.rank + r0 + i];
-                S.Ab[(size_t)(r0 + i) * BMAX + kk] = S.M[(size_t)r * ld + S.J[kk]];
+                uint32_t r = S.perm[I.rank + r0 + i];
+                S.Ab[(size_t)(r0 + i) * BMAX + kk] = S.M[(size_t)r * ld + I.J[kk]];
             }
         } else {
             // X[k][c] = sum_s Z[k][s] M[R[s]][c0 + c]
             const uint32_t cb = (lt - ng) * XCOLS;
-            const uint32_t cols = min(XCOLS, S.cend - S.c0 - cb);
+            const uint32_t cols = min(XCOLS, I.cend - I.c0 - cb);
             __syncthreads();
             for (uint32_t idx = tid; idx < pb * pb; idx += blockDim.x) Zs[idx / pb][idx % pb] = S.Z[(idx / pb) * BMAX + idx % pb];
             for (uint32_t idx = tid; idx < pb * XCOLS; idx += blockDim.x) {
                 uint32_t s = idx / XCOLS, c = idx - s * XCOLS;
-                Ms[s][c] = (c < cols) ? S.M[(size_t)S.R[s] * ld + S.c0 + cb + c] : 0;
+                Ms[s][c] = (c < cols) ? S.M[(size_t)I.R[s] * ld + I.c0 + cb + c] : 0;
             }
             __syncthreads();
             const uint64_t sb = f.safe_batch;
@@ -324,22 +401,21 @@ __global__ __launch_bounds__(256) void k_gather_x(RankMat *mats, uint32_t nmat, 
 // k_update (VALU Barrett): S = M[O, c0:cend] - Ab X
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
-                                                     const RankPlan *plan, FieldParams f) {
+                                                     const RankPlan *plan, FieldParams f, uint32_t par, uint32_t far) {
     __shared__ uint64_t As[UT][BMAX + 1];
     __shared__ uint64_t Xs[BMAX][UT + 1];
-    const uint32_t total = plan->upd_total;
+    const uint32_t total = far ? plan->far_total : plan->near_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t ty = tid >> 4, tx = tid & 15;
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint32_t mi = find_mat(upd_off, nmat, t);
         RankMat &S = mats[mi];
+        const IterState &I = S.it[par];
         const uint32_t lt = t - upd_off[mi];
-        const uint32_t O = S.n - S.rank, pb = S.pb, ld = S.ld;
-        const uint32_t cols = S.cend - S.c0;
-        const uint32_t nct = cdiv(cols, UT);
-        const uint32_t tr = lt / nct, tc = lt - tr * nct;
-        const uint32_t r0 = tr * UT, cb = tc * UT;
-        const uint32_t nr = min(UT, O - r0), nc = min(UT, cols - cb);
+        const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld;
+        uint32_t cbase, cols, r0, cb, nr, nc;
+        update_range(I, far, cbase, cols);
+        if (!tile_geom(O, cbase, cols, far, lt, r0, cb, nr, nc)) continue;  // padding tile (uniform)
         __syncthreads();
         for (uint32_t idx = tid; idx < UT * pb; idx += blockDim.x) {
             uint32_t i = idx / pb, kk = idx - i * pb;
@@ -393,37 +469,34 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
         for (int a = 0; a < 4; a++) {
             const uint32_t i = ty * 4 + a;
             if (i >= nr) continue;
-            const uint32_t r = S.perm[S.rank + r0 + i];
-            uint64_t *mrow = S.M + (size_t)r * ld + S.c0 + cb;
+            const uint32_t r = S.perm[I.rank + r0 + i];
+            uint64_t *mrow = S.M + (size_t)r * ld + I.c0 + cb;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const uint32_t cc = tx * 4 + c;
                 if (cc >= nc) continue;
                 uint64_t v = sub_mod(mrow[cc], tot[a][c], p);
                 if (pb > 0) mrow[cc] = v;
-                const uint32_t gcol = S.c0 + cb + cc;
-                if (v != 0 && gcol < S.jend) nzcol = min(nzcol, gcol);
+                const uint32_t gcol = I.c0 + cb + cc;
+                if (v != 0 && gcol < I.jend) nzcol = min(nzcol, gcol);
             }
         }
-        if (S.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
-        if (tid == 0 && pb > 0) {
-            atomicAdd(&S.macs, (unsigned long long)nr * nc * pb);
-            atomicAdd(&S.upd_bytes, 16ull * nr * nc + 8ull * pb * (nr + nc));
-        }
+        if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_finish: advance the panel; bring a missed pivot to the top of the block
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active) {
+__global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active, uint32_t par) {
     RankMat &S = mats[blockIdx.x];
     if (S.done) return;
     __shared__ uint32_t best;
+    const IterState &I = S.it[par];
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n;
     uint32_t jstar = S.first_nz_col;
-    const bool miss = S.resid && jstar < S.jend;
+    const bool miss = I.resid && jstar < I.jend;
     if (miss) {
         if (tid == 0) best = 0xffffffffu;
         __syncthreads();
@@ -433,14 +506,13 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
         }
         if (mine != 0xffffffffu) atomicMin(&best, mine);
         __syncthreads();
-        if (tid == 0 && best != 0xffffffffu && best != S.rank) {
-            uint32_t t = S.perm[S.rank];
-            S.perm[S.rank] = S.perm[best];
-            S.perm[best] = t;
-        }
+        // The far part of this iteration's update still reads perm (its digit rows
+        // follow the current order), so the swap to position rank is only recorded
+        // here; the next k_block_ge reads through it and the next k_plan commits it.
+        if (tid == 0 && best != 0xffffffffu && best != S.rank) S.pend = best;
     }
     if (tid == 0) {
-        S.col = miss ? jstar : S.jend;
+        S.col = miss ? jstar : I.jend;
         if (S.col >= n || S.rank >= n) {
             S.done = 1;
             atomicSub(active, 1u);
@@ -449,36 +521,39 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 }
 
 // ---------------------------------------------------------------------------
-// Host launchers (one panel iteration of every matrix in the batch)
+// Host launchers
 // ---------------------------------------------------------------------------
-void launch_block_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                       uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan) {
+void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par) {
     if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
     } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
     } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b);
-        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(1024), 0, st, dmats, f);
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
     }
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, gx_off, upd_off, plan);
 }
 
-void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active) {
-    hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active);
+void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.plan,
+                       par);
 }
 
-void launch_rank_iteration(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                           uint32_t *gx_off, uint32_t *upd_off, RankPlan *plan, uint32_t *active, uint32_t grid_gx,
-                           uint32_t grid_upd, hipEvent_t ev_upd0, hipEvent_t ev_upd1) {
-    launch_block_plan(st, dmats, nmat, f, b, gx_off, upd_off, plan);
-    hipLaunchKernelGGL(k_gather_x, dim3(grid_gx), dim3(256), 0, st, dmats, nmat, gx_off, plan, f);
-    if (ev_upd0) hipEventRecord(ev_upd0, st);
-    hipLaunchKernelGGL(k_update_valu, dim3(grid_upd), dim3(256), 0, st, dmats, nmat, upd_off, plan, f);
-    if (ev_upd1) hipEventRecord(ev_upd1, st);
-    launch_finish(st, dmats, nmat, active);
+void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par) {
+    hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active, par);
+}
+
+void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
+                         uint32_t par) {
+    hipLaunchKernelGGL(k_gather_x, dim3(rl.grid_gx), dim3(256), 0, st, dmats, nmat, rl.gx_off, rl.plan, f, par);
+}
+
+void launch_update_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
+                         uint32_t par, uint32_t far) {
+    hipLaunchKernelGGL(k_update_valu, dim3(far ? rl.grid_far : rl.grid_near), dim3(256), 0, st, dmats, nmat,
+                       far ? rl.far_off : rl.near_off, rl.plan, f, par, far);
 }
 
 }  // namespace mdl
