@@ -140,6 +140,38 @@ def bm_batch(ctx, steps):
     return out
 
 
+def bm_long(ctx, steps):
+    """Long Berlekamp-Massey: one 200,000-element MIXMAX-256 (skip 2) sequence, seed 1."""
+    seq = W.c5_bm()
+    field = M.Field(M61)
+    dev = [ctx.to_device(seq, square=False)]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lc = ctx.lc_batch(field, dev)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.lc_batch(field, dev, profile=True)
+    st = ctx.stats()
+    fx = fixture("c5_bm.json")
+    dev[0].free()
+    return {"n": int(seq.size), "steps_per_s": seq.size / dt, "ms": dt * 1e3, "us_per_step": dt * 1e6 / seq.size,
+            "kernel_ms": st["bm_ms"], "lc": lc[0], "lc_match_fixture": (fx["lc"] == lc[0]) if fx else None}
+
+
+def rank_batch_u32(ctx, steps):
+    """256 matrices 2048 x 2048 over 65521, u32 storage (the small-prime batch)."""
+    mats = W.c5_rank(count=256)
+    field = M.Field(65521)
+    dev = [ctx.to_device(m) for m in mats]
+    ranks, t, st, _ = run(ctx, field, dev, M.METHOD_AUTO, steps, 1)
+    fx = fixture("c5_rank_coranks.json")
+    for d in dev:
+        d.free()
+    return {"matrices": 256, "n": 2048, "p": 65521, "columns_per_s": 256 * 2048 / t, "ms": t * 1e3,
+            "update_ms": st["update_ms"], "limbs": st["limb_count"],
+            "coranks_match_fixture": ([2048 - r for r in ranks] == fx["coranks"]) if fx else None}
+
+
 def run(ctx, field, dev, method, steps, warmup):
     for _ in range(warmup):
         ctx.rank_batch(field, dev, method=method)
@@ -165,6 +197,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-extras", action="store_true", help="skip the 8192^2 and Berlekamp-Massey lines")
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 256 MiB copy kernel")
+    ap.add_argument("--full", action="store_true",
+                    help="also the 8192^2 matrix over 2^31-1, the 16384^2 matrices and the 256 x 2048^2 u32 batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -230,6 +264,12 @@ def main():
     if not args.no_extras:
         out["single_8192_p61"] = single_large(ctx, 8192, M61, 2)
         out["bm_batch_1024"] = bm_batch(ctx, 2)
+        out["bm_long_200k"] = bm_long(ctx, 1)
+    if args.full:
+        out["single_8192_p31"] = single_large(ctx, 8192, (1 << 31) - 1, 2)
+        out["single_16384_p61"] = single_large(ctx, 16384, M61, 1)
+        out["single_16384_p31"] = single_large(ctx, 16384, (1 << 31) - 1, 1)
+        out["rank_batch_256x2048_u32_p65521"] = rank_batch_u32(ctx, 2)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(ctx)
     if not args.no_cpu:

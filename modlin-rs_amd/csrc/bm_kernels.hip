@@ -44,7 +44,7 @@ __device__ __forceinline__ uint64_t block_sum_mod(uint64_t v, uint64_t p, uint64
     return v;
 }
 
-template <int NT>
+template <int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     BmSeq &Q = seqs[blockIdx.x];
     const uint32_t n = Q.n;
@@ -68,6 +68,7 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     __syncthreads();
     uint32_t l = 0, bl = 1, m = 1;
     uint64_t bb_inv = 1 % p;
+    unsigned long long bytes = 0;  // kept in a register, stored once at the end
     for (uint32_t i = 0; i < n; i++) {
         uint64_t ah = 0, al = 0, part = 0, cnt = 0;
         for (uint32_t j = 1 + tid; j <= l; j += NT) {
@@ -75,27 +76,27 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
             mul_wide(c[j], s[i - j], h, lo);
             add128(ah, al, h, lo);
             if (++cnt == sb) {
-                part = add_mod(part, reduce128(f, ah, al), p);
+                part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
                 ah = al = 0;
                 cnt = 0;
             }
         }
-        if (cnt) part = add_mod(part, reduce128(f, ah, al), p);
+        if (cnt) part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
         uint64_t d = add_mod(block_sum_mod<NT>(part, p, red), s[i], p);
-        if (tid == 0) Q.bytes += 8ull + 16ull * l;
+        bytes += 8ull + 16ull * l;
         if (d == 0) {
             m++;
             continue;
         }
-        const uint64_t coef = mul_mod(f, d, bb_inv);
+        const uint64_t coef = mul_mod_t<MODE>(f, d, bb_inv);
         const uint32_t jmax = min(bl, n - m);
         if (2 * l <= i) {
             const uint32_t old_l = l;
-            if (tid == 0) Q.bytes += 24ull * jmax + 16ull * (old_l + 1);
+            bytes += 24ull * jmax + 16ull * (old_l + 1);
             for (uint32_t j = tid; j <= old_l; j += NT) t[j] = c[j];
             if (tid == 0) sh_inv = inv_mod_bin(d, p);
             __syncthreads();
-            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod(f, coef, b[j]), p);
+            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod_t<MODE>(f, coef, b[j]), p);
             uint64_t *tmp = b;
             b = t;
             t = tmp;
@@ -104,17 +105,17 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
             l = i + 1 - old_l;
             m = 1;
         } else {
-            if (tid == 0) Q.bytes += 24ull * jmax;
-            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod(f, coef, b[j]), p);
+            bytes += 24ull * jmax;
+            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod_t<MODE>(f, coef, b[j]), p);
             m++;
         }
         __syncthreads();
     }
-    if (tid == 0) Q.lc = l;
+    if (tid == 0) {
+        Q.lc = l;
+        Q.bytes = bytes;
+    }
 }
-
-template __global__ void k_bm_batch<256>(BmSeq *, FieldParams);
-template __global__ void k_bm_batch<1024>(BmSeq *, FieldParams);
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -182,10 +183,16 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     rc = copy_h2d(ctx, ds, hs.data(), count * sizeof(BmSeq));
     if (rc) return rc;
     if (profile) MDL_HIP(hipEventRecord(ctx->evA, st));
-    if (count == 1 && maxn >= 4096)
-        hipLaunchKernelGGL(k_bm_batch<1024>, dim3((uint32_t)count), dim3(1024), 0, st, ds, f);
-    else
-        hipLaunchKernelGGL(k_bm_batch<256>, dim3((uint32_t)count), dim3(256), 0, st, ds, f);
+    const bool wide = count == 1 && maxn >= 4096;
+#define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
+    if (f.mode == MODE_M61) {
+        if (wide) MDL_BM(1024, MODE_M61); else MDL_BM(256, MODE_M61);
+    } else if (f.mode == MODE_M31) {
+        if (wide) MDL_BM(1024, MODE_M31); else MDL_BM(256, MODE_M31);
+    } else {
+        if (wide) MDL_BM(1024, MODE_GENERIC); else MDL_BM(256, MODE_GENERIC);
+    }
+#undef MDL_BM
     if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
     MDL_HIP(hipGetLastError());
     rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));

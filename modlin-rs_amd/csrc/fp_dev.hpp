@@ -199,6 +199,12 @@ MDL_HD uint64_t mul_mod_t(const FieldParams &f, uint64_t a, uint64_t b) {
     return reduce_prod(f, h, l);
 }
 template <int MODE>
+MDL_HD uint64_t reduce_t(const FieldParams &f, uint64_t th, uint64_t tl) {  // any 128-bit t
+    if (MODE == MODE_M61) return fold_m61(th, tl);
+    if (MODE == MODE_M31) return fold_m31(th, tl);
+    return reduce128(f, th, tl);
+}
+template <int MODE>
 MDL_HD uint64_t ff_step_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
     if (MODE == MODE_M61) {
         const uint64_t ne = P61 - e;
