@@ -79,7 +79,9 @@ typedef struct modlin_seq_desc {
     uint32_t mem;
 } modlin_seq_desc_t;
 
-#define MODLIN_FLAG_PROFILE 1u /* time the dominant kernels with HIP events */
+#define MODLIN_FLAG_PROFILE 1u  /* time the dominant kernels with HIP events (direct launches) */
+#define MODLIN_FLAG_NO_GRAPH 2u /* launch each iteration block directly instead of replaying the
+                                   context's captured hipGraph of it */
 
 typedef struct modlin_opts {
     uint32_t method; /* modlin_method */
@@ -100,6 +102,7 @@ typedef struct modlin_stats {
     double total_ms;        /* wall time of the device part of the call */
     double update_bytes;    /* algorithmic HBM bytes of the trailing updates: 16 per updated
                                element (read + write) + 8 per operand element per tile */
+    double graph;           /* 1 if the rank loop replayed a captured hipGraph */
 } modlin_stats_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */

@@ -90,6 +90,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->evA) hipEventDestroy(ctx->evA);
     if (ctx->evB) hipEventDestroy(ctx->evB);
+    if (ctx->rank_graph) hipGraphExecDestroy(ctx->rank_graph);
     if (ctx->ev_ge) hipEventDestroy(ctx->ev_ge);
     if (ctx->ev_fin) hipEventDestroy(ctx->ev_fin);
     if (ctx->aux) hipStreamDestroy(ctx->aux);

@@ -290,7 +290,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     const uint32_t nmat = (uint32_t)count;
     rl.grid_gx = 1024;
     rl.grid_near = (uint32_t)ctx->cu_count * 2;
-    rl.grid_far = (uint32_t)ctx->cu_count * 8;
+    rl.grid_far = (uint32_t)ctx->cu_count * (use_mfma ? 2 : 8);  // multiples of 8 (XCD numbering)
+    rl.far_items = use_mfma ? 1u : 0u;
     const int check_every = 8;
     std::vector<hipEvent_t> evs;
     if (profile) {
@@ -301,36 +302,78 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     uint32_t h_active = active;
     uint64_t iters = 0;
     hipStream_t aux = ctx->aux;
-    MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
-    while (h_active > 0) {
+    // One block of check_every iterations: block elimination and inverse on the
+    // aux stream once the previous finish has run; plan, gather, near update and
+    // finish on the main stream; the far update overlapping the next block
+    // elimination.  (check_every is even, so iteration parities repeat per block.)
+    auto enqueue_block = [&](bool prof) -> hipError_t {
         for (int s = 0; s < check_every; s++) {
-            const uint32_t par = (uint32_t)((iters + s) & 1);
-            // block elimination and inverse once the previous finish has run
-            MDL_HIP(hipStreamWaitEvent(aux, ctx->ev_fin, 0));
+            const uint32_t par = (uint32_t)(s & 1);
+            hipError_t e;
+            if ((e = hipStreamWaitEvent(aux, ctx->ev_fin, 0)) != hipSuccess) return e;
             launch_block(aux, dmats, nmat, f, b, par);
-            MDL_HIP(hipEventRecord(ctx->ev_ge, aux));
-            MDL_HIP(hipStreamWaitEvent(st, ctx->ev_ge, 0));
+            if ((e = hipEventRecord(ctx->ev_ge, aux)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(st, ctx->ev_ge, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, -1);
             else
                 launch_gather_plain(st, dmats, nmat, f, rl, par);
-            if (profile) MDL_HIP(hipEventRecord(evs[4 * s], st));
+            if (prof) hipEventRecord(evs[4 * s], st);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, 0);
             else
                 launch_update_plain(st, dmats, nmat, f, rl, par, 0);
-            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 1], st));
+            if (prof) hipEventRecord(evs[4 * s + 1], st);
             launch_finish(st, dmats, nmat, dactive, par);
-            MDL_HIP(hipEventRecord(ctx->ev_fin, st));
-            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 2], st));
+            if ((e = hipEventRecord(ctx->ev_fin, st)) != hipSuccess) return e;
+            if (prof) hipEventRecord(evs[4 * s + 2], st);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, 1);
             else
                 launch_update_plain(st, dmats, nmat, f, rl, par, 1);
-            if (profile) MDL_HIP(hipEventRecord(evs[4 * s + 3], st));
+            if (prof) hipEventRecord(evs[4 * s + 3], st);
         }
-        MDL_HIP(hipGetLastError());
+        return hipGetLastError();
+    };
+    // hipGraph of the block (not while profiling: those events are timed per launch).
+    bool use_graph = !profile && !(opts && (opts->flags & MODLIN_FLAG_NO_GRAPH));
+    if (use_graph) {
+        char key[256];
+        snprintf(key, sizeof key, "%p/%u/%u/%d/%llu/%u/%u/%u/%u", (void *)base, nmat, b, (int)use_mfma,
+                 (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L);
+        if (!ctx->rank_graph || ctx->rank_graph_key != key) {
+            if (ctx->rank_graph) {
+                hipGraphExecDestroy(ctx->rank_graph);
+                ctx->rank_graph = nullptr;
+                ctx->rank_graph_key.clear();
+            }
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t e1 = hipEventRecord(ctx->ev_fin, st);
+                if (e1 == hipSuccess) e1 = enqueue_block(false);
+                e = hipStreamEndCapture(st, &graph);
+                if (e == hipSuccess) e = e1;
+            }
+            if (e == hipSuccess && graph) e = hipGraphInstantiate(&ctx->rank_graph, graph, nullptr, nullptr, 0);
+            if (graph) hipGraphDestroy(graph);
+            if (e == hipSuccess && ctx->rank_graph) {
+                ctx->rank_graph_key = key;
+            } else {
+                (void)hipGetLastError();  // capture unavailable: launch the blocks directly
+                ctx->rank_graph = nullptr;
+                use_graph = false;
+            }
+        }
+    }
+    ctx->stats.graph = use_graph ? 1.0 : 0.0;
+    MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
+    while (h_active > 0) {
+        if (use_graph)
+            MDL_HIP(hipGraphLaunch(ctx->rank_graph, st));
+        else
+            MDL_HIP(enqueue_block(profile));
         iters += check_every;
         rc = copy_d2h(ctx, &h_active, dactive, sizeof(uint32_t));
         if (rc) return rc;

@@ -21,6 +21,10 @@ struct modlin_ctx {
     // second stream for the look-ahead panel elimination, and its handshake events
     hipStream_t aux = nullptr;
     hipEvent_t ev_ge = nullptr, ev_fin = nullptr;
+    // captured 8-iteration launch block of the rank loop, reused while its
+    // arguments (arena, batch shape, field, method) are unchanged
+    hipGraphExec_t rank_graph = nullptr;
+    std::string rank_graph_key;
     int cu_count = 256;
     // pinned host staging buffers for host <-> device transfers (double-buffered)
     void *pinned[2] = {nullptr, nullptr};

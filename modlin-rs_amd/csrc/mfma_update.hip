@@ -356,6 +356,109 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     }
 }
 
+// Far part of the update, by row band: work item (row band, XCD group x)
+// updates the column tiles tc = x, x + 8, ... of that row band.  The row band's
+// A digit tile is staged in LDS once and stays there while the group's B digit
+// tiles stream through (the next one prefetched into registers), so each tile
+// stages 32 KiB instead of 64.  Items are numbered rows-major with the group
+// fastest; with a grid that is a multiple of 8, item t runs on XCD t mod 8 = x,
+// so an XCD's L2 holds its own column tiles.
+template <int L, int MODE>
+__global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
+                                                    const RankPlan *plan, FieldParams f, LimbParams lp, uint32_t par) {
+    constexpr uint32_t KA = L * KSTEP;
+    constexpr uint32_t KAP = KA + 16;
+    constexpr int NC = 2 * L - 1;
+    constexpr int NW = (NC + 3) / 4;
+    constexpr uint32_t CH = KA / 16;
+    constexpr uint32_t NCH = UT * CH;
+    constexpr uint32_t NV = (NCH + 511) / 512;
+    __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
+    const uint32_t total = plan->far_total;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
+    const uint64_t p = f.p;
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t mi = find_mat(item_off, nmat, item);
+        RankMat &S = mats[mi];
+        const IterState &I = S.it[par];
+        const uint32_t lt = item - item_off[mi];
+        const uint32_t x = lt & 7, band = lt >> 3;
+        const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld, rank = I.rank, c0 = I.c0;
+        uint32_t cbase, width;
+        update_range(I, 1, cbase, width);
+        const uint32_t C = cdiv(width, UT);
+        if (pb == 0 || x >= C) continue;  // uniform
+        const uint32_t r0 = band * UT, nr = min(UT, O - r0);
+        __syncthreads();  // the previous item's last tile is done with As
+        {
+            const v4i *ga = (const v4i *)(S.Al + (size_t)r0 * KA);
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++) {
+                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
+                if (NCH % 512 == 0 || v < NCH) *(v4i *)(As + row * KAP + ch * 16) = ga[v];
+            }
+        }
+        uint64_t *mrow[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t i = wr + 4 * (lane >> 4) + r;
+            mrow[r] = (i < nr) ? S.M + (size_t)S.perm[rank + r0 + i] * ld + c0 : nullptr;
+        }
+        v4i rb[NV];
+        {
+            const v4i *gb = (const v4i *)(S.Xl + (size_t)(cbase + x * UT) * KA);
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++)
+                if (NCH % 512 == 0 || tid + j * 512 < NCH) rb[j] = gb[tid + j * 512];
+        }
+        for (uint32_t tc = x; tc < C; tc += 8) {
+            const uint32_t cb = cbase + tc * UT, nc = min(UT, cbase + width - cb);
+            __syncthreads();  // the previous tile is done with Bs (and As is written)
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++) {
+                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
+                if (NCH % 512 == 0 || v < NCH) *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
+            }
+            __syncthreads();
+            if (tc + 8 < C) {  // prefetch the next B digit tile
+                const v4i *gb = (const v4i *)(S.Xl + (size_t)(cbase + (tc + 8) * UT) * KA);
+#pragma unroll
+                for (uint32_t j = 0; j < NV; j++)
+                    if (NCH % 512 == 0 || tid + j * 512 < NCH) rb[j] = gb[tid + j * 512];
+            }
+            uint64_t cv[2][4];
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t c = wc + 16 * b + (lane & 15);
+                    cv[b][r] = (mrow[r] && c < nc) ? mrow[r][cb + c] : 0;
+                }
+            v4i acc[NC][2];
+#pragma unroll
+            for (int s = 0; s < NC; s++) acc[s][0] = acc[s][1] = v4i{0, 0, 0, 0};
+            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+            // epilogue: C - sum_s 256^s P_s (mod p)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t c = wc + 16 * b + (lane & 15);
+                    if (!mrow[r] || c >= nc) continue;
+                    int64_t w[NW];
+#pragma unroll
+                    for (int q = 0; q < NW; q++) w[q] = 0;
+#pragma unroll
+                    for (int s = 0; s < NC; s++) w[s >> 2] += (int64_t)acc[s][b][r] << (8 * (s & 3));
+                    const uint64_t vr = fold_words<MODE, NW>(f, lp, w);
+                    mrow[r][cb + c] = sub_mod(add_mod(cv[b][r], lp.k0, p), vr, p);
+                }
+        }
+    }
+}
+
 // Launchers: the digit gather and the two parts of the MFMA update.
 template <int L, int MODE>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
@@ -363,9 +466,12 @@ static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const F
     if (what < 0)
         hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(rl.grid_gx), dim3(512), 0, st, dmats, nmat, rl.gx_off,
                            rl.plan, f, lp, par);
+    else if (what == 0)
+        hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(rl.grid_near), dim3(512), 0, st, dmats, nmat, rl.near_off,
+                           rl.plan, f, lp, par, 0u);
     else
-        hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(what ? rl.grid_far : rl.grid_near), dim3(512), 0, st, dmats,
-                           nmat, what ? rl.far_off : rl.near_off, rl.plan, f, lp, par, (uint32_t)what);
+        hipLaunchKernelGGL((k_update_far<L, MODE>), dim3(rl.grid_far), dim3(512), 0, st, dmats, nmat, rl.far_off,
+                           rl.plan, f, lp, par);
 }
 
 // what = -1: digit gather and X product; 0: near update; 1: far update.

@@ -83,6 +83,7 @@ struct RankLaunch {
     uint32_t *gx_off, *near_off, *far_off;  // nmat + 1 offsets each
     RankPlan *plan;
     uint32_t grid_gx, grid_near, grid_far;
+    uint32_t far_items;                     // far work counted as (row band, XCD group) items (MFMA)
 };
 
 // Constants of the signed-digit limb representation used by the MFMA update.

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(512) void k_block_z(RankMat *mats, FieldParams f, u
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *near_off,
-                                              uint32_t *far_off, RankPlan *plan, uint32_t par) {
+                                              uint32_t *far_off, RankPlan *plan, uint32_t par, uint32_t far_items) {
     __shared__ uint32_t sg[256], sn[256], sf[256];
     __shared__ uint32_t base_g, base_n, base_f;
     const uint32_t tid = threadIdx.x;
@@ -289,7 +289,8 @@ __global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint
                     update_range(I, 0, base, wn);
                     un = part_tiles(O, wn, 0);
                     update_range(I, 1, base, wf);
-                    uf = part_tiles(O, wf, 1);
+                    // far work: XCD-numbered tiles (VALU), or (row band, XCD group) items (MFMA)
+                    uf = far_items ? (wf > 0 ? cdiv(O, UT) * 8 : 0) : part_tiles(O, wf, 1);
                     if (I.pb > 0) {
                         // The updates' work: O x cols x pb products; per tile 16 B per
                         // element and 8 B per operand element, summed over the tiles of
@@ -538,7 +539,7 @@ void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldPara
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.plan,
-                       par);
+                       par, rl.far_items);
 }
 
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par) {

@@ -30,6 +30,7 @@ METHOD_AUTO, METHOD_PLAIN, METHOD_MFMA = 0, 1, 2
 ELEM_U64, ELEM_U32 = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 FLAG_PROFILE = 1
+FLAG_NO_GRAPH = 2
 P_FLOOR = 1e-307  # stats.rs:16
 
 PRNG_XOROSHIRO128PP, PRNG_MIXMAX17, PRNG_MIXMAX240, PRNG_MIXMAX256, PRNG_SPLITMIX64, PRNG_XOSHIRO256SS = range(6)
@@ -64,7 +65,7 @@ class _Opts(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("macs", "update_ms", "update_launches", "limb_count", "iterations", "bm_bytes", "bm_ms",
-                 "total_ms", "update_bytes")]
+                 "total_ms", "update_bytes", "graph")]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
@@ -233,10 +234,10 @@ class Context:
         return arr, keep
 
     def rank_batch(self, field: Field, mats: Sequence, method: int = METHOD_AUTO, panel: int = 0,
-                   profile: bool = False) -> list[int]:
+                   profile: bool = False, graph: bool = True) -> list[int]:
         arr, keep = self._descs(mats, True)
         out = np.zeros(max(1, len(mats)), dtype=np.uint64)
-        opts = _Opts(method, panel, FLAG_PROFILE if profile else 0, 0)
+        opts = _Opts(method, panel, (FLAG_PROFILE if profile else 0) | (0 if graph else FLAG_NO_GRAPH), 0)
         self._check(self._lib.modlin_rank_batch(self._h, ctypes.byref(field._f), arr, len(mats), ctypes.byref(opts),
                                                 _u64p(out)), "rank_batch")
         return [int(x) for x in out[:len(mats)]]

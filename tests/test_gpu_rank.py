@@ -157,3 +157,21 @@ def test_readme_c1_both_methods(ctx):
     for method in METHODS:
         r = _rank_all(ctx, M61, [mm, ctl], method)
         assert [500 - r[0], 500 - r[1]] == [432, 0]
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_graph_replay_matches_direct(ctx, method):
+    """The captured hipGraph of the iteration block, replayed (and re-captured when
+    the batch changes), gives the same ranks as direct launches and the oracle."""
+    rng = np.random.default_rng(3)
+    batches = [workloads.c2(count=8, max_side=600),
+               [rng.integers(0, M61, size=(n, n), dtype=np.uint64) for n in (300, 129, 64)],
+               workloads.c2(count=8, max_side=600)]
+    f = modlin.Field(M61)
+    for mats in batches:
+        want = [oracle.rank(M61, a, a.shape[0]) for a in mats]
+        assert ctx.rank_batch(f, mats, method=method, graph=False) == want
+        assert ctx.stats()["graph"] == 0.0
+        for _ in range(2):
+            assert ctx.rank_batch(f, mats, method=method) == want
+            assert ctx.stats()["graph"] == 1.0
