@@ -18,6 +18,7 @@
  *   modlin_lc_left_tail_pvalue stats::lc_left_tail_pvalue    src/stats.rs:101-126
  *   modlin_pretty_p_value     stats::pretty_p_value          src/stats.rs:129-137
  *   modlin_prng_fill          Prng::try_new + next_u64       src/prng.rs:46-343
+ *   modlin_prng_fill_device   the fill loops, on the device   main.rs:94-105, 153-155
  *
  * Ownership: the caller owns every buffer.  modlin_rank leaves m unspecified
  * (the reference also destroys it, holding partial factors).  A context owns
@@ -146,6 +147,17 @@ enum modlin_prng {
  * Returns MODLIN_EINVAL for a zero MIXMAX seed (prng.rs:153-155). */
 int modlin_prng_fill(uint32_t kind, uint64_t seed, uint64_t skip, uint64_t *out, uint64_t count);
 const char *modlin_prng_name(uint32_t kind);
+/* The fill loops of main.rs:94-105 and main.rs:153-155 on the device: writes
+ * out[i] = Field::reduce(next_u64()) for the outputs skip .. skip+count-1 of the
+ * stream (kind, seed), straight into device memory `out` (u64, or u32 when
+ * p < 2^32), bit-identical to modlin_prng_fill.  f == NULL writes the raw
+ * next_u64() words (u64 only).  keep > 0 decimates the stream first: keep
+ * outputs, drop `drop` (the mixmax-lux feature uses 5 and 9, prng.rs:272-279).
+ * Positions are reached by exact jump-ahead, so the cost does not grow with
+ * skip.  Runs on the context's stream and returns when the fill is complete.
+ * Returns MODLIN_EINVAL for a zero MIXMAX seed (prng.rs:153-155) or bad sizes. */
+int modlin_prng_fill_device(modlin_ctx_t *ctx, uint32_t kind, uint64_t seed, uint64_t skip, uint32_t keep,
+                            uint32_t drop, const modlin_field_t *f, uint32_t elem, void *out, uint64_t count);
 
 /* --- Workload helpers (host; the fill side of main.rs:86-105) ---------------- */
 /* out[i][c] = sum_k coef[i][k] * rows[k][c] mod p, for i < nout, k < nsrc, c < n

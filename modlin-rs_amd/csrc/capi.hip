@@ -86,6 +86,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (ctx->aux) hipStreamSynchronize(ctx->aux);
     if (ctx->arena) hipFree(ctx->arena);
     mdl::release_pinned(ctx);
+    mdl::release_generators(ctx);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->evA) hipEventDestroy(ctx->evA);
@@ -240,6 +241,12 @@ int modlin_prng_fill(uint32_t kind, uint64_t seed, uint64_t skip, uint64_t *out,
         case MODLIN_PRNG_MIXMAX256: { MixMax256 g; if (!g.seed(seed)) return MODLIN_EINVAL; run(g); return MODLIN_OK; }
         default: return MODLIN_EINVAL;
     }
+}
+
+int modlin_prng_fill_device(modlin_ctx_t *ctx, uint32_t kind, uint64_t seed, uint64_t skip, uint32_t keep,
+                            uint32_t drop, const modlin_field_t *f, uint32_t elem, void *out, uint64_t count) {
+    if (!ctx) return MODLIN_EINVAL;
+    return mdl::prng_fill_device(ctx, kind, seed, skip, keep, drop, f, elem, out, count);
 }
 
 int modlin_dev_alloc(modlin_ctx_t *ctx, uint64_t bytes, void **out) {

@@ -10,8 +10,10 @@
 // The reference selects its generator when it is compiled (Cargo features,
 // prng.rs:1-45); here -G/--generator selects it at run time (default: mixmax,
 // the N = 240 generator) and --lux adds the keep-5/drop-9 decimation
-// (prng.rs:246-279).  Progress goes to stderr; stdout carries exactly the
-// reference's lines.
+// (prng.rs:246-279).  Matrix and sequence entries are generated on the GPU
+// (modlin_prng_fill_device, exact jump-ahead to each group's first output) and
+// never cross PCIe; --host-gen generates them on the host instead.  Progress
+// goes to stderr; stdout carries exactly the reference's lines.
 #include <ctype.h>
 #include <inttypes.h>
 #include <stdio.h>
@@ -52,7 +54,8 @@ void help() {
         "[default: mixmax]\n"
         "      --lux\n          MIXMAX decimation: keep 5 outputs, drop 9.\n"
         "      --method <auto|plain|mfma>\n          Trailing-update method of the rank test. [default: auto]\n"
-        "      --batch-gib <GIB>\n          Host memory per batched call. [default: 8]\n"
+        "      --batch-gib <GIB>\n          Memory per batched call. [default: 8]\n"
+        "      --host-gen\n          Generate the entries on the host and copy them to the GPU.\n"
         "  -h, --help\n          Print help\n",
         kUsage);
 }
@@ -160,11 +163,12 @@ struct SourceOf : Source {
 struct Picked {
     Source *src;
     std::string name;
+    uint32_t kind;
 };
 
 Picked make_source(const std::string &gen, bool lux, uint64_t seed) {
     using namespace modlin::prng;
-    Picked pk{nullptr, ""};
+    Picked pk{nullptr, "", MODLIN_PRNG_XOROSHIRO128PP};
     const char *err = "the zero seed is not allowed for MIXMAX";  // prng.rs:153-155
     if (gen == "xoroshiro128pp") {
         pk.src = new SourceOf<Xoroshiro128pp>(Xoroshiro128pp(seed));
@@ -178,6 +182,7 @@ Picked make_source(const std::string &gen, bool lux, uint64_t seed) {
     else if (gen == "mixmax256") kind = MODLIN_PRNG_MIXMAX256;
     else die("unknown generator '" + gen + "' (xoroshiro128pp, mixmax, mixmax17, mixmax256)");
     pk.name = modlin_prng_name(kind);
+    pk.kind = kind;
     if (lux) pk.name += ", lux: keep 5 drop 9";
     bool ok = false;
     if (kind == MODLIN_PRNG_MIXMAX240) {
@@ -206,7 +211,7 @@ Picked make_source(const std::string &gen, bool lux, uint64_t seed) {
 
 int main(int argc, char **argv) {
     uint64_t side = 0, length = 0, reps = 1, p = 0, seed = 0;
-    bool have_r = false, have_l = false, have_p = false, have_reps = false, lux = false;
+    bool have_r = false, have_l = false, have_p = false, have_reps = false, lux = false, host_gen = false;
     std::string gen = "mixmax", method = "auto";
     double batch_gib = 8.0;
     auto need = [&](int &i, const char *flag) -> const char * {
@@ -251,6 +256,8 @@ int main(int argc, char **argv) {
             lux = true;
         } else if (a == "--method") {
             method = value("--method <METHOD>");
+        } else if (a == "--host-gen") {
+            host_gen = true;
         } else if (a == "--batch-gib") {
             batch_gib = atof(value("--batch-gib <GIB>").c_str());
         } else if (!a.empty() && a[0] == '-' && a != "-") {
@@ -315,20 +322,43 @@ int main(int argc, char **argv) {
     if (group > reps) group = reps;
     std::vector<uint64_t> data;
     std::vector<uint64_t> res(group);
+    void *dev = nullptr;  // device-generated entries of one group
+    if (!host_gen && (rc = modlin_dev_alloc(ctx, group * item * 8, &dev)) != MODLIN_OK) {
+        fprintf(stderr, "Error: %s (status %d)\n", modlin_ctx_last_error(ctx), rc);
+        modlin_ctx_destroy(ctx);
+        return 1;
+    }
     for (uint64_t first = 0; first < reps; first += group) {
         const uint64_t cnt = std::min<uint64_t>(group, reps - first);
-        data.resize(cnt * item);
-        for (uint64_t i = 0; i < cnt * item; i++) data[i] = pk.src->next() % p;  // field.reduce (main.rs:96, 154)
+        uint64_t *base;
+        uint32_t mem;
+        if (host_gen) {
+            data.resize(cnt * item);
+            for (uint64_t i = 0; i < cnt * item; i++) data[i] = pk.src->next() % p;  // field.reduce (main.rs:96, 154)
+            base = data.data();
+            mem = MODLIN_MEM_HOST;
+        } else {
+            // the same successive outputs, from output first * item on (main.rs:94-105, 153-155)
+            rc = modlin_prng_fill_device(ctx, pk.kind, seed, first * item, lux ? 5 : 0, lux ? 9 : 0, &field,
+                                         MODLIN_ELEM_U64, dev, cnt * item);
+            if (rc != MODLIN_OK) {
+                fprintf(stderr, "Error: %s (status %d)\n", modlin_ctx_last_error(ctx), rc);
+                modlin_ctx_destroy(ctx);
+                return 1;
+            }
+            base = (uint64_t *)dev;
+            mem = MODLIN_MEM_DEVICE;
+        }
         fprintf(stderr, "%s %" PRIu64 "-%" PRIu64 "/%" PRIu64 " on the GPU...\n", have_r ? "Matrices" : "Sequences",
                 first + 1, first + cnt, reps);
         if (have_r) {
             std::vector<modlin_mat_desc_t> d(cnt);
-            for (uint64_t i = 0; i < cnt; i++) d[i] = {data.data() + i * item, n, MODLIN_ELEM_U64, MODLIN_MEM_HOST};
+            for (uint64_t i = 0; i < cnt; i++) d[i] = {base + i * item, n, MODLIN_ELEM_U64, mem};
             modlin_opts_t o{m, 0, 0, 0};
             rc = modlin_rank_batch(ctx, &field, d.data(), cnt, &o, res.data());
         } else {
             std::vector<modlin_seq_desc_t> d(cnt);
-            for (uint64_t i = 0; i < cnt; i++) d[i] = {data.data() + i * item, n, MODLIN_ELEM_U64, MODLIN_MEM_HOST};
+            for (uint64_t i = 0; i < cnt; i++) d[i] = {base + i * item, n, MODLIN_ELEM_U64, mem};
             rc = modlin_lc_batch(ctx, &field, d.data(), cnt, nullptr, res.data());
         }
         if (rc != MODLIN_OK) {
@@ -350,6 +380,7 @@ int main(int argc, char **argv) {
         }
         fflush(stdout);
     }
+    if (dev) modlin_dev_free(ctx, dev);
     modlin_ctx_destroy(ctx);
     delete pk.src;
     return 0;

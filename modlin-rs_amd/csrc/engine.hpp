@@ -30,6 +30,10 @@ struct modlin_ctx {
     void *pinned[2] = {nullptr, nullptr};
     size_t pinned_bytes = 0;
     hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+    // device generators: tabulated jump powers per generator kind, and a workspace
+    void *gen_tab[8] = {};
+    void *gen_ws = nullptr;
+    size_t gen_ws_bytes = 0;
 };
 
 namespace mdl {
@@ -59,6 +63,11 @@ void release_pinned(modlin_ctx *ctx);
 
 int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint64_t count, const modlin_opts_t *opts,
                uint64_t *ranks, modlin_progress_fn progress, void *user);
+// Device fill of a generator stream (prng_dev.hip); see modlin_prng_fill_device.
+int prng_fill_device(modlin_ctx *ctx, uint32_t kind, uint64_t seed, uint64_t skip, uint32_t keep, uint32_t drop,
+                     const modlin_field_t *field, uint32_t elem, void *out, uint64_t count);
+void release_generators(modlin_ctx *ctx);
+
 int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_t count, const modlin_opts_t *opts,
              uint64_t *lcs);
 

@@ -1,22 +1,31 @@
 // Input generators of modlin (src/prng.rs) plus the two published generators the
-// benchmark workloads use (SplitMix64, xoshiro256**).  Host code: the matrices
-// and sequences are filled outside the timed region, as in main.rs:86-105.
+// benchmark workloads use (SplitMix64, xoshiro256**).  The state transitions are
+// __host__ __device__: the host generators below and the device fill kernels
+// (prng_dev.hip) run the same operations word for word.
 #pragma once
 #include <stdint.h>
 
 #include <array>
 #include <vector>
 
+#include "fp_dev.hpp"
+
+#if defined(__clang__)
+#define MDL_PRAGMA(x) _Pragma(#x)
+#else
+#define MDL_PRAGMA(x)
+#endif
+
 namespace modlin {
 namespace prng {
 
-inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+MDL_HD uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
 // SplitMix64 (public-domain splitmix64.c): add the golden gamma, then finalise.
 struct SplitMix64 {
     uint64_t x;
-    explicit SplitMix64(uint64_t seed) : x(seed) {}
-    uint64_t next_u64() {
+    MDL_HD explicit SplitMix64(uint64_t seed) : x(seed) {}
+    MDL_HD uint64_t next_u64() {
         uint64_t z = (x += 0x9e3779b97f4a7c15ull);
         z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
         z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
@@ -28,12 +37,13 @@ struct SplitMix64 {
 struct Xoroshiro128pp {
     static constexpr const char *NAME = "xoroshiro128++";
     uint64_t s0, s1;
-    explicit Xoroshiro128pp(uint64_t seed) {
+    MDL_HD Xoroshiro128pp() : s0(0), s1(0) {}
+    MDL_HD explicit Xoroshiro128pp(uint64_t seed) {
         SplitMix64 z(seed);
         s0 = z.next_u64();
         s1 = z.next_u64();
     }
-    uint64_t next_u64() {
+    MDL_HD uint64_t next_u64() {
         const uint64_t a = s0;
         uint64_t b = s1;
         const uint64_t r = rotl(a + b, 17) + a;
@@ -47,11 +57,12 @@ struct Xoroshiro128pp {
 // xoshiro256** (public-domain xoshiro256starstar.c), state from four SplitMix64 draws.
 struct Xoshiro256ss {
     uint64_t s[4];
-    explicit Xoshiro256ss(uint64_t seed) {
+    MDL_HD Xoshiro256ss() : s{0, 0, 0, 0} {}
+    MDL_HD explicit Xoshiro256ss(uint64_t seed) {
         SplitMix64 z(seed);
-        for (auto &w : s) w = z.next_u64();
+        for (int i = 0; i < 4; i++) s[i] = z.next_u64();
     }
-    uint64_t next_u64() {
+    MDL_HD uint64_t next_u64() {
         const uint64_t r = rotl(s[1] * 5, 7) * 9;
         const uint64_t t = s[1] << 17;
         s[2] ^= s[0];
@@ -64,15 +75,67 @@ struct Xoshiro256ss {
     }
 };
 
-// MIXMAX over 2^61 - 1: the matrix generator of CERN ROOT's MixMaxEngine, as
-// in prng.rs:92-280.  N is the dimension, SPECIALMUL/SPECIAL the per-N matrix
-// tweak (SPECIAL = UINT64_MAX stands for ROOT's -1), SKIP the extra iterations
-// before each re-iteration.
+// The MIXMAX state transition over 2^61 - 1 (ROOT's iterate_raw_vec,
+// prng.rs:191-222), on the words v[0], v[S], ..., v[(N-1) S].  N is the
+// dimension, SPECIALMUL/SPECIAL the per-N matrix tweak (SPECIAL = UINT64_MAX
+// stands for ROOT's -1).  Values are kept in the reference's folded form, which
+// is canonical except that a few residues (0, 1, 2) may be held as p, p + 1 or
+// p + 2; the device fill kernels rely on exactly this property when they start
+// a stream from a jumped (canonical) state.
+template <int N, int SPECIALMUL, uint64_t SPECIAL>
+struct MixMaxMap {
+    static constexpr uint64_t P = 2305843009213693951ull;  // 2^61 - 1
+    static constexpr int BITS = 61;
+    MDL_HD static uint64_t fold(uint64_t k) { return (k & P) + (k >> BITS); }
+    MDL_HD static uint64_t addm(uint64_t a, uint64_t b) { return fold(a + b); }
+    MDL_HD static uint64_t fmul(uint64_t a, uint64_t b) {  // fmodmul_m61(0, a, b)
+        uint64_t hi, lo;
+        mdl::mul_wide(a, b, hi, lo);
+        return fold((lo & P) + hi * 8 + (lo >> BITS));
+    }
+    // Applies the matrix in place and returns the new sumtot.
+    template <int S>
+    MDL_HD static uint64_t iterate(uint64_t *v, uint64_t sumtot_in) {
+        const uint64_t temp2 = v[S];
+        uint64_t tv = sumtot_in;
+        v[0] = tv;
+        uint64_t sum = tv, ovf = 0, tp = 0;
+        constexpr int UN = N <= 32 ? N : 4;
+        MDL_PRAGMA(unroll UN)
+        for (int i = 1; i < N; i++) {
+            if (SPECIALMUL != 0) {
+                const uint64_t tpo = ((tp << SPECIALMUL) & P) | (tp >> (BITS - SPECIALMUL));
+                tp = addm(tp, v[i * S]);
+                tv = fold(tv + tp + tpo);
+            } else {
+                tp = addm(tp, v[i * S]);
+                tv = addm(tv, tp);
+            }
+            v[i * S] = tv;
+            const uint64_t s = sum + tv;
+            ovf += s < sum;
+            sum = s;
+        }
+        if (SPECIAL != 0) {
+            const uint64_t t2 = SPECIAL == UINT64_MAX ? P - temp2 : fmul(SPECIAL, temp2);
+            v[2 * S] = addm(v[2 * S], t2);
+            const uint64_t s = sum + t2;
+            ovf += s < sum;
+            sum = s;
+        }
+        return fold(fold(sum) + (ovf << 3));
+    }
+};
+
+// The MIXMAX generator of CERN ROOT's MixMaxEngine, as in prng.rs:92-280.  SKIP
+// is the number of extra iterations before each re-iteration.
 template <int N, int SPECIALMUL, uint64_t SPECIAL, int SKIP>
 class MixMax {
   public:
-    static constexpr uint64_t P = 2305843009213693951ull;  // 2^61 - 1
-    static constexpr int BITS = 61;
+    using Map = MixMaxMap<N, SPECIALMUL, SPECIAL>;
+    static constexpr uint64_t P = Map::P;
+    static constexpr int DIM = N, SPECIAL_MUL = SPECIALMUL, SKIP_ITERS = SKIP;
+    static constexpr uint64_t SPECIAL_ENTRY = SPECIAL;
 
     // Returns false for the zero seed (prng.rs:153-155).
     bool seed(uint64_t sd) {
@@ -86,7 +149,7 @@ class MixMax {
             ovf += s < sum;
             sum = s;
         }
-        sumtot_ = fold(fold(sum) + (ovf << 3));
+        sumtot_ = Map::fold(Map::fold(sum) + (ovf << 3));
         counter_ = N;
         return true;
     }
@@ -102,43 +165,12 @@ class MixMax {
         return v_[1];
     }
     uint64_t next_u64() { return next_raw() << 3; }  // prng.rs:268
+    // The seeded state, before the first draw (words as the reference holds them).
+    const uint64_t *state() const { return v_.data(); }
+    uint64_t sumtot() const { return sumtot_; }
 
   private:
-    static uint64_t fold(uint64_t k) { return (k & P) + (k >> BITS); }
-    static uint64_t addm(uint64_t a, uint64_t b) { return fold(a + b); }
-    static uint64_t fmul(uint64_t cum, uint64_t a, uint64_t b) {
-        const unsigned __int128 s = (unsigned __int128)a * b + cum;
-        const uint64_t lo = (uint64_t)s, hi = (uint64_t)(s >> 64);
-        return fold((lo & P) + hi * 8 + (lo >> BITS));
-    }
-    void iterate() {  // prng.rs:191-222
-        const uint64_t temp2 = v_[1];
-        uint64_t tv = sumtot_;
-        v_[0] = tv;
-        uint64_t sum = tv, ovf = 0, tp = 0;
-        for (int i = 1; i < N; i++) {
-            if (SPECIALMUL != 0) {
-                const uint64_t tpo = ((tp << SPECIALMUL) & P) | (tp >> (BITS - SPECIALMUL));
-                tp = addm(tp, v_[i]);
-                tv = fold(tv + tp + tpo);
-            } else {
-                tp = addm(tp, v_[i]);
-                tv = addm(tv, tp);
-            }
-            v_[i] = tv;
-            const uint64_t s = sum + tv;
-            ovf += s < sum;
-            sum = s;
-        }
-        if (SPECIAL != 0) {
-            const uint64_t t2 = SPECIAL == UINT64_MAX ? P - temp2 : fmul(0, SPECIAL, temp2);
-            v_[2] = addm(v_[2], t2);
-            const uint64_t s = sum + t2;
-            ovf += s < sum;
-            sum = s;
-        }
-        sumtot_ = fold(fold(sum) + (ovf << 3));
-    }
+    void iterate() { sumtot_ = Map::template iterate<1>(v_.data(), sumtot_); }
     std::array<uint64_t, N> v_{};
     uint64_t sumtot_ = 0;
     int counter_ = N;

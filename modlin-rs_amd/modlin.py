@@ -41,7 +41,7 @@ EXPORTS = [
     "modlin_ctx_create", "modlin_ctx_destroy", "modlin_ctx_last_error", "modlin_ctx_last_stats",
     "modlin_rank", "modlin_linear_complexity", "modlin_rank_batch", "modlin_lc_batch",
     "modlin_corank_prob", "modlin_corank_tail_pvalue", "modlin_lc_left_tail_pvalue", "modlin_pretty_p_value",
-    "modlin_prng_fill", "modlin_prng_name",
+    "modlin_prng_fill", "modlin_prng_name", "modlin_prng_fill_device",
     "modlin_util_combine_rows", "modlin_util_lfsr",
     "modlin_dev_alloc", "modlin_dev_free", "modlin_memcpy_h2d", "modlin_memcpy_d2h", "modlin_synchronize",
 ]
@@ -107,6 +107,8 @@ def load() -> ctypes.CDLL:
     lib.modlin_prng_fill.argtypes = [ctypes.c_uint32, u64, u64, P64, u64]
     lib.modlin_prng_name.argtypes = [ctypes.c_uint32]
     lib.modlin_prng_name.restype = ctypes.c_char_p
+    lib.modlin_prng_fill_device.argtypes = [vp, ctypes.c_uint32, u64, u64, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.POINTER(_FieldT), ctypes.c_uint32, vp, u64]
     lib.modlin_util_combine_rows.argtypes = [u64, P64, u64, u64, P64, u64, P64]
     lib.modlin_util_lfsr.argtypes = [u64, P64, u64, P64, u64, P64]
     lib.modlin_dev_alloc.argtypes = [vp, u64, ctypes.POINTER(vp)]
@@ -261,6 +263,24 @@ class Context:
         n = int(round(a.size ** 0.5)) if square else a.size
         return DeviceBuffer(self, ptr.value, n, ELEM_U32 if a.dtype == np.uint32 else ELEM_U64, a.nbytes)
 
+    def prng_fill_device(self, kind: int, seed: int, count: int, skip: int = 0, field: "Field | None" = None,
+                         elem: int = ELEM_U64, keep: int = 0, drop: int = 0, square: bool = True) -> "DeviceBuffer":
+        """Outputs skip .. skip+count-1 of a generator stream, reduced by `field`
+        (raw next_u64() words when None), generated on the device
+        (main.rs:94-105 fill loop; prng.rs:46-343)."""
+        nbytes = count * (4 if elem == ELEM_U32 else 8)
+        ptr = ctypes.c_void_p()
+        self._check(self._lib.modlin_dev_alloc(self._h, nbytes, ctypes.byref(ptr)), "dev_alloc")
+        fp = ctypes.byref(field._f) if field is not None else None
+        rc = self._lib.modlin_prng_fill_device(self._h, kind, seed, skip, keep, drop, fp, elem, ptr, count)
+        if rc != MODLIN_OK:
+            self._lib.modlin_dev_free(self._h, ptr)
+            self._check(rc, "prng_fill_device")
+        n = int(round(count ** 0.5)) if square else count
+        if square and n * n != count:
+            n = count
+        return DeviceBuffer(self, ptr.value, n, elem, nbytes)
+
     def synchronize(self):
         self._check(self._lib.modlin_synchronize(self._h), "synchronize")
 
@@ -270,6 +290,13 @@ class DeviceBuffer:
 
     def __init__(self, ctx: Context, ptr: int, n: int, elem: int, nbytes: int):
         self.ctx, self.ptr, self.n, self.elem, self.nbytes = ctx, ptr, n, elem, nbytes
+
+    def to_host(self) -> np.ndarray:
+        a = np.empty(self.nbytes // (4 if self.elem == ELEM_U32 else 8),
+                     dtype=np.uint32 if self.elem == ELEM_U32 else np.uint64)
+        self.ctx._check(self.ctx._lib.modlin_memcpy_d2h(self.ctx._h, a.ctypes.data, self.ptr, self.nbytes),
+                        "memcpy_d2h")
+        return a
 
     def free(self):
         if self.ptr:

@@ -36,3 +36,18 @@ def test_reps_use_disjoint_stretches():
         corank = n - oracle.rank(p, data[i * n * n:(i + 1) * n * n], n)
         want.append(f"Matrix {i + 1}/{reps}\tcorank={corank}\tp={oracle.pretty_p_value(oracle.corank_tail_pvalue(p, n, corank))}")
     assert r.stdout.splitlines()[3:] == want
+
+
+@pytest.mark.parametrize("args", [
+    ["-G", "mixmax", "--lux", "-R", "300", "4", "-p", "1000000007", "-S", "9"],
+    ["-G", "xoroshiro128pp", "-R", "257", "3", "-p", "65521", "-S", "0x1f"],
+    ["-G", "mixmax256", "-L", "3000", "5", "-p", "2305843009213693951", "-S", "2"],
+    ["-G", "mixmax17", "-R", "600", "2", "-p", "2147483647", "-S", "1", "--batch-gib", "0.001"],
+])
+def test_device_generation_matches_host(args):
+    # the default device fill and --host-gen must print the same lines
+    print("cli", args, flush=True)
+    a = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    b = subprocess.run([CLI, *args, "--host-gen"], capture_output=True, text=True, timeout=120)
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    assert a.stdout == b.stdout
