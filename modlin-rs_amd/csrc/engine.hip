@@ -5,6 +5,8 @@
 // stream), plan, gather, near update, finish (main stream), then the far update
 // (main stream) -- which overlaps the next iteration's block elimination: the
 // next panel's columns lie in the near part, already updated.
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -292,7 +294,15 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     rl.grid_near = (uint32_t)ctx->cu_count * 2;
     rl.grid_far = (uint32_t)ctx->cu_count * (use_mfma ? 2 : 8);  // multiples of 8 (XCD numbering)
     rl.far_items = use_mfma ? 1u : 0u;
-    const int check_every = 8;
+    int check_every = 8;
+    if (const char *env = getenv("MODLIN_TUNE")) {  // "near=N,far=N,gx=N,every=N" (experiments)
+        unsigned v;
+        const char *q;
+        if ((q = strstr(env, "near=")) && sscanf(q + 5, "%u", &v) == 1 && v) rl.grid_near = (v + 7) / 8 * 8;
+        if ((q = strstr(env, "far=")) && sscanf(q + 4, "%u", &v) == 1 && v) rl.grid_far = (v + 7) / 8 * 8;
+        if ((q = strstr(env, "gx=")) && sscanf(q + 3, "%u", &v) == 1 && v) rl.grid_gx = v;
+        if ((q = strstr(env, "every=")) && sscanf(q + 6, "%u", &v) == 1 && v) check_every = (int)((v + 1) / 2 * 2);
+    }
     std::vector<hipEvent_t> evs;
     if (profile) {
         evs.resize(4 * check_every);
@@ -340,8 +350,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     bool use_graph = !profile && !(opts && (opts->flags & MODLIN_FLAG_NO_GRAPH));
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof key, "%p/%u/%u/%d/%llu/%u/%u/%u/%u", (void *)base, nmat, b, (int)use_mfma,
-                 (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L);
+        snprintf(key, sizeof key, "%p/%u/%u/%d/%llu/%u/%u/%u/%u/%d", (void *)base, nmat, b, (int)use_mfma,
+                 (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L, check_every);
         if (!ctx->rank_graph || ctx->rank_graph_key != key) {
             if (ctx->rank_graph) {
                 hipGraphExecDestroy(ctx->rank_graph);

@@ -234,6 +234,23 @@ MDL_HD uint64_t pow_mod(const FieldParams &f, uint64_t a, uint64_t e) {  // fp.r
 
 MDL_HD uint64_t inv_mod(const FieldParams &f, uint64_t a) { return pow_mod(f, a, f.p - 2); }  // fp.rs:111-113
 
+// a^(p-2) = 1 / a for a in [1, p) (Fermat, fp.rs:111-113) by left-to-right
+// square-and-multiply over the bits of p - 2.  The exponent is the same in every
+// lane, so a wavefront inverts 64 values in lockstep (no divergence).
+template <int MODE>
+MDL_HD uint64_t inv_fermat_t(const FieldParams &f, uint64_t a) {
+    const uint64_t e = f.p - 2;
+    if (e == 0) return 1;  // p = 2
+    int b = 63;
+    while (!((e >> b) & 1)) b--;
+    uint64_t r = a;
+    for (b--; b >= 0; b--) {
+        r = mul_mod_t<MODE>(f, r, r);
+        if ((e >> b) & 1) r = mul_mod_t<MODE>(f, r, a);
+    }
+    return r;
+}
+
 // Inverse of a in [1, p) by the binary extended Euclidean algorithm (Knuth,
 // TAOCP vol. 2, 4.5.2): about 2 bitlen(p) short steps instead of the ~2 bitlen(p)
 // dependent Barrett products of Fermat.  Any exact inverse is bit-identical to

@@ -100,7 +100,7 @@ struct MixMaxMap {
         uint64_t tv = sumtot_in;
         v[0] = tv;
         uint64_t sum = tv, ovf = 0, tp = 0;
-        constexpr int UN = N <= 32 ? N : 4;
+        [[maybe_unused]] constexpr int UN = N <= 32 ? N : 4;
         MDL_PRAGMA(unroll UN)
         for (int i = 1; i < N; i++) {
             if (SPECIALMUL != 0) {

@@ -66,16 +66,16 @@ struct RankPlan {
 // Record of one block elimination, written by k_block_ge and replayed by
 // k_block_z (u64 offsets into RankMat::Eta): the multiplier e of every block row
 // at every step, the pivot rows' entries U[r][s] in the pivot columns (upper
-// triangle), and per step the pivot, 1 / pivot, the pivot row's own-row
-// coefficient and the swapped position.
+// triangle), and per step the pivot, the pivot row's own-row coefficient and the
+// swapped position.
 constexpr uint32_t ETA_E = 0;                          // [step][row]
 constexpr uint32_t ETA_U = BMAX * BMAX;                // [pivot row][pivot slot]
 constexpr uint32_t ETA_PI = 2 * BMAX * BMAX;           // [step]
-constexpr uint32_t ETA_DINV = ETA_PI + BMAX;           // [step]
-constexpr uint32_t ETA_OWNK = ETA_DINV + BMAX;         // [step]
+constexpr uint32_t ETA_OWNK = ETA_PI + BMAX;           // [step]
 constexpr uint32_t ETA_SWQ = ETA_OWNK + BMAX;          // [step]
 constexpr uint32_t ETA_WORDS = ETA_SWQ + BMAX;
-constexpr uint32_t ZG = 8;                             // workgroups (of 8 waves) per matrix in k_block_z
+constexpr uint32_t ZG = 16;                            // workgroups per matrix in k_block_z
+constexpr uint32_t ZSLOTS = BMAX / ZG;                 // slot waves per workgroup (plus one inverse wave)
 
 
 // Device buffers and grid sizes of the per-iteration launches.

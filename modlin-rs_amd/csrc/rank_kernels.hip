@@ -131,9 +131,8 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         eta[ETA_E + idx] = Et[r][s];
         eta[ETA_U + idx] = (s > r && s < pb) ? U[r][Jl[s]] : 0;
     }
-    if (tid < pb) {
+    if (tid < pb) {  // (the pivots' inverses are computed by k_block_z, off this kernel's path)
         eta[ETA_PI + tid] = piK[tid];
-        eta[ETA_DINV + tid] = inv_mod_bin(piK[tid], p);
         eta[ETA_OWNK + tid] = ownK[tid];
         eta[ETA_SWQ + tid] = swq[tid];
     }
@@ -177,7 +176,11 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // -e own[s] and is then carried through the later steps (x <- pi x - e y, with
 // the recorded swaps).  Then Z = U^-1 F by back substitution.  Every slot s
 // (a column of F and of Z) is independent: one wave per slot, lane = block row,
-// with the step's pivot row broadcast from its lane -- no barriers.
+// with the step's pivot row broadcast from its lane -- no barriers.  The chain
+// is latency-bound, so the per-step scalars (pivot, swap, own coefficient) sit
+// in the lane registers of their step and are read with readlane.  One more
+// wave per workgroup inverts the pivots (Fermat, in lockstep) while the slot
+// waves run their forward chains; the back substitution waits for it.
 // ---------------------------------------------------------------------------
 // The value of lane `src` (wave-uniform) in every lane.
 __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
@@ -187,13 +190,12 @@ __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(512) void k_block_z(RankMat *mats, FieldParams f, uint32_t par) {
-    constexpr uint32_t WPG = 8;  // waves (slots) per workgroup; ZG * WPG = BMAX
+__global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, FieldParams f, uint32_t par) {
     RankMat &S = mats[blockIdx.x / ZG];
     if (S.done) return;
     const uint32_t pb = S.it[par].pb;
-    const uint32_t s0 = (blockIdx.x % ZG) * WPG;
-    if (s0 >= pb) return;
+    const uint32_t s0 = (blockIdx.x % ZG) * ZSLOTS;
+    if (s0 >= pb) return;  // uniform over the workgroup
     __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
     __shared__ uint64_t Ur[BMAX][BMAX];  // U[r][t], t > r
     __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
@@ -203,39 +205,48 @@ __global__ __launch_bounds__(512) void k_block_z(RankMat *mats, FieldParams f, u
     const uint64_t p = f.p;
     for (uint32_t idx = s0 * BMAX + tid; idx < pb * BMAX; idx += blockDim.x) Ek[idx / BMAX][idx % BMAX] = eta[ETA_E + idx];
     for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) Ur[idx / BMAX][idx % BMAX] = eta[ETA_U + idx];
-    if (tid < pb) {
-        piK[tid] = eta[ETA_PI + tid];
-        dinv[tid] = eta[ETA_DINV + tid];
-        ownK[tid] = eta[ETA_OWNK + tid];
-        swq[tid] = (uint32_t)eta[ETA_SWQ + tid];
+    if (tid < BMAX) {
+        piK[tid] = tid < pb ? eta[ETA_PI + tid] : 1;
+        ownK[tid] = tid < pb ? eta[ETA_OWNK + tid] : 0;
+        swq[tid] = tid < pb ? (uint32_t)eta[ETA_SWQ + tid] : tid;
     }
     __syncthreads();
     const uint32_t s = s0 + wave;
-    if (s >= pb) return;  // no barrier follows
-
-    // Forward: ev = F[lane][s].
-    uint64_t ev = 0;
-    uint64_t e_next = Ek[s][lane];
-    for (uint32_t k = s; k < pb; k++) {
-        const uint64_t e = e_next;
-        if (k + 1 < pb) e_next = Ek[k + 1][lane];
-        const uint32_t q = swq[k];
-        if (q != k) {
-            const uint64_t vk = lane_bcast(ev, k), vq = lane_bcast(ev, q);
-            ev = (lane == k) ? vq : (lane == q) ? vk : ev;
+    uint64_t fv = 0;
+    if (wave == ZSLOTS) {
+        // the inverse wave
+        dinv[lane] = lane < pb ? inv_fermat_t<MODE>(f, piK[lane]) : 0;
+    } else if (s < pb) {
+        // Forward: ev = F[lane][s].
+        const uint32_t my_q = swq[lane];
+        const uint64_t my_pi = piK[lane], my_own = ownK[lane];
+        uint64_t ev = 0;
+        uint64_t e_next = Ek[s][lane];
+        for (uint32_t k = s; k < pb; k++) {
+            const uint64_t e = e_next;
+            if (k + 1 < pb) e_next = Ek[k + 1][lane];
+            const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)my_q, (int)k);
+            if (q != k) {
+                const uint64_t vk = lane_bcast(ev, k), vq = lane_bcast(ev, q);
+                ev = (lane == k) ? vq : (lane == q) ? vk : ev;
+            }
+            const uint64_t yk = (k == s) ? lane_bcast(my_own, k) : lane_bcast(ev, k);
+            const uint64_t x = (k == s) ? 0 : ev;
+            const uint64_t pik = lane_bcast(my_pi, k);
+            if (lane > k && e != 0) ev = ff_step_t<MODE>(f, pik, x, e, yk);
         }
-        const uint64_t yk = (k == s) ? ownK[k] : lane_bcast(ev, k);
-        const uint64_t x = (k == s) ? 0 : ev;
-        if (lane > k && e != 0) ev = ff_step_t<MODE>(f, piK[k], x, e, yk);
+        fv = lane > s ? ev : (lane == s ? my_own : 0);
     }
-    const uint64_t fv = lane > s ? ev : (lane == s ? ownK[s] : 0);
+    __syncthreads();  // the inverses are in LDS
+    if (wave == ZSLOTS || s >= pb) return;
 
     // Back substitution, column-oriented: once z[t] is known every lane l < t adds
     // U[l][t] z[t] to its sum; lane t-1 is then complete.
+    const uint64_t my_dinv = dinv[lane];
     uint64_t acc = 0, z = 0;
     for (int t = (int)pb - 1; t >= 0; t--) {
         const uint64_t u = Ur[lane][t];
-        if (lane == (uint32_t)t) z = mul_mod_t<MODE>(f, dinv[t], sub_mod(fv, acc, p));
+        if (lane == (uint32_t)t) z = mul_mod_t<MODE>(f, my_dinv, sub_mod(fv, acc, p));
         const uint64_t zt = lane_bcast(z, (uint32_t)t);
         if (lane < (uint32_t)t) acc = add_mod(acc, mul_mod_t<MODE>(f, u, zt), p);
     }
@@ -527,13 +538,13 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par) {
     if (f.mode == MODE_M61) {
         hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
     } else if (f.mode == MODE_M31) {
         hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
     } else {
         hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(512), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
     }
 }
 
