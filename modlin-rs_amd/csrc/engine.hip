@@ -28,59 +28,6 @@ void launch_update_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const Fi
 void launch_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                  const RankLaunch &rl, uint32_t par, int what);
 
-// Signed 8-bit digits for the balanced residues |x| <= (p-1)/2.
-uint32_t limb_count(uint64_t p) {
-    typedef unsigned __int128 u128;
-    const u128 half = (p - 1) / 2;
-    u128 cap = 127, pw = 1;
-    uint32_t L = 1;
-    while (cap < half) {
-        pw *= 256;
-        cap += 127 * pw;
-        L++;
-    }
-    return L;
-}
-
-LimbParams make_limb_params(uint64_t p) {
-    typedef unsigned __int128 u128;
-    LimbParams lp{};
-    lp.L = limb_count(p);
-    u128 x = 1 % p;
-    for (int l = 0; l < 16; l++) {
-        lp.pow256[l] = (uint64_t)x;
-        x = (x * 256) % p;
-    }
-    const int nc = 2 * (int)lp.L - 1;
-    u128 k0 = 0;
-    for (int q = 0; 4 * q < nc; q++) k0 = (k0 + lp.pow256[4 * q]) % p;  // sum of 2^(32 q) mod p
-    lp.k0 = (uint64_t)((k0 * ((((u128)1) << 49) % p)) % p);
-    return lp;
-}
-
-FieldParams make_params(uint64_t p) {
-    typedef unsigned __int128 u128;
-    FieldParams f{};
-    f.p = p;
-    u128 mu = (~(u128)0) / p;
-    f.mu_hi = (uint64_t)(mu >> 64);
-    f.mu_lo = (uint64_t)mu;
-    uint32_t k = 64 - __builtin_clzll(p);
-    f.k = k;
-    f.big = (k == 63) ? 1u : 0u;
-    u128 muk = (((u128)1) << (2 * k)) / p;
-    u128 mup = muk << (63 - k);
-    f.mup = (mup >> 64) ? ~(uint64_t)0 : (uint64_t)mup;
-    f.r64 = (uint64_t)((((u128)1) << 64) % p);
-    u128 maxp = (u128)(p - 1) * (u128)(p - 1);
-    u128 sb = (~(u128)0) / (maxp ? maxp : 1);
-    if (sb > (u128)UINT64_MAX) sb = UINT64_MAX;
-    if (sb < 1) sb = 1;
-    f.safe_batch = (uint64_t)sb;
-    f.mode = p == P61 ? MODE_M61 : (p == P31 ? MODE_M31 : MODE_GENERIC);
-    return f;
-}
-
 int hip_fail(modlin_ctx *ctx, hipError_t e, const char *what) {
     ctx->err = std::string(what) + ": " + hipGetErrorString(e);
     return MODLIN_EHIP;

@@ -38,9 +38,6 @@ struct modlin_ctx {
 
 namespace mdl {
 
-// Host set-up of the device field parameters from p (Field::new, fp.rs:49-61).
-FieldParams make_params(uint64_t p);
-
 // Records a HIP failure on the context and returns MODLIN_EHIP.
 int hip_fail(modlin_ctx *ctx, hipError_t e, const char *what);
 

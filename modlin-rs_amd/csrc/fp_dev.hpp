@@ -43,6 +43,30 @@ struct FieldParams {
     uint32_t pad;
 };
 
+// Host set-up of the field parameters from p (Field::new, fp.rs:49-61).
+inline FieldParams make_params(uint64_t p) {
+    typedef unsigned __int128 u128;
+    FieldParams f{};
+    f.p = p;
+    const u128 mu = (~(u128)0) / p;
+    f.mu_hi = (uint64_t)(mu >> 64);
+    f.mu_lo = (uint64_t)mu;
+    const uint32_t k = 64 - __builtin_clzll(p);
+    f.k = k;
+    f.big = (k == 63) ? 1u : 0u;
+    const u128 muk = (((u128)1) << (2 * k)) / p;
+    const u128 mup = muk << (63 - k);
+    f.mup = (mup >> 64) ? ~(uint64_t)0 : (uint64_t)mup;
+    f.r64 = (uint64_t)((((u128)1) << 64) % p);
+    const u128 maxp = (u128)(p - 1) * (u128)(p - 1);
+    u128 sb = (~(u128)0) / (maxp ? maxp : 1);
+    if (sb > (u128)UINT64_MAX) sb = UINT64_MAX;
+    if (sb < 1) sb = 1;
+    f.safe_batch = (uint64_t)sb;
+    f.mode = p == P61 ? MODE_M61 : (p == P31 ? MODE_M31 : MODE_GENERIC);
+    return f;
+}
+
 // t = th 2^64 + tl mod 2^61 - 1 (any 128-bit t): 2^64 == 8.
 MDL_HD uint64_t fold_m61(uint64_t th, uint64_t tl) {
     uint64_t x = (tl & P61) + (tl >> 61) + ((th << 3) & P61) + (th >> 58);  // < 2^63

@@ -1,0 +1,187 @@
+// Signed-digit limb representation of F_p elements and the exact recombination
+// of int8 limb products (the epilogue of the matrix-core trailing update).
+//
+// An element x in [0, p) is mapped to its balanced residue
+// x^ = x - p [x > (p-1)/2], |x^| <= (p-1)/2, and written with L signed 8-bit
+// digits x^ = sum_l d_l 256^l, d_l in [-128, 127].  A product of two K-deep
+// digit panels (K <= 64) is A X = sum_s 256^s P_s over the NC = 2L-1 shift
+// classes s = la + lb, P_s = sum_{la+lb=s} A_la X_lb, each accumulated exactly in
+// int32 by the int8 MFMA: a class has at most 8 digit pairs and
+// -16256 <= d_a d_b <= 16384, so -2^23 < P_s <= 2^23.
+//
+// Recombination.  Two neighbouring classes fit one 32-bit word:
+//   P_2t + 256 P_2t+1  lies in [-HOFF, 2^32 - HOFF), HOFF = 8 * 64 * 16256 * 257,
+// so with the offset HOFF added (it is the initial value of the even classes'
+// accumulators, which then stay inside int32) the pair word
+//   h_t = acc_2t + 256 acc_2t+1   (mod 2^32)
+// is exact and non-negative.  The NH = L pair words sit 16 bits apart:
+//   S = sum_t h_t 2^(16 t) = sum_s 256^s P_s + sum_t HOFF 2^(16 t),
+// whose even words are disjoint 32-bit fields and whose odd words are the same
+// shifted by 16, so S is a two-operand add of 32-bit word strings.  It is
+// reduced mod p (Mersenne folds for 2^61 - 1 and 2^31 - 1, otherwise weights
+// 2^(32 i) mod p and one Barrett reduction), and the constant
+// k0 = sum_t HOFF 2^(16 t) mod p is subtracted by the caller.  Every step is
+// integer-exact, so the result is the canonical residue of the product.
+#pragma once
+#include <stdint.h>
+
+#include "fp_dev.hpp"
+
+namespace mdl {
+
+constexpr uint32_t HOFF = 2139029504u;  // 8 pairs * 64 terms * 16256 * 257
+
+struct LimbParams {
+    uint64_t pow256[16];  // 256^l mod p
+    uint64_t pow32[5];    // 2^(32 i) mod p: weights of the recombined 32-bit words
+    uint64_t k0;          // sum_{t < L} HOFF 2^(16 t) mod p (the pair words' offsets)
+    uint64_t kc;          // k0 + 2p (limb_sub_m61)
+    uint32_t L;           // digits per element
+    uint32_t pad;
+};
+
+// Digits needed for the balanced residues: the smallest L with
+// 127 (256^L - 1) / 255 >= (p - 1) / 2.
+inline uint32_t limb_count(uint64_t p) {
+    typedef unsigned __int128 u128;
+    const u128 half = (p - 1) / 2;
+    u128 cap = 127, pw = 1;
+    uint32_t L = 1;
+    while (cap < half) {
+        pw *= 256;
+        cap += 127 * pw;
+        L++;
+    }
+    return L;
+}
+
+inline LimbParams make_limb_params(uint64_t p) {
+    typedef unsigned __int128 u128;
+    LimbParams lp{};
+    lp.L = limb_count(p);
+    u128 x = 1 % p;
+    for (int l = 0; l < 16; l++) {
+        lp.pow256[l] = (uint64_t)x;
+        x = (x * 256) % p;
+    }
+    for (int i = 0; i < 4; i++) lp.pow32[i] = lp.pow256[4 * i];
+    lp.pow32[4] = (uint64_t)(((u128)lp.pow256[12] * lp.pow256[4]) % p);  // 2^128
+    u128 k0 = 0, w = 1 % p;  // w = 2^(16 t) mod p
+    for (uint32_t t = 0; t < lp.L; t++) {
+        k0 = (k0 + (u128)HOFF % p * w) % p;
+        w = (w * 65536) % p;
+    }
+    lp.k0 = (uint64_t)k0;
+    lp.kc = (uint64_t)(k0 + 2 * (u128)p);
+    return lp;
+}
+
+// a + b + cin with the carry out (one add-with-carry instruction on the GPU).
+MDL_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout) {
+#if defined(__clang__)
+    unsigned co;
+    const uint32_t r = __builtin_addc(a, b, cin, &co);
+    cout = co;
+    return r;
+#else
+    const uint64_t v = (uint64_t)a + b + cin;
+    cout = (uint32_t)(v >> 32);
+    return (uint32_t)v;
+#endif
+}
+
+// (hi:lo) >> 16, low 32 bits.
+MDL_HD uint32_t funnel16(uint32_t hi, uint32_t lo) { return (lo >> 16) | (hi << 16); }
+
+// S = sum_t h_t 2^(16 t) as five 32-bit words (NH <= 8 pair words).
+template <int NH>
+MDL_HD void pair_words_sum(const uint32_t *h, uint32_t *s) {
+    uint32_t e[5] = {0, 0, 0, 0, 0}, q[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < NH; t += 2) e[t / 2] = h[t];
+#pragma unroll
+    for (int t = 1; t < NH; t += 2) {  // h_t 2^(16 t) = (h_t << 16) 2^(32 (t-1)/2)
+        q[(t - 1) / 2] |= h[t] << 16;
+        q[(t + 1) / 2] |= h[t] >> 16;
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const uint64_t v = (uint64_t)e[i] + q[i] + c;
+        s[i] = (uint32_t)v;
+        c = (uint32_t)(v >> 32);
+    }
+}
+
+// Residue mod p of S = sum_s 256^s P_s + sum_t HOFF 2^(16 t), from the class
+// accumulators acc[0..NC) of one element (even classes started at HOFF, odd
+// ones at 0).  Returns the canonical residue in [0, p).
+template <int MODE, int NC>
+MDL_HD uint64_t limb_residue(const FieldParams &f, const LimbParams &lp, const int32_t *acc) {
+    constexpr int NH = (NC + 1) / 2;
+    uint32_t h[NH];
+#pragma unroll
+    for (int t = 0; t < NH; t++) {
+        uint32_t v = (uint32_t)acc[2 * t];
+        if (2 * t + 1 < NC) v += (uint32_t)acc[2 * t + 1] << 8;
+        h[t] = v;
+    }
+    if constexpr (MODE == MODE_M31 && NH == 4) {
+        // 2^32 == 2, 2^48 == 2^17 (mod 2^31 - 1): S == h0 + 2^16 h1 + 2 h2 + 2^17 h3 < 2^50
+        const uint64_t x = (uint64_t)h[0] + ((uint64_t)h[1] << 16) + ((uint64_t)h[2] << 1) + ((uint64_t)h[3] << 17);
+        return fold_m31(0, x);
+    }
+    uint32_t s[5];
+    pair_words_sum<NH>(h, s);
+    if constexpr (MODE == MODE_M61 && NH == 8) {
+        // S = lo + hi 2^64 + s4 2^128 with 2^64 == 8 and 2^128 == 2^6 (mod 2^61 - 1)
+        const uint64_t lo = ((uint64_t)s[1] << 32) | s[0], hi = ((uint64_t)s[3] << 32) | s[2];
+        uint64_t x = (lo & P61) + (lo >> 61) + ((hi << 3) & P61) + (hi >> 58) + ((uint64_t)s[4] << 6);  // < 2^63
+        x = (x & P61) + (x >> 61);                                                                       // <= p + 3
+        return x >= P61 ? x - P61 : x;
+    }
+    // weights 2^(32 i) mod p; the sum is below 5 * 2^95 < 2^98
+    constexpr int NS = (16 * (NH - 1) + 32) / 32 + 1;
+    uint64_t th = 0, tl = s[0];
+#pragma unroll
+    for (int i = 1; i < NS; i++) {
+        uint64_t ph, pl;
+        mul_wide((uint64_t)s[i], lp.pow32[i], ph, pl);
+        add128(th, tl, ph, pl);
+    }
+    return f.k >= 49 ? reduce_prod(f, th, tl) : reduce128(f, th, tl);
+}
+
+// (cv - sum_s 256^s P_s) mod 2^61 - 1 for cv in [0, p), from the 15 class
+// accumulators of L = 8 digits (HOFF-offset even classes); kc = k0 + 2p.  The
+// same sum as limb_residue in 32-bit add-with-carry steps, with the final
+// reduction shared by the subtraction.
+MDL_HD uint64_t limb_sub_m61(uint64_t cv, uint64_t kc, const int32_t *acc) {
+    uint32_t h[8];
+#pragma unroll
+    for (int t = 0; t < 7; t++) h[t] = (uint32_t)acc[2 * t] + ((uint32_t)acc[2 * t + 1] << 8);
+    h[7] = (uint32_t)acc[14];
+    // S = (h6 h4 h2 h0) + 2^16 (h7 h5 h3 h1) as five words s0..s4 (s4 <= 2^16)
+    uint32_t c;
+    const uint32_t s0 = addc32(h[0], h[1] << 16, 0, c);
+    const uint32_t s1 = addc32(h[2], funnel16(h[3], h[1]), c, c);
+    const uint32_t s2 = addc32(h[4], funnel16(h[5], h[3]), c, c);
+    const uint32_t s3 = addc32(h[6], funnel16(h[7], h[5]), c, c);
+    const uint32_t s4 = (h[7] >> 16) + c;
+    // x = (lo & p) + (lo >> 61) + ((hi << 3) & p) + (hi >> 58) + 2^6 s4 < 2^62 + 2^24,
+    // lo = (s1 s0), hi = (s3 s2): 2^64 == 8 and 2^128 == 2^6
+    const uint32_t small = (s1 >> 29) + (s3 >> 26) + (s4 << 6);
+    uint32_t xl = addc32(s0, s2 << 3, 0, c);
+    uint32_t xh = (s1 & 0x1fffffffu) + (((s3 << 3) | (s2 >> 29)) & 0x1fffffffu) + c;
+    xl = addc32(xl, small, 0, c);
+    xh += c;
+    // y = (x & p) + (x >> 61) <= p + 3
+    const uint32_t yl = addc32(xl, xh >> 29, 0, c);
+    const uint32_t yh = (xh & 0x1fffffffu) + c;
+    // r = cv + k0 + 2p - y in [p - 3, 4p), congruent to cv - (S - k0)
+    uint64_t r = cv + kc - (((uint64_t)yh << 32) | yl);
+    r = (r & P61) + (r >> 61);  // <= p + 3
+    return r >= P61 ? r - P61 : r;
+}
+
+}  // namespace mdl

@@ -9,11 +9,13 @@
 //   * A X = sum_s 256^s P_s with P_s = sum_{la+lb=s} A_la X_lb, s < 2L-1: each
 //     digit pair is one int8 GEMM with K = 64 (the panel), accumulated exactly in
 //     int32 per shift class (|P_s| <= 8 * 64 * 2^14 = 2^23);
-//   * the epilogue folds the classes into four signed 49-bit words
-//     w_i = sum_{s in [4i, 4i+4)} P_s 256^(s-4i), maps them to residues with the
-//     constants 2^(32 i) mod p, and reduces once with the reference's Barrett
-//     reduction (fp.rs:71-79).  Every result is the canonical residue, so the
-//     update equals the VALU path and the CPU reference bit for bit.
+//   * the epilogue (limb_residue, limb.hpp) joins neighbouring classes into
+//     exact 32-bit pair words -- the even classes' accumulators start at an
+//     offset that keeps them non-negative -- adds the pair words as two 32-bit
+//     word strings and reduces the sum once (Mersenne folds, or the weights
+//     2^(32 i) mod p and the reference's Barrett reduction, fp.rs:71-79).  Every
+//     result is the canonical residue, so the update equals the VALU path and
+//     the CPU reference bit for bit.
 //
 // k_gather_split  digit planes of A = M[O, J] (rows x L*64, K-contiguous) and of
 //                 X = Z M[R, c0:] (columns x L*64, K-contiguous), fusing the X
@@ -27,13 +29,6 @@
 namespace mdl {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-
-// 128-bit product of u (< 2^50) and c (< 2^63), accumulated into (h, l).
-__device__ __forceinline__ void mac128(uint64_t &h, uint64_t &l, uint64_t u, uint64_t c) {
-    uint64_t ph, pl;
-    mul_wide(u, c, ph, pl);
-    add128(h, l, ph, pl);
-}
 
 // Geometry of one 64 x 64 update tile.
 struct TileMeta {
@@ -58,37 +53,25 @@ __device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, cons
     return m;
 }
 
-// sum_i w_i 2^(32 i) mod p for signed |w_i| < 2^48 (i < NW), returned as the
-// residue of sum_i (w_i + 2^49) 2^(32 i); the caller subtracts lp.k0.
-template <int MODE, int NW>
-__device__ __forceinline__ uint64_t fold_words(const FieldParams &f, const LimbParams &lp, const int64_t *w) {
-    constexpr uint64_t OFF = 1ull << 49;
-    if (MODE == MODE_M61) {
-        // 2^32, 2^64 == 8, 2^96 == 2^35 (mod 2^61 - 1)
-        uint64_t s = (uint64_t)(w[0] + (int64_t)OFF);
-        if (NW > 1) {
-            const uint64_t u = (uint64_t)(w[1] + (int64_t)OFF);
-            s += ((u & ((1ull << 29) - 1)) << 32) + (u >> 29);
-        }
-        if (NW > 2) s += (uint64_t)(w[2] + (int64_t)OFF) << 3;
-        if (NW > 3) {
-            const uint64_t u = (uint64_t)(w[3] + (int64_t)OFF);
-            s += ((u & ((1ull << 26) - 1)) << 35) + (u >> 26);
-        }
-        return fold_m61(0, s);
-    }
-    if (MODE == MODE_M31) {
-        // 2^(32 i) == 2^i (mod 2^31 - 1)
-        uint64_t s = 0;
+// Residue of the product for element (b, r) of this lane's accumulators.
+template <int MODE, int NC>
+__device__ __forceinline__ uint64_t acc_residue(const FieldParams &f, const LimbParams &lp, v4i (*acc)[2], int b,
+                                                int r) {
+    int32_t P[NC];
 #pragma unroll
-        for (int q = 0; q < NW; q++) s += (uint64_t)(w[q] + (int64_t)OFF) << q;
-        return fold_m31(0, s);
-    }
-    uint64_t h = 0, l = (uint64_t)(w[0] + (int64_t)OFF);
+    for (int s = 0; s < NC; s++) P[s] = acc[s][b][r];
+    return limb_residue<MODE, NC>(f, lp, P);
+}
+
+// (cv - product) mod p for element (b, r) of this lane's accumulators.
+template <int MODE, int NC>
+__device__ __forceinline__ uint64_t acc_sub(const FieldParams &f, const LimbParams &lp, uint64_t cv, v4i (*acc)[2],
+                                            int b, int r) {
+    int32_t P[NC];
 #pragma unroll
-    for (int q = 1; q < NW; q++) mac128(h, l, (uint64_t)(w[q] + (int64_t)OFF), lp.pow256[4 * q]);
-    // the sum is below 2^115: the single-word Barrett applies once 2k >= 116
-    return f.k >= 58 ? reduce_prod(f, h, l) : reduce128(f, h, l);
+    for (int s = 0; s < NC; s++) P[s] = acc[s][b][r];
+    if constexpr (MODE == MODE_M61 && NC == 15) return limb_sub_m61(cv, lp.kc, P);
+    return sub_mod(add_mod(cv, lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
 }
 
 // Balanced signed digits of four consecutive elements, packed per digit plane:
@@ -111,12 +94,16 @@ __device__ __forceinline__ void digits4(const uint64_t *x, uint64_t p, uint32_t 
     }
 }
 
-// One 64 x 64 x 64 limb product on the matrix cores: acc[s][b] += sum_{la+lb=s}
-// A_la B_lb for this wave's 16 x 32 block (rows wr.., columns wc..).  As / Bs hold
-// the digit rows (K-contiguous, L planes of 64 bytes, row stride KAP).
+// One 64 x 64 x 64 limb product on the matrix cores: acc[s][b] = init_s +
+// sum_{la+lb=s} A_la B_lb for this wave's 16 x 32 block (rows wr.., columns wc..),
+// with init_s = HOFF for the even classes and 0 for the odd ones (limb.hpp): the
+// first product of each class takes its initial value as the MFMA's C operand.
+// As / Bs hold the digit rows (K-contiguous, L planes of 64 bytes, row stride KAP).
 template <int L, uint32_t KAP>
 __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs, uint32_t lane, uint32_t wr,
                                               uint32_t wc, v4i (*acc)[2]) {
+    constexpr int H = (int)HOFF;
+    const v4i hoff = {H, H, H, H}, zero = {0, 0, 0, 0};
     v4i bf[L][2];
     const uint32_t koff = 16 * (lane >> 4);
 #pragma unroll
@@ -128,8 +115,13 @@ __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs
         const v4i af = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
 #pragma unroll
         for (int lb = 0; lb < L; lb++) {
-            acc[la + lb][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][0], acc[la + lb][0], 0, 0, 0);
-            acc[la + lb][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][1], acc[la + lb][1], 0, 0, 0);
+            const int s = la + lb;
+            const bool first = (la == 0) || (lb == L - 1);  // loop order: la outer, lb inner
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const v4i c = first ? ((s & 1) ? zero : hoff) : acc[s][b];
+                acc[s][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[lb][b], c, 0, 0, 0);
+            }
         }
     }
 }
@@ -145,7 +137,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                                                       uint32_t par) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;
-    constexpr int NC = 2 * L - 1, NW = (NC + 3) / 4;
+    constexpr int NC = 2 * L - 1;
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = plan->gx_total;
@@ -203,8 +195,6 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
         }
         __syncthreads();
         v4i acc[NC][2];
-#pragma unroll
-        for (int q = 0; q < NC; q++) acc[q][0] = acc[q][1] = v4i{0, 0, 0, 0};
         limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
         // X = sum_s 256^s P_s mod p, then its digits (4 consecutive k per lane and column)
 #pragma unroll
@@ -212,14 +202,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             const uint32_t c = wc + 16 * b + (lane & 15);
             uint64_t x[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int64_t w[NW];
-#pragma unroll
-                for (int q = 0; q < NW; q++) w[q] = 0;
-#pragma unroll
-                for (int q = 0; q < NC; q++) w[q >> 2] += (int64_t)acc[q][b][r] << (8 * (q & 3));
-                x[r] = sub_mod(fold_words<MODE, NW>(f, lp, w), lp.k0, p);
-            }
+            for (int r = 0; r < 4; r++) x[r] = sub_mod(acc_residue<MODE, NC>(f, lp, acc, b, r), lp.k0, p);
             if (c < cols) {
                 uint32_t dw[L];
                 digits4<L>(x, p, dw);
@@ -238,7 +221,6 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;    // LDS row stride: consecutive rows start 4 banks apart
     constexpr int NC = 2 * L - 1;        // shift classes
-    constexpr int NW = (NC + 3) / 4;     // 32-bit-aligned words of the recombined value
     constexpr uint32_t CH = KA / 16;     // 16-byte chunks per digit row
     constexpr uint32_t NCH = UT * CH;              // chunks per operand tile
     constexpr uint32_t NV = (NCH + 511) / 512;     // chunks per thread per operand tile
@@ -247,7 +229,6 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     const uint32_t total = far ? plan->far_total : plan->near_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
-    const uint64_t p = f.p;
 
     // operand prefetch registers: the next tile's digit chunks
     v4i ra[NV], rb[NV];
@@ -329,8 +310,6 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                     cv[b][r] = (mrow[r] && c < nc) ? mrow[r][c] : 0;
                 }
             v4i acc[NC][2];
-#pragma unroll
-            for (int s = 0; s < NC; s++) acc[s][0] = acc[s][1] = v4i{0, 0, 0, 0};
             limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
             // epilogue: C - sum_s 256^s P_s (mod p)
 #pragma unroll
@@ -339,13 +318,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                 for (int r = 0; r < 4; r++) {
                     const uint32_t c = wc + 16 * b + (lane & 15);
                     if (!mrow[r] || c >= nc) continue;
-                    int64_t w[NW];
-#pragma unroll
-                    for (int q = 0; q < NW; q++) w[q] = 0;
-#pragma unroll
-                    for (int s = 0; s < NC; s++) w[s >> 2] += (int64_t)acc[s][b][r] << (8 * (s & 3));
-                    const uint64_t vr = fold_words<MODE, NW>(f, lp, w);
-                    const uint64_t res = sub_mod(add_mod(cv[b][r], lp.k0, p), vr, p);
+                    const uint64_t res = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
                     mrow[r][c] = res;
                     const uint32_t gcol = c0 + cb + c;
                     if (res != 0 && gcol < jend) nzcol = min(nzcol, gcol);
@@ -369,7 +342,6 @@ __global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;
     constexpr int NC = 2 * L - 1;
-    constexpr int NW = (NC + 3) / 4;
     constexpr uint32_t CH = KA / 16;
     constexpr uint32_t NCH = UT * CH;
     constexpr uint32_t NV = (NCH + 511) / 512;
@@ -378,7 +350,6 @@ __global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat
     const uint32_t total = plan->far_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
-    const uint64_t p = f.p;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
         RankMat &S = mats[mi];
@@ -437,8 +408,6 @@ __global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat
                     cv[b][r] = (mrow[r] && c < nc) ? mrow[r][cb + c] : 0;
                 }
             v4i acc[NC][2];
-#pragma unroll
-            for (int s = 0; s < NC; s++) acc[s][0] = acc[s][1] = v4i{0, 0, 0, 0};
             limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
             // epilogue: C - sum_s 256^s P_s (mod p)
 #pragma unroll
@@ -447,13 +416,7 @@ __global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat
                 for (int r = 0; r < 4; r++) {
                     const uint32_t c = wc + 16 * b + (lane & 15);
                     if (!mrow[r] || c >= nc) continue;
-                    int64_t w[NW];
-#pragma unroll
-                    for (int q = 0; q < NW; q++) w[q] = 0;
-#pragma unroll
-                    for (int s = 0; s < NC; s++) w[s >> 2] += (int64_t)acc[s][b][r] << (8 * (s & 3));
-                    const uint64_t vr = fold_words<MODE, NW>(f, lp, w);
-                    mrow[r][cb + c] = sub_mod(add_mod(cv[b][r], lp.k0, p), vr, p);
+                    mrow[r][cb + c] = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
                 }
         }
     }

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "fp_dev.hpp"
+#include "limb.hpp"
 
 namespace mdl {
 
@@ -84,17 +85,6 @@ struct RankLaunch {
     RankPlan *plan;
     uint32_t grid_gx, grid_near, grid_far;
     uint32_t far_items;                     // far work counted as (row band, XCD group) items (MFMA)
-};
-
-// Constants of the signed-digit limb representation used by the MFMA update.
-// An element x in [0, p) is first mapped to its balanced residue
-// x^ = x - p [x > (p-1)/2], |x^| <= (p-1)/2, then written with L signed 8-bit
-// digits: x^ = sum_l d_l 256^l, d_l in [-128, 127].
-struct LimbParams {
-    uint64_t pow256[16];  // 256^l mod p
-    uint64_t k0;          // 2^49 * sum_{i : 4i < 2L-1} 256^(4i) mod p (epilogue offset)
-    uint32_t L;           // digits per element
-    uint32_t pad;
 };
 
 constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task

@@ -1,0 +1,125 @@
+// Exactness of the limb recombination (modlin-rs_amd/csrc/limb.hpp) on the CPU.
+//
+// For arbitrary int8 digit panels -- uniform digits, and the extreme panels that
+// drive every shift class to its upper or lower bound -- the class accumulators
+// are formed exactly as the matrix cores form them (even classes start at HOFF,
+// int32 accumulation, checked for overflow), then recombined with limb_residue
+// and limb_sub_m61.  The result must equal the product sum_k A_k X_k mod p with
+// A_k = sum_l a_kl 256^l computed directly in 128-bit integers.
+//
+// Built and run by tests/test_limb.py; prints "bad=0" and exits 0 on success.
+#include <stdint.h>
+#include <stdio.h>
+
+#include <random>
+
+#include "limb.hpp"
+
+using namespace mdl;
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+static std::mt19937_64 rng(12345);
+static int bad = 0;
+
+// Digit of panel kind `kind` at (k, l): 0 uniform, 1 all -128 (classes at their
+// maximum), 2 -128 against 127 (minimum), 3 random extremes, 4 alternating.
+static void digits(int kind, int l, int &da, int &dx) {
+    switch (kind) {
+        case 0: da = (int)(rng() % 256) - 128; dx = (int)(rng() % 256) - 128; break;
+        case 1: da = -128; dx = -128; break;
+        case 2: da = -128; dx = 127; break;
+        case 3: da = (rng() & 1) ? -128 : 127; dx = (rng() & 1) ? -128 : 127; break;
+        default: da = (l & 1) ? -128 : 127; dx = (l & 1) ? 127 : -128; break;
+    }
+}
+
+static i128 value_mod(const int8_t *d, int L, uint64_t p) {
+    i128 v = 0, w = 1;
+    for (int l = 0; l < L; l++) {
+        v += (i128)d[l] * w;
+        w *= 256;
+    }
+    v %= (i128)p;
+    return v < 0 ? v + (i128)p : v;
+}
+
+template <int MODE, int L>
+static void run(uint64_t p, int trials, int kind) {
+    constexpr int NC = 2 * L - 1;
+    const FieldParams f = make_params(p);
+    const LimbParams lp = make_limb_params(p);
+    for (int tr = 0; tr < trials; tr++) {
+        const int K = kind == 0 ? 1 + (int)(rng() % 64) : 64;
+        int8_t a[64][8], x[64][8];
+        for (int k = 0; k < K; k++)
+            for (int l = 0; l < L; l++) {
+                int da, dx;
+                digits(kind, l, da, dx);
+                a[k][l] = (int8_t)da;
+                x[k][l] = (int8_t)dx;
+            }
+        int32_t acc[NC];
+        for (int s = 0; s < NC; s++) acc[s] = (s & 1) ? 0 : (int32_t)HOFF;
+        for (int k = 0; k < K; k++)
+            for (int la = 0; la < L; la++)
+                for (int lb = 0; lb < L; lb++) {
+                    const int64_t v = (int64_t)acc[la + lb] + (int64_t)a[k][la] * x[k][lb];
+                    if (v > INT32_MAX || v < INT32_MIN) {
+                        printf("int32 overflow p=%llu L=%d kind=%d\n", (unsigned long long)p, L, kind);
+                        bad++;
+                        return;
+                    }
+                    acc[la + lb] = (int32_t)v;
+                }
+        i128 want = 0;
+        for (int k = 0; k < K; k++)
+            want = (want + (i128)(((u128)value_mod(a[k], L, p) * (u128)value_mod(x[k], L, p)) % p)) % (i128)p;
+        uint64_t got = limb_residue<MODE, NC>(f, lp, acc);
+        got = got >= lp.k0 ? got - lp.k0 : got + p - lp.k0;
+        if ((uint64_t)want != got) {
+            if (bad < 8)
+                printf("limb_residue mismatch p=%llu L=%d kind=%d K=%d\n", (unsigned long long)p, L, kind, K);
+            bad++;
+        }
+        if (MODE == MODE_M61 && L == 8) {
+            const uint64_t cv = rng() % p;
+            const uint64_t sub = limb_sub_m61(cv, lp.kc, acc);
+            const uint64_t exp = cv >= (uint64_t)want ? cv - (uint64_t)want : cv + p - (uint64_t)want;
+            if (sub != exp) {
+                if (bad < 8) printf("limb_sub_m61 mismatch kind=%d K=%d\n", kind, K);
+                bad++;
+            }
+        }
+    }
+}
+
+template <int L>
+static void generic(uint64_t p, int trials) {
+    for (int kind = 0; kind < 5; kind++) run<MODE_GENERIC, L>(p, kind ? 20 : trials, kind);
+}
+
+int main() {
+    const int n = 20000;
+    for (int kind = 0; kind < 5; kind++) {
+        run<MODE_M61, 8>(P61, kind ? 50 : 10 * n, kind);
+        run<MODE_M31, 4>(P31, kind ? 50 : 10 * n, kind);
+    }
+    const uint64_t primes[] = {2ull, 3ull, 5ull, 7ull, 127ull, 131ull, 251ull, 257ull, 65521ull, 65537ull,
+                               16777213ull, 4294967291ull, 1099511627689ull, 281474976710597ull,
+                               72057594037927931ull, P31, P61, 4611686018427387847ull, 9223372036854775783ull};
+    for (uint64_t p : primes) {
+        switch (limb_count(p)) {
+            case 1: generic<1>(p, n); break;
+            case 2: generic<2>(p, n); break;
+            case 3: generic<3>(p, n); break;
+            case 4: generic<4>(p, n); break;
+            case 5: generic<5>(p, n); break;
+            case 6: generic<6>(p, n); break;
+            case 7: generic<7>(p, n); break;
+            default: generic<8>(p, n); break;
+        }
+    }
+    printf("bad=%d\n", bad);
+    return bad != 0;
+}
