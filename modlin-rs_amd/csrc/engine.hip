@@ -154,6 +154,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + sizeof(RankPlan), A);
     const size_t off_active = cur;
     cur = align_up(cur + sizeof(uint32_t), A);
+    const size_t off_trash = cur;
+    cur = align_up(cur + TRASH_WORDS * sizeof(uint64_t), A);
     for (uint64_t i = 0; i < count; i++) {
         const uint64_t n = mats[i].n;
         if (n > 0xffffffffull / 2 || (n && n > (((uint64_t)1) << 62) / n / 8)) {
@@ -229,6 +231,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     rl.near_off = (uint32_t *)(base + off_upd);
     rl.far_off = (uint32_t *)(base + off_far);
     rl.plan = (RankPlan *)(base + off_plan);
+    rl.trash = (uint64_t *)(base + off_trash);
     uint32_t *dactive = (uint32_t *)(base + off_active);
     rc = copy_h2d(ctx, dmats, hm.data(), count * sizeof(RankMat));
     if (rc) return rc;
@@ -239,7 +242,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     const uint32_t nmat = (uint32_t)count;
     rl.grid_gx = 1024;
     rl.grid_near = (uint32_t)ctx->cu_count * 2;
-    rl.grid_far = (uint32_t)ctx->cu_count * (use_mfma ? 2 : 8);  // multiples of 8 (XCD numbering)
+    rl.grid_far = (uint32_t)ctx->cu_count * (use_mfma ? 4 : 8);  // multiples of 8 (XCD numbering)
     rl.far_items = use_mfma ? 1u : 0u;
     int check_every = 8;
     if (const char *env = getenv("MODLIN_TUNE")) {  // "near=N,far=N,gx=N,every=N" (experiments)

@@ -329,6 +329,28 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     }
 }
 
+// The same limb product for a 16 x 16 block with the roles of the operands
+// swapped in the register file: the wave's L A digit fragments (its 16 rows) are
+// held, and the B fragment of its 16 columns is read per digit plane lb.
+template <int L, uint32_t KAP>
+__device__ __forceinline__ void limb_tile_mma_ahold(const v4i *af, const int8_t *Bs, uint32_t lane, uint32_t wc,
+                                                    v4i *acc) {
+    constexpr int H = (int)HOFF;
+    const v4i hoff = {H, H, H, H}, zero = {0, 0, 0, 0};
+    const uint32_t koff = 16 * (lane >> 4);
+#pragma unroll
+    for (int lb = 0; lb < L; lb++) {
+        const v4i bf = *(const v4i *)(Bs + (wc + (lane & 15)) * KAP + lb * KSTEP + koff);
+#pragma unroll
+        for (int la = 0; la < L; la++) {
+            const int s = la + lb;
+            const bool first = (lb == 0) || (la == L - 1);  // loop order: lb outer, la inner
+            const v4i c = first ? ((s & 1) ? zero : hoff) : acc[s];
+            acc[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[la], bf, c, 0, 0, 0);
+        }
+    }
+}
+
 // Far part of the update, by row band: work item (row band, XCD group x)
 // updates the column tiles tc = x, x + 8, ... of that row band.  The row band's
 // A digit tile is staged in LDS once and stays there while the group's B digit
@@ -336,88 +358,132 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
 // stages 32 KiB instead of 64.  Items are numbered rows-major with the group
 // fastest; with a grid that is a multiple of 8, item t runs on XCD t mod 8 = x,
 // so an XCD's L2 holds its own column tiles.
+//
+// A workgroup is 4 waves; wave w owns rows 16 w .. 16 w + 15 of the 64 x 64 tile
+// and computes its four 16 x 16 column quarters one after the other.  Two
+// workgroups share a CU and synchronise independently, so the matrix-core work
+// of one overlaps the epilogue (VALU) of the other.
+constexpr uint32_t FAR_THREADS = 256;
+
 template <int L, int MODE>
-__global__ __launch_bounds__(512) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
-                                                    const RankPlan *plan, FieldParams f, LimbParams lp, uint32_t par) {
+__global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
+                                                               const RankPlan *plan, FieldParams f, LimbParams lp,
+                                                               uint32_t par, uint64_t *trash) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;
     constexpr int NC = 2 * L - 1;
     constexpr uint32_t CH = KA / 16;
     constexpr uint32_t NCH = UT * CH;
-    constexpr uint32_t NV = (NCH + 511) / 512;
+    constexpr uint32_t NV = (NCH + FAR_THREADS - 1) / FAR_THREADS;
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = plan->far_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
+    const uint32_t wr = wave * 16;
+    const uint32_t koff = 16 * (lane >> 4);
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
-        RankMat &S = mats[mi];
+        // Everything the item needs, read once into registers: the stores below go
+        // through global pointers that the compiler cannot tell apart from *mats.
+        const RankMat &S = mats[mi];
         const IterState &I = S.it[par];
         const uint32_t lt = item - item_off[mi];
         const uint32_t x = lt & 7, band = lt >> 3;
         const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld, rank = I.rank, c0 = I.c0;
+        const gptr<uint64_t> Mg = to_global(S.M);
+        const gptr<const uint32_t> permg = to_global((const uint32_t *)S.perm);
+        const gptr<const v4i> Alg = to_global((const v4i *)S.Al);
+        const gptr<const v4i> Xlg = to_global((const v4i *)S.Xl);
         uint32_t cbase, width;
         update_range(I, 1, cbase, width);
         const uint32_t C = cdiv(width, UT);
         if (pb == 0 || x >= C) continue;  // uniform
         const uint32_t r0 = band * UT, nr = min(UT, O - r0);
+        v4i rb[NV];
         __syncthreads();  // the previous item's last tile is done with As
         {
-            const v4i *ga = (const v4i *)(S.Al + (size_t)r0 * KA);
+            const gptr<const v4i> ga = Alg + (size_t)r0 * (KA / 16);
+#pragma unroll
+            for (uint32_t j = 0; j < NV; j++)
+                if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = ga[tid + j * FAR_THREADS];
 #pragma unroll
             for (uint32_t j = 0; j < NV; j++) {
-                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
-                if (NCH % 512 == 0 || v < NCH) *(v4i *)(As + row * KAP + ch * 16) = ga[v];
+                const uint32_t v = tid + j * FAR_THREADS, row = v / CH, ch = v - row * CH;
+                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(As + row * KAP + ch * 16) = rb[j];
             }
         }
-        uint64_t *mrow[4];
+        // This lane's rows; a row outside the band loads and stores through `tr`.
+        const gptr<uint64_t> tr = to_global(trash) + tid;
+        gptr<uint64_t> mrow[4];
+        bool rok[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const uint32_t i = wr + 4 * (lane >> 4) + r;
-            mrow[r] = (i < nr) ? S.M + (size_t)S.perm[rank + r0 + i] * ld + c0 : nullptr;
+            rok[r] = i < nr;
+            mrow[r] = Mg + (size_t)permg[rank + r0 + min(i, nr - 1)] * ld + c0;
         }
-        v4i rb[NV];
         {
-            const v4i *gb = (const v4i *)(S.Xl + (size_t)(cbase + x * UT) * KA);
+            const gptr<const v4i> gb = Xlg + (size_t)(cbase + x * UT) * (KA / 16);
 #pragma unroll
             for (uint32_t j = 0; j < NV; j++)
-                if (NCH % 512 == 0 || tid + j * 512 < NCH) rb[j] = gb[tid + j * 512];
+                if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = gb[tid + j * FAR_THREADS];
+        }
+        __syncthreads();  // As is written
+        v4i af[L];  // this wave's A fragments: rows wr + (lane & 15), K bytes koff.. of each plane
+#pragma unroll
+        for (int la = 0; la < L; la++) af[la] = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
+        // C values of this lane's elements, one tile ahead: cvb[q][r] = row r, column
+        // 16 q + (lane & 15) of the tile; quarter q of the next tile is loaded as soon
+        // as quarter q of this one is done, so the loads have a whole tile to land.
+        uint64_t cvb[4][4];
+        {
+            const uint32_t cb = cbase + x * UT, nc = min(UT, cbase + width - cb);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t c = 16 * q + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; r++) cvb[q][r] = *((rok[r] && c < nc) ? mrow[r] + cb + c : tr);
+            }
         }
         for (uint32_t tc = x; tc < C; tc += 8) {
             const uint32_t cb = cbase + tc * UT, nc = min(UT, cbase + width - cb);
-            __syncthreads();  // the previous tile is done with Bs (and As is written)
+            const uint32_t cbn = cb + 8 * UT;
+            const uint32_t ncn = tc + 8 < C ? min(UT, cbase + width - cbn) : 0;
+            __syncthreads();  // the previous tile is done with Bs
 #pragma unroll
             for (uint32_t j = 0; j < NV; j++) {
-                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
-                if (NCH % 512 == 0 || v < NCH) *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
+                const uint32_t v = tid + j * FAR_THREADS, row = v / CH, ch = v - row * CH;
+                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
             }
             __syncthreads();
             if (tc + 8 < C) {  // prefetch the next B digit tile
-                const v4i *gb = (const v4i *)(S.Xl + (size_t)(cbase + (tc + 8) * UT) * KA);
+                const gptr<const v4i> gb = Xlg + (size_t)(cbase + (tc + 8) * UT) * (KA / 16);
 #pragma unroll
                 for (uint32_t j = 0; j < NV; j++)
-                    if (NCH % 512 == 0 || tid + j * 512 < NCH) rb[j] = gb[tid + j * 512];
+                    if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = gb[tid + j * FAR_THREADS];
             }
-            uint64_t cv[2][4];
 #pragma unroll
-            for (int b = 0; b < 2; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint32_t c = wc + 16 * b + (lane & 15);
-                    cv[b][r] = (mrow[r] && c < nc) ? mrow[r][cb + c] : 0;
-                }
-            v4i acc[NC][2];
-            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
-            // epilogue: C - sum_s 256^s P_s (mod p)
-#pragma unroll
-            for (int b = 0; b < 2; b++)
+            for (int q = 0; q < 4; q++) {
+                const uint32_t c = 16 * q + (lane & 15);
+                v4i acc[NC];
+                limb_tile_mma_ahold<L, KAP>(af, Bs, lane, 16 * q, acc);
+                // epilogue: C - sum_s 256^s P_s (mod p)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const uint32_t c = wc + 16 * b + (lane & 15);
-                    if (!mrow[r] || c >= nc) continue;
-                    mrow[r][cb + c] = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
+                    int32_t P[NC];
+#pragma unroll
+                    for (int k = 0; k < NC; k++) P[k] = acc[k][r];
+                    uint64_t v;
+                    if constexpr (MODE == MODE_M61 && NC == 15)
+                        v = limb_sub_m61(cvb[q][r], lp.kc, P);
+                    else
+                        v = sub_mod(add_mod(cvb[q][r], lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
+                    *((rok[r] && c < nc) ? mrow[r] + cb + c : tr) = v;
                 }
+                // quarter q of the next tile
+#pragma unroll
+                for (int r = 0; r < 4; r++) cvb[q][r] = *((rok[r] && c < ncn) ? mrow[r] + cbn + c : tr);
+            }
         }
     }
 }
@@ -433,8 +499,8 @@ static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const F
         hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(rl.grid_near), dim3(512), 0, st, dmats, nmat, rl.near_off,
                            rl.plan, f, lp, par, 0u);
     else
-        hipLaunchKernelGGL((k_update_far<L, MODE>), dim3(rl.grid_far), dim3(512), 0, st, dmats, nmat, rl.far_off,
-                           rl.plan, f, lp, par);
+        hipLaunchKernelGGL((k_update_far<L, MODE>), dim3(rl.grid_far), dim3(FAR_THREADS), 0, st, dmats, nmat, rl.far_off,
+                           rl.plan, f, lp, par, rl.trash);
 }
 
 // what = -1: digit gather and X product; 0: near update; 1: far update.

@@ -85,12 +85,29 @@ struct RankLaunch {
     RankPlan *plan;
     uint32_t grid_gx, grid_near, grid_far;
     uint32_t far_items;                     // far work counted as (row band, XCD group) items (MFMA)
+    uint64_t *trash;                        // TRASH_WORDS words that lanes with no element load and store
 };
+
+// Words of the per-launch trash area: a lane whose element lies outside the
+// matrix loads from and stores to trash[tid] instead, so that every load and
+// store of the update kernels is unconditional (no per-lane branches, and exact
+// outstanding-memory counts for the compiler's waits).
+constexpr uint32_t TRASH_WORDS = 1024;
 
 constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task
 constexpr uint32_t XCOLS = 64;         // columns per X task
 constexpr uint32_t UT = 64;            // trailing-update tile edge
 constexpr uint32_t KSTEP = 64;         // int8 K per MFMA step (one digit plane of a panel)
+
+// Pointers into global memory.  Loads and stores through them compile to global_*
+// instructions; through a generic pointer they are flat_*, whose completion also
+// holds up every wait for LDS results.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+    return (gptr<T>)p;
+}
 
 __device__ __forceinline__ uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     const uint32_t s0 = (blockIdx.x % ZG) * ZSLOTS;
     if (s0 >= pb) return;  // uniform over the workgroup
     __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
-    __shared__ uint64_t Ur[BMAX][BMAX];  // U[r][t], t > r
+    __shared__ uint64_t Ur[BMAX][BMAX + 1];  // U[r][t], t > r (padded: the back substitution reads columns)
     __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
     __shared__ uint32_t swq[BMAX];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
