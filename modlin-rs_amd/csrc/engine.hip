@@ -18,7 +18,8 @@
 namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
-void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par);
+void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
+                  uint32_t L);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
 void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
@@ -140,7 +141,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
 
     // Workspace layout (256-byte aligned regions).
     std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offE(count), offS(count),
-        offAl(count), offXl(count);
+        offAl(count), offXl(count), offZl(count);
     size_t cur = 0;
     const size_t A = 256;
     cur = align_up(cur + count * sizeof(RankMat), A);
@@ -178,11 +179,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offE[i] = cur; cur = align_up(cur + ETA_WORDS * 8, A);
         offS[i] = 0;
         if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
-        offAl[i] = offXl[i] = 0;
+        offAl[i] = offXl[i] = offZl[i] = 0;
         if (use_mfma) {
             const size_t npad = (n + UT - 1) / UT * UT;
             offAl[i] = cur; cur = align_up(cur + npad * KA, A);
             offXl[i] = cur; cur = align_up(cur + npad * KA, A);
+            offZl[i] = cur; cur = align_up(cur + BMAX * KA, A);
         }
     }
     int rc = arena_reserve(ctx, cur);
@@ -204,6 +206,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.Eta = (uint64_t *)(base + offE[i]);
         S.Al = use_mfma ? (int8_t *)(base + offAl[i]) : nullptr;
         S.Xl = use_mfma ? (int8_t *)(base + offXl[i]) : nullptr;
+        S.Zl = use_mfma ? (int8_t *)(base + offZl[i]) : nullptr;
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
         S.done = (n == 0) ? 1u : 0u;
@@ -271,7 +274,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             const uint32_t par = (uint32_t)(s & 1);
             hipError_t e;
             if ((e = hipStreamWaitEvent(aux, ctx->ev_fin, 0)) != hipSuccess) return e;
-            launch_block(aux, dmats, nmat, f, b, par);
+            launch_block(aux, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u);
             if ((e = hipEventRecord(ctx->ev_ge, aux)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(st, ctx->ev_ge, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);

@@ -228,13 +228,26 @@ MDL_HD uint64_t reduce_t(const FieldParams &f, uint64_t th, uint64_t tl) {  // a
     if (MODE == MODE_M31) return fold_m31(th, tl);
     return reduce128(f, th, tl);
 }
+// b x + d y mod 2^61 - 1 for x, y, b, d < 2^61, b and d given split into 31-bit
+// halves (b0 = b mod 2^31, b1 = b >> 31).  Both products share the two sums
+//   s = x0 b0 + x1 2 b1 + y0 d0 + y1 2 d1 < 3 2^62,   t = x0 b1 + x1 b0 + y0 d1 + y1 d0 < 2^63
+// (2^62 == 2), so b x + d y == s + t 2^31 and a single fold reduces them.
+MDL_HD uint64_t lin2_m61(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
+    const uint32_t x0 = (uint32_t)x & 0x7fffffffu, x1 = (uint32_t)(x >> 31);
+    const uint32_t y0 = (uint32_t)y & 0x7fffffffu, y1 = (uint32_t)(y >> 31);
+    const uint64_t s = (uint64_t)x0 * b0 + (uint64_t)x1 * (2u * b1) + (uint64_t)y0 * d0 + (uint64_t)y1 * (2u * d1);
+    const uint64_t t = (uint64_t)x0 * b1 + (uint64_t)x1 * b0 + (uint64_t)y0 * d1 + (uint64_t)y1 * d0;
+    uint64_t r = s + (t >> 30) + ((t & 0x3fffffffull) << 31);  // < 2^64
+    r = (r & P61) + (r >> 61);                                   // <= p + 7
+    return r >= P61 ? r - P61 : r;
+}
+
 template <int MODE>
 MDL_HD uint64_t ff_step_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
     if (MODE == MODE_M61) {
         const uint64_t ne = P61 - e;
-        const uint64_t u = fold_m61_64(mul_m61_lazy(x, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31)));
-        const uint64_t v = fold_m61_64(mul_m61_lazy(y, (uint32_t)ne & 0x7fffffffu, (uint32_t)(ne >> 31)));
-        return canon_m61(u + v);
+        return lin2_m61(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)ne & 0x7fffffffu,
+                        (uint32_t)(ne >> 31));
     }
     if (MODE == MODE_M31) {
         uint64_t h, l, h2, l2;

@@ -171,16 +171,11 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
         const uint32_t cb = (lt - ng) * XCOLS;
         const uint32_t cols = min(XCOLS, I.cend - I.c0 - cb);
         __syncthreads();
-        // As[k][la*64 + s] = digit la of Z[k][s];  Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
-        for (uint32_t idx = tid; idx < UT * (KSTEP / 4); idx += blockDim.x) {
-            const uint32_t k = idx / (KSTEP / 4), s4 = (idx - k * (KSTEP / 4)) * 4;
-            uint64_t x[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) x[j] = (k < pb && s4 + j < pb) ? S.Z[k * BMAX + s4 + j] : 0;
-            uint32_t w[L];
-            digits4<L>(x, p, w);
-#pragma unroll
-            for (int la = 0; la < L; la++) *(uint32_t *)(As + k * KAP + la * KSTEP + s4) = w[la];
+        // As[k][la*64 + s] = digit la of Z[k][s] (split once by k_block_z);
+        // Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
+        for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
+            const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
+            *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
         }
         for (uint32_t idx = tid; idx < XCOLS * (KSTEP / 4); idx += blockDim.x) {
             const uint32_t c = idx % XCOLS, s4 = (idx / XCOLS) * 4;

@@ -43,6 +43,7 @@ struct RankMat {
     uint64_t *Eta;     // row operations of the last block elimination (ETA_* layout)
     int8_t *Al;        // int8 limb planes of Ab (MFMA path)
     int8_t *Xl;        // int8 limb planes of the shifted X (MFMA path)
+    int8_t *Zl;        // int8 limb planes of Z, rows K-contiguous (MFMA path)
     uint32_t n, ld;
     uint32_t rank, col;      // pivots found; first column not yet eliminated
     uint32_t first_nz_col;   // smallest panel column with a nonzero residual
@@ -94,7 +95,7 @@ struct RankLaunch {
 // outstanding-memory counts for the compiler's waits).
 constexpr uint32_t TRASH_WORDS = 1024;
 
-constexpr uint32_t GATHER_ROWS = 256;  // rows per gather task
+constexpr uint32_t GATHER_ROWS = 64;   // rows per gather task
 constexpr uint32_t XCOLS = 64;         // columns per X task
 constexpr uint32_t UT = 64;            // trailing-update tile edge
 constexpr uint32_t KSTEP = 64;         // int8 K per MFMA step (one digit plane of a panel)

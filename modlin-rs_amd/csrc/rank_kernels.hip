@@ -104,12 +104,22 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
             const uint64_t e = U[row][j];
             if (sub == 0) Et[k][row] = e;
             if (e != 0) {
+                // Lane `sub` owns the row's columns j + 1 + sub + 16 m (m < 4); the
+                // slot with offset nT is the own column.  All reads are issued before
+                // the independent products.
                 const uint32_t nT = w - j - 1;
-                for (uint32_t it = sub; it <= nT; it += 16) {
-                    const uint32_t c = it < nT ? j + 1 + it : COWN;
-                    const uint64_t y = it < nT ? U[k][c] : 0;
-                    U[row][c] = ff_step_t<MODE>(f, pi, U[row][c], e, y);
+                uint64_t xv[4], yv[4];
+                uint32_t cc[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const uint32_t it = sub + 16 * m;
+                    cc[m] = it < nT ? j + 1 + it : COWN;
+                    yv[m] = it < nT ? U[k][cc[m]] : 0;
+                    xv[m] = U[row][cc[m]];
                 }
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    if (sub + 16 * m <= nT) U[row][cc[m]] = ff_step_t<MODE>(f, pi, xv[m], e, yv[m]);
             }
         }
         if (tid == 0) {
@@ -189,13 +199,30 @@ __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Balanced signed digits of Z[k][s] for the matrix-core X product (L > 0):
+// Zl[k][la*64 + s] = digit la, zero outside the pb x pb block.
+__device__ __forceinline__ void z_digits(int8_t *Zl, uint32_t L, uint32_t k, uint32_t s, uint64_t z, uint64_t p) {
+    int64_t v = balanced(z, p);
+    for (uint32_t la = 0; la < L; la++) {
+        const int8_t d = (int8_t)(v & 0xff);
+        Zl[(size_t)k * L * KSTEP + la * KSTEP + s] = d;
+        v = (v - d) >> 8;
+    }
+}
+
 template <int MODE>
-__global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, FieldParams f, uint32_t par) {
+__global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, FieldParams f, uint32_t par,
+                                                               uint32_t L) {
     RankMat &S = mats[blockIdx.x / ZG];
     if (S.done) return;
     const uint32_t pb = S.it[par].pb;
     const uint32_t s0 = (blockIdx.x % ZG) * ZSLOTS;
-    if (s0 >= pb) return;  // uniform over the workgroup
+    if (s0 >= pb) {  // uniform over the workgroup: no slots, only zero digit columns
+        if (L)
+            for (uint32_t idx = threadIdx.x; idx < BMAX * ZSLOTS; idx += blockDim.x)
+                z_digits(S.Zl, L, idx / ZSLOTS, s0 + idx % ZSLOTS, 0, f.p);
+        return;
+    }
     __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
     __shared__ uint64_t Ur[BMAX][BMAX + 1];  // U[r][t], t > r (padded: the back substitution reads columns)
     __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
@@ -238,7 +265,11 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
         fv = lane > s ? ev : (lane == s ? my_own : 0);
     }
     __syncthreads();  // the inverses are in LDS
-    if (wave == ZSLOTS || s >= pb) return;
+    if (wave == ZSLOTS) return;
+    if (s >= pb) {
+        if (L) z_digits(S.Zl, L, lane, s, 0, p);
+        return;
+    }
 
     // Back substitution, column-oriented: once z[t] is known every lane l < t adds
     // U[l][t] z[t] to its sum; lane t-1 is then complete.
@@ -251,6 +282,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
         if (lane < (uint32_t)t) acc = add_mod(acc, mul_mod_t<MODE>(f, u, zt), p);
     }
     if (lane < pb) S.Z[lane * BMAX + s] = z;
+    if (L) z_digits(S.Zl, L, lane, s, lane < pb ? z : 0, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,16 +567,17 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par) {
+void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
+                  uint32_t L) {
     if (f.mode == MODE_M61) {
         hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else if (f.mode == MODE_M31) {
         hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else {
         hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
-        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par);
+        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     }
 }
 

@@ -1,0 +1,59 @@
+// The GPU field arithmetic (modlin-rs_amd/csrc/fp_dev.hpp) on the CPU, against
+// exact 128-bit remainders: the fused Mersenne step lin2_m61 (b x + d y), the
+// fraction-free step ff_step_t in every reduction mode, and mul_mod_t.  Inputs are
+// random canonical residues plus the corners 0, 1, p - 1, 2^31 - 1, 2^31, 2^32.
+// Built and run by tests/test_limb.py; prints "bad=0" and exits 0 on success.
+#include <stdint.h>
+#include <stdio.h>
+
+#include <random>
+
+#include "fp_dev.hpp"
+
+using namespace mdl;
+typedef unsigned __int128 u128;
+
+static std::mt19937_64 rng(777);
+static int bad = 0;
+
+static uint64_t pick(uint64_t p, int i) {
+    const uint64_t corners[] = {0, 1, p - 1, p - 2, 0x7fffffffull % p, 0x80000000ull % p, 0x100000000ull % p};
+    return i < 7 ? corners[i] : rng() % p;
+}
+
+template <int MODE>
+static void run(uint64_t p, int n) {
+    const FieldParams f = make_params(p);
+    for (int it = 0; it < n; it++) {
+        const uint64_t pi = pick(p, it % 23), x = pick(p, (it / 23) % 11), e = pick(p, (it / 7) % 13), y = pick(p, it % 9);
+        const uint64_t want = (uint64_t)(((u128)pi * x % p + (u128)(p - e) % p * y % p) % p);
+        const uint64_t got = ff_step_t<MODE>(f, pi, x, e, y);
+        if (got != want) {
+            if (bad < 8) printf("ff_step mismatch p=%llu\n", (unsigned long long)p);
+            bad++;
+        }
+        const uint64_t m = mul_mod_t<MODE>(f, x, y);
+        if (m != (uint64_t)((u128)x * y % p)) {
+            if (bad < 8) printf("mul_mod mismatch p=%llu\n", (unsigned long long)p);
+            bad++;
+        }
+        if (MODE == MODE_M61) {
+            const uint64_t l = lin2_m61(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)e & 0x7fffffffu,
+                                        (uint32_t)(e >> 31));
+            if (l != (uint64_t)(((u128)pi * x + (u128)e * y) % p)) {
+                if (bad < 8) printf("lin2_m61 mismatch\n");
+                bad++;
+            }
+        }
+    }
+}
+
+int main() {
+    run<MODE_M61>(P61, 2000000);
+    run<MODE_M31>(P31, 500000);
+    const uint64_t primes[] = {2ull, 3ull, 65521ull, 4294967291ull, 1099511627689ull, 4611686018427387847ull,
+                               9223372036854775783ull, P61, P31};
+    for (uint64_t p : primes) run<MODE_GENERIC>(p, 200000);
+    printf("bad=%d\n", bad);
+    return bad != 0;
+}
