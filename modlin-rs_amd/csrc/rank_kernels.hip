@@ -32,6 +32,13 @@
 
 namespace mdl {
 
+// The value of lane `src` (wave-uniform) in every lane.
+__device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // ---------------------------------------------------------------------------
 // k_block_ge: pivot search and fraction-free LU of the panel part of the top block
 // ---------------------------------------------------------------------------
@@ -64,29 +71,48 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     }
     for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
     __syncthreads();
-    for (uint32_t idx = tid; idx < BMAX * UW; idx += blockDim.x) {
-        const uint32_t q = idx / UW, c = idx - q * UW;
-        uint64_t v = 0;
-        if (q < Bt) {
-            if (c < w) v = S.M[(size_t)rowid[q] * ld + col + c];
-            else if (c == COWN) v = 1 % p;
+    {
+        // 16 lanes per block row, 4 panel columns each: all loads issued first.
+        const uint32_t q = tid >> 4, c0 = tid & 15;
+        const uint64_t *src = S.M + (size_t)rowid[q < Bt ? q : 0] * ld + col;
+        uint64_t v[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = c0 + 16 * m;
+            v[m] = (q < Bt && c < w) ? src[c] : 0;
         }
-        U[q][c] = v;
+#pragma unroll
+        for (int m = 0; m < 4; m++) U[q][c0 + 16 * m] = v[m];
+        if (c0 == 0) U[q][COWN] = q < Bt ? 1 % p : 0;
     }
     __syncthreads();
 
-    const uint32_t lane = tid & 63;
-    const uint32_t row = tid >> 4, sub = tid & 15;
+    // Wave v owns the panel columns v, v + 16, v + 32, v + 48 (and, at step j, the
+    // own column if v = j mod 16); lane = block row.  A step reads column j (for the
+    // ballot, and as this lane's multiplier e), and speculatively its own columns of
+    // rows `lane` and k, all at once; the pivot comes by readlane.
+    const uint32_t lane = tid & 63, wave = tid >> 6;
     uint32_t k = 0;
     for (uint32_t j = 0; j < w && k < Bt; j++) {
-        // Pivot search: first block position q >= k with a nonzero in column j.
-        // Every wave evaluates the same ballot, so no barrier is needed.
-        bool nz = (lane >= k) && (lane < Bt) && (U[lane][j] != 0);
-        uint64_t mask = __ballot(nz);
+        uint64_t colv = U[lane][j];
+        const bool own = wave == (j & 15);
+        uint64_t xv[5], yv[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = wave + 16 * m;
+            if (c > j && c < w) {  // uniform over the wave
+                xv[m] = U[lane][c];
+                yv[m] = U[k][c];
+            }
+        }
+        if (own) xv[4] = U[lane][COWN];
+        // Pivot search: first block position q >= k with a nonzero in column j
+        // (rows at or beyond Bt are zero).  Every wave evaluates the same ballot.
+        const uint64_t mask = __ballot(lane >= k && colv != 0);
         if (mask == 0) continue;  // uniform: no pivot in this column
-        uint32_t q = __ffsll((unsigned long long)mask) - 1;
-        if (q != k) {  // swap block positions k and q (fp.rs:158-162)
-            __syncthreads();  // every wave has read column j for its ballot before rows move
+        const uint32_t q = __ffsll((unsigned long long)mask) - 1;
+        if (q != k) {  // swap block positions k and q (fp.rs:158-162), then read again
+            __syncthreads();  // every wave has read its values before rows move
             for (uint32_t c = tid; c < UW; c += blockDim.x) {
                 uint64_t t = U[k][c]; U[k][c] = U[q][c]; U[q][c] = t;
             }
@@ -94,33 +120,32 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
                 uint32_t r = rowid[k]; rowid[k] = rowid[q]; rowid[q] = r;
             }
             __syncthreads();
+            colv = U[lane][j];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const uint32_t c = wave + 16 * m;
+                if (c > j && c < w) {
+                    xv[m] = U[lane][c];
+                    yv[m] = U[k][c];
+                }
+            }
+            if (own) xv[4] = U[lane][COWN];
         }
         // Fraction-free elimination below the pivot (fp.rs:166-174 without the
         // division): x <- pi x - e y on the panel columns after j for every row
         // under k with e = U[row][j] != 0; own scales by pi.  Row k is only read
         // and column j is never written, so no barrier separates reads and writes.
-        const uint64_t pi = U[k][j];
-        if (row > k && row < Bt) {
-            const uint64_t e = U[row][j];
-            if (sub == 0) Et[k][row] = e;
-            if (e != 0) {
-                // Lane `sub` owns the row's columns j + 1 + sub + 16 m (m < 4); the
-                // slot with offset nT is the own column.  All reads are issued before
-                // the independent products.
-                const uint32_t nT = w - j - 1;
-                uint64_t xv[4], yv[4];
-                uint32_t cc[4];
+        const uint64_t pi = lane_bcast(colv, k);
+        const uint64_t e = colv;
+        const bool act = lane > k && lane < Bt;
+        if (wave == 0 && act) Et[k][lane] = e;
+        if (act && e != 0) {
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    const uint32_t it = sub + 16 * m;
-                    cc[m] = it < nT ? j + 1 + it : COWN;
-                    yv[m] = it < nT ? U[k][cc[m]] : 0;
-                    xv[m] = U[row][cc[m]];
-                }
-#pragma unroll
-                for (int m = 0; m < 4; m++)
-                    if (sub + 16 * m <= nT) U[row][cc[m]] = ff_step_t<MODE>(f, pi, xv[m], e, yv[m]);
+            for (int m = 0; m < 4; m++) {
+                const uint32_t c = wave + 16 * m;
+                if (c > j && c < w) U[lane][c] = ff_step_t<MODE>(f, pi, xv[m], e, yv[m]);
             }
+            if (own) U[lane][COWN] = ff_step_t<MODE>(f, pi, xv[4], e, 0);
         }
         if (tid == 0) {
             Jl[k] = j;
@@ -192,12 +217,6 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // wave per workgroup inverts the pivots (Fermat, in lockstep) while the slot
 // waves run their forward chains; the back substitution waits for it.
 // ---------------------------------------------------------------------------
-// The value of lane `src` (wave-uniform) in every lane.
-__device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // Balanced signed digits of Z[k][s] for the matrix-core X product (L > 0):
 // Zl[k][la*64 + s] = digit la, zero outside the pb x pb block.
@@ -230,8 +249,33 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t *eta = S.Eta;
     const uint64_t p = f.p;
-    for (uint32_t idx = s0 * BMAX + tid; idx < pb * BMAX; idx += blockDim.x) Ek[idx / BMAX][idx % BMAX] = eta[ETA_E + idx];
-    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) Ur[idx / BMAX][idx % BMAX] = eta[ETA_U + idx];
+    {
+        // Ek rows s0 .. pb-1 and Ur rows 0 .. pb-1 in 16-byte pieces, all loads issued
+        // before the LDS stores: at most (2 BMAX x BMAX / 2) / blockDim pieces each.
+        constexpr uint32_t NT = 64 * (ZSLOTS + 1), HALF = BMAX * BMAX / 2, NP = (2 * HALF + NT - 1) / NT;
+        const uint32_t ne = (pb - s0) * (BMAX / 2), nu = pb * (BMAX / 2);
+        const ulonglong2 *ge = (const ulonglong2 *)(eta + ETA_E + s0 * BMAX);
+        const ulonglong2 *gu = (const ulonglong2 *)(eta + ETA_U);
+        ulonglong2 v[NP];
+#pragma unroll
+        for (uint32_t i = 0; i < NP; i++) {
+            const uint32_t idx = tid + i * NT;
+            v[i] = idx < ne ? ge[idx] : (idx >= HALF && idx - HALF < nu) ? gu[idx - HALF] : ulonglong2{0, 0};
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < NP; i++) {
+            const uint32_t idx = tid + i * NT;
+            if (idx < ne) {
+                const uint32_t r = s0 + (2 * idx) / BMAX, c = (2 * idx) % BMAX;
+                Ek[r][c] = v[i].x;
+                Ek[r][c + 1] = v[i].y;
+            } else if (idx >= HALF && idx - HALF < nu) {
+                const uint32_t r = (2 * (idx - HALF)) / BMAX, c = (2 * (idx - HALF)) % BMAX;
+                Ur[r][c] = v[i].x;
+                Ur[r][c + 1] = v[i].y;
+            }
+        }
+    }
     if (tid < BMAX) {
         piK[tid] = tid < pb ? eta[ETA_PI + tid] : 1;
         ownK[tid] = tid < pb ? eta[ETA_OWNK + tid] : 0;
