@@ -88,7 +88,8 @@ typedef struct modlin_opts {
     uint32_t method; /* modlin_method */
     uint32_t panel;  /* panel width (0 = tuned default, else 1..64) */
     uint32_t flags;  /* MODLIN_FLAG_* */
-    uint32_t reserved;
+    uint32_t coop_min; /* BM: a batch of one sequence at least this long runs cooperatively on
+                          the whole GPU (0 = default, 65536) */
 } modlin_opts_t;
 
 /* Counters of the last rank / BM call on a context. */
@@ -104,6 +105,7 @@ typedef struct modlin_stats {
     double update_bytes;    /* algorithmic HBM bytes of the trailing updates: 16 per updated
                                element (read + write) + 8 per operand element per tile */
     double graph;           /* 1 if the rank loop replayed a captured hipGraph */
+    double bm_coop;         /* workgroups of the cooperative BM kernel (0: one per sequence) */
 } modlin_stats_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */

@@ -11,7 +11,14 @@
 //     (fp.rs:292-294, 301-303); on a length change (2l <= i, fp.rs:287-299) the old
 //     c[0..l] is first copied to t and b/t swap roles by pointer.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
 #include "engine.hpp"
+#include "rank_common.hpp"
 
 namespace mdl {
 
@@ -117,6 +124,220 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_bm_coop: one long sequence, cooperatively on the whole GPU.
+//
+// G workgroups each own a contiguous slice [lo, hi) of the indices of c, b and t.
+// Per step a workgroup forms the partial discrepancy sum_{j in slice, 1<=j<=l}
+// c[j] s[i-j] and publishes it; one grid barrier later every workgroup adds the G
+// partials (addition mod p is associative, so d is the reference's sum) and
+// applies the axpy c[j] -= coef b[j-m] to its own slice.  The scalar state
+// (l, bl, m, 1/bb) is replicated: every workgroup derives it from the same d.
+//
+// Memory: a slice of c is only touched by its owner and s is read-only, so both
+// stay in the ordinary caches.  b and t cross slices (b is read shifted by m; t
+// becomes b at a length change) and the partials cross workgroups: they are
+// accessed with device-scope atomic loads and stores, which are coherent across
+// the XCDs' L2s, so no L2 write-back or invalidation is needed at the barrier --
+// only a wait for each wave's own stores before its workgroup arrives.
+//
+// The launch is cooperative, so all workgroups are resident and every barrier
+// completes; each wait is still bounded (BM_SPIN_LIMIT polls), and a wait that
+// runs out sets the error flag instead of hanging.
+// ---------------------------------------------------------------------------
+struct BmCoop {
+    unsigned int arrive;       // grid barrier arrivals (monotonic)
+    unsigned int timeout;      // set if a barrier wait ran out
+    uint64_t part[2][1024];  // partial discrepancies, double-buffered by step parity
+};
+constexpr uint32_t BM_COOP_THREADS = 512;
+constexpr uint32_t BM_COOP_MIN = 65536;  // default length from which one sequence runs cooperatively
+constexpr uint32_t BM_SPIN_LIMIT = 1u << 26;
+
+__device__ __forceinline__ uint64_t ld_dev(gptr<const uint64_t> a) {
+    return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(gptr<uint64_t> a, uint64_t v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void coop_barrier(BmCoop *cs, unsigned int target) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's device-scope stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(&cs->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&cs->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins == BM_SPIN_LIMIT) {
+                __hip_atomic_store(&cs->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldParams f, BmCoop *cs) {
+    constexpr int NT = BM_COOP_THREADS;
+    constexpr int PF = 4;  // prefetched b operands per thread
+    BmSeq &Q = *seq;
+    const uint32_t n = Q.n, G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const gptr<const uint64_t> s = to_global(Q.s);
+    gptr<uint64_t> c = to_global(Q.c), b = to_global(Q.b), t = to_global(Q.t);
+    gptr<uint64_t> part = to_global(&cs->part[0][0]);
+    const uint64_t p = f.p, sb = f.safe_batch;
+    const uint32_t span = (n + G - 1) / G;
+    const uint32_t lo = min(n, g * span), hi = min(n, lo + span);
+    __shared__ uint64_t red[NT / 64];
+    __shared__ uint64_t sh_d;
+    unsigned int bars = 0;
+    const bool tagged = f.p <= P61;  // partials below 2^61 leave 3 bits for a step tag
+
+    for (uint32_t j = lo + tid; j < hi; j += NT) {
+        c[j] = j == 0 ? 1 % p : 0;
+        st_dev(b + j, j == 0 ? 1 % p : 0);
+        st_dev(t + j, 0);
+    }
+    coop_barrier(cs, ++bars * G);  // b is initialised everywhere
+    // The reference's state (fp.rs:267-270) with the saved discrepancy bb itself in
+    // place of its inverse: the update is the inversionless c <- bb c - d x^m b,
+    // which scales c (c[0] included), b and every later discrepancy by nonzero
+    // factors, so the zero tests, the length changes and l are the reference's.
+    uint32_t l = 0, bl = 1, m = 1;
+    uint64_t bb = 1 % p;
+    bool fresh_b = false;  // b was rewritten at the previous step (no prefetch)
+    unsigned long long bytes = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t jmax = min(bl, n - m);
+        const uint32_t a0 = max(lo, m), a1 = min(hi, m + jmax);
+        // this thread's indices are j = lo + tid + q NT; the axpy ones (j in [a0, a1))
+        // start at q = q0, and the first PF of them are prefetched
+        const uint32_t first = lo + tid;
+        const uint32_t q0 = a0 > first ? (a0 - first + NT - 1) / NT : 0;
+        uint64_t bpre[PF];
+        if (!fresh_b) {
+#pragma unroll
+            for (int r = 0; r < PF; r++) {
+                const uint32_t j = first + (q0 + r) * NT;
+                bpre[r] = j < a1 ? ld_dev(b + (j - m)) : 0;
+            }
+        }
+        // this slice's part of d = sum_{j=0..l} c[j] s[i-j] (fp.rs:274-278 with c[0],
+        // which the inversionless update scales, in place of the reference's 1)
+        uint64_t ah = 0, al = 0, mine = 0, cnt = 0;
+        const uint32_t j1 = min(hi, l + 1);
+        for (uint32_t j = lo + tid; j < j1; j += NT) {
+            uint64_t h, lw;
+            mul_wide(c[j], s[i - j], h, lw);
+            add128(ah, al, h, lw);
+            if (++cnt == sb) {
+                mine = add_mod(mine, reduce_t<MODE>(f, ah, al), p);
+                ah = al = 0;
+                cnt = 0;
+            }
+        }
+        if (cnt) mine = add_mod(mine, reduce_t<MODE>(f, ah, al), p);
+        mine = block_sum_mod<NT>(mine, p, red);
+        const gptr<uint64_t> slot = part + (i & 1) * 1024;
+        if (tagged) {
+            // Publish the partial with the step's tag in bits 61..63 (the slot of
+            // this parity was last written two steps ago, with the previous tag;
+            // tags start at 1, so the zeroed slots never match); wave 0 then polls
+            // every slot until it carries this step's tag.  One round trip.
+            const uint64_t tag = (uint64_t)(((i >> 1) + 1) & 7) << 61;
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's t stores are complete
+            __syncthreads();
+            if (tid == 0) st_dev(slot + g, mine | tag);
+            if (tid < 64) {
+                uint64_t v = 0;
+                for (uint32_t w = lane; w < G; w += 64) {
+                    uint64_t x = ld_dev(slot + w);
+                    uint32_t spins = 0;
+                    while ((x & ~P61) != tag) {
+                        if (++spins == BM_SPIN_LIMIT) {
+                            __hip_atomic_store(&cs->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        x = ld_dev(slot + w);
+                    }
+                    v = add_mod(v, x & P61, p);
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v = add_mod(v, __shfl_xor(v, off, 64), p);
+                if (tid == 0) sh_d = v;
+            }
+        } else {
+            if (tid == 0) st_dev(slot + g, mine);
+            coop_barrier(cs, ++bars * G);
+            if (tid < 64) {
+                uint64_t v = 0;
+                for (uint32_t w = lane; w < G; w += 64) v = add_mod(v, ld_dev(slot + w), p);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v = add_mod(v, __shfl_xor(v, off, 64), p);
+                if (tid == 0) sh_d = v;
+            }
+        }
+        __syncthreads();
+        const uint64_t d = sh_d;
+        bytes += 8ull + 16ull * l;
+        if (d == 0) {  // fp.rs:279-282
+            m++;
+            fresh_b = false;
+            continue;
+        }
+        if (fresh_b) {
+#pragma unroll
+            for (int r = 0; r < PF; r++) {
+                const uint32_t j = first + (q0 + r) * NT;
+                bpre[r] = j < a1 ? ld_dev(b + (j - m)) : 0;
+            }
+        }
+        // c[j] <- bb c[j] - d b[j - m] on this slice (the b term for j in [m, m + jmax))
+        const bool change = 2 * l <= i;  // length change (fp.rs:287-299): t <- c[0..l] first
+        const uint32_t old_l = l;
+        bytes += 24ull * jmax + (change ? 16ull * (old_l + 1) : 0);
+        if (change)
+            for (uint32_t j = lo + tid; j < min(hi, old_l + 1); j += NT) st_dev(t + j, c[j]);
+        const uint32_t top = min(hi, max(l + 1, m + jmax));
+        uint32_t q = 0;
+        for (uint32_t j = first; j < top; j += NT, q++) {
+            uint64_t y = 0;
+            if (j >= a0 && j < a1) {
+                const uint32_t k = q - q0;  // this thread's k-th axpy element
+                if (k < (uint32_t)PF) {
+#pragma unroll
+                    for (int r = 0; r < PF; r++)
+                        if (r == (int)k) y = bpre[r];
+                } else {
+                    y = ld_dev(b + (j - m));
+                }
+            }
+            c[j] = ff_step_t<MODE>(f, bb, c[j], d, y);
+        }
+        if (change) {
+            const gptr<uint64_t> tmp = b;
+            b = t;
+            t = tmp;
+            bl = old_l + 1;
+            bb = d;
+            l = i + 1 - old_l;
+            m = 1;
+            fresh_b = true;
+        } else {
+            m++;
+            fresh_b = false;
+        }
+        __syncthreads();
+    }
+    if (g == 0 && tid == 0) {
+        Q.lc = l;
+        Q.bytes = bytes;
+    }
+}
+
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 __global__ void k_widen_seq(const uint32_t *src, uint64_t *dst, size_t count) {
@@ -151,6 +372,8 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     }
     const size_t off_stage = cur;
     cur = align_up(cur + maxn * 4, 256);
+    const size_t off_coop = cur;
+    cur = align_up(cur + sizeof(BmCoop), 256);
     int rc = arena_reserve(ctx, cur);
     if (rc) return rc;
     char *base = (char *)ctx->arena;
@@ -183,6 +406,48 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     rc = copy_h2d(ctx, ds, hs.data(), count * sizeof(BmSeq));
     if (rc) return rc;
     if (profile) MDL_HIP(hipEventRecord(ctx->evA, st));
+    // One long sequence: the cooperative kernel over the whole GPU (one workgroup
+    // per CU, at most 1024 partials).
+    const uint32_t coop_min = (opts && opts->coop_min) ? opts->coop_min : BM_COOP_MIN;
+    if (count == 1 && maxn >= coop_min) {
+        BmCoop *cs = (BmCoop *)(base + off_coop);
+        MDL_HIP(hipMemsetAsync(cs, 0, sizeof(BmCoop), st));
+        uint32_t G = (uint32_t)std::min(ctx->cu_count, 1024);
+        if (const char *env = getenv("MODLIN_TUNE")) {  // "bm=G" (experiments)
+            unsigned v;
+            const char *q = strstr(env, "bm=");
+            if (q && sscanf(q + 3, "%u", &v) == 1 && v >= 1 && v <= 1024) G = v;
+        }
+        void *args[] = {(void *)&ds, (void *)&f, (void *)&cs};
+        hipError_t e;
+        if (f.mode == MODE_M61)
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M61>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
+        else if (f.mode == MODE_M31)
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M31>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
+        else
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_GENERIC>, dim3(G), dim3(BM_COOP_THREADS), args, 0,
+                                           st);
+        MDL_HIP(e);
+        if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+        uint32_t timeout = 0;
+        rc = copy_d2h(ctx, &timeout, &cs->timeout, sizeof(timeout));
+        if (rc) return rc;
+        if (timeout) {
+            ctx->err = "cooperative Berlekamp-Massey: a grid barrier wait ran out";
+            return MODLIN_EHIP;
+        }
+        rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));
+        if (rc) return rc;
+        lcs[0] = hs[0].lc;
+        ctx->stats.bm_bytes = (double)hs[0].bytes;
+        ctx->stats.bm_coop = (double)G;
+        if (profile) {
+            float ms = 0;
+            MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
+            ctx->stats.bm_ms = ms;
+        }
+        return MODLIN_OK;
+    }
     const bool wide = count == 1 && maxn >= 4096;
 #define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
     if (f.mode == MODE_M61) {

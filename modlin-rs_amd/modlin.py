@@ -59,13 +59,13 @@ class _MatDesc(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_uint32), ("panel", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("coop_min", ctypes.c_uint32)]
 
 
 class _Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("macs", "update_ms", "update_launches", "limb_count", "iterations", "bm_bytes", "bm_ms",
-                 "total_ms", "update_bytes", "graph")]
+                 "total_ms", "update_bytes", "graph", "bm_coop")]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
@@ -244,10 +244,12 @@ class Context:
                                                 _u64p(out)), "rank_batch")
         return [int(x) for x in out[:len(mats)]]
 
-    def lc_batch(self, field: Field, seqs: Sequence, profile: bool = False) -> list[int]:
+    def lc_batch(self, field: Field, seqs: Sequence, profile: bool = False, coop_min: int = 0) -> list[int]:
+        """Linear complexities of a batch.  A batch of one sequence of length at
+        least coop_min (0: the library default) runs cooperatively on the whole GPU."""
         arr, keep = self._descs(seqs, False)
         out = np.zeros(max(1, len(seqs)), dtype=np.uint64)
-        opts = _Opts(0, 0, FLAG_PROFILE if profile else 0, 0)
+        opts = _Opts(0, 0, FLAG_PROFILE if profile else 0, coop_min)
         self._check(self._lib.modlin_lc_batch(self._h, ctypes.byref(field._f), arr, len(seqs), ctypes.byref(opts),
                                               _u64p(out)), "lc_batch")
         return [int(x) for x in out[:len(seqs)]]

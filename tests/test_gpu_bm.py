@@ -84,3 +84,34 @@ def test_readme_mixmax256_long(ctx):
     s = workloads.c5_bm(200_000)
     print("BM n=200000 MIXMAX-256", flush=True)
     assert ctx.linear_complexity(modlin.Field(M61), s) == 65280   # README.md:107
+
+
+@pytest.mark.parametrize("p", [2, 65521, (1 << 31) - 1, M61, (1 << 63) - 25])
+def test_cooperative_matches_oracle(ctx, p):
+    """One sequence on the whole GPU (forced down to short lengths with coop_min):
+    uniform, recurrences of known order, constant and zero sequences, MIXMAX."""
+    rng = np.random.default_rng(11)
+    f = modlin.Field(p)
+    cases = [("uniform", rng.integers(0, p, size=5000, dtype=np.uint64))]
+    for L in [1, 37, 700]:
+        coef = rng.integers(1, p, size=L, dtype=np.uint64)
+        init = rng.integers(0, p, size=L, dtype=np.uint64)
+        cases.append((f"lfsr{L}", modlin.lfsr(p, coef, init, 2 * L + 3000)))
+    cases.append(("const", np.full(3000, 1 % p, dtype=np.uint64)))
+    cases.append(("zero", np.zeros(2500, dtype=np.uint64)))
+    cases.append(("mixmax17", (modlin.prng_fill(modlin.PRNG_MIXMAX17, 5, 4000) % np.uint64(p)).astype(np.uint64)))
+    for label, s in cases:
+        print("coop BM", p, label, s.size, flush=True)
+        got = ctx.lc_batch(f, [s], coop_min=1000)
+        assert ctx.stats()["bm_coop"] > 0, label
+        assert got[0] == oracle.lc(p, s), (p, label)
+
+
+def test_cooperative_equals_single_workgroup(ctx):
+    s = (modlin.prng_fill(modlin.PRNG_XOROSHIRO128PP, 3, 20000) % np.uint64(M61)).astype(np.uint64)
+    f = modlin.Field(M61)
+    coop = ctx.lc_batch(f, [s], coop_min=1)
+    assert ctx.stats()["bm_coop"] > 0
+    single = ctx.lc_batch(f, [s], coop_min=1 << 31)
+    assert ctx.stats()["bm_coop"] == 0
+    assert coop == single == [oracle.lc(M61, s)]
