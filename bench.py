@@ -76,7 +76,7 @@ def traffic_per_launch():
     """HBM bytes per launch of the MFMA trailing update on this workload, from
     the committed counter summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
     same counters on a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "k_update_mfma_batched_rank_64_p61_counters.txt")
+    path = os.path.join(ROOT, "profiles", "update_kernels_batched_rank_64_p61_counters.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
