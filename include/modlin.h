@@ -19,6 +19,10 @@
  *   modlin_pretty_p_value     stats::pretty_p_value          src/stats.rs:129-137
  *   modlin_prng_fill          Prng::try_new + next_u64       src/prng.rs:46-343
  *   modlin_prng_fill_device   the fill loops, on the device   main.rs:94-105, 153-155
+ *   modlin_ctx_set_tuning, modlin_ctx_load_tuning, modlin_tuning_select
+ *                             no counterpart: launch parameters of this library, chosen by
+ *                             modlin-rs_amd/autotune.py (the reference's PANEL = 64 and
+ *                             TILE = 256 constants, src/fp.rs:26-32, are fixed)
  *
  * Ownership: the caller owns every buffer.  modlin_rank leaves m unspecified
  * (the reference also destroys it, holding partial factors).  A context owns
@@ -108,6 +112,19 @@ typedef struct modlin_stats {
     double bm_coop;         /* workgroups of the cooperative BM kernel (0: one per sequence) */
 } modlin_stats_t;
 
+/* Launch parameters of the rank and BM engines.  A zero field means "from the
+   context's tuning table, else the built-in default".  No value changes a result:
+   rank and linear complexity do not depend on blocking or on launch shapes. */
+typedef struct modlin_tuning {
+    uint32_t far_per_cu;  /* far-update workgroups per CU (default 4 MFMA, 8 VALU) */
+    uint32_t near_per_cu; /* near-update workgroups per CU (default 2) */
+    uint32_t gather_wgs;  /* workgroups of the gather / X-product kernel (default 1024) */
+    uint32_t check_every; /* panel iterations between host checks, even (default 8) */
+    uint32_t panel;       /* panel width 1..64 (default 64; modlin_opts.panel overrides) */
+    uint32_t bm_coop_wgs; /* workgroups of the cooperative BM, 1..1024 (default: one per CU) */
+    uint32_t reserved[2];
+} modlin_tuning_t;
+
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */
 int modlin_field_init(uint64_t p, modlin_field_t *out);
 uint64_t modlin_reduce(const modlin_field_t *f, uint64_t x);
@@ -118,6 +135,23 @@ int modlin_ctx_create(int device, modlin_ctx_t **out);
 void modlin_ctx_destroy(modlin_ctx_t *ctx);
 const char *modlin_ctx_last_error(const modlin_ctx_t *ctx);
 int modlin_ctx_last_stats(const modlin_ctx_t *ctx, modlin_stats_t *out);
+
+/* Tuning.  A table file has one entry per line,
+ *     <rank|bm> <mode> <side> <count> far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs
+ * with mode m61 (p = 2^61-1), m31 (p = 2^31-1) or gL (other p, L int8 digits per
+ * element), side the power of two at or above the largest n of a call (at least
+ * 256), count 1 or many; '#' starts a comment.  A call uses the entry of its kind,
+ * mode and count with the nearest side, else built-in defaults.
+ * modlin_ctx_create loads tuning.txt from the library's directory if it exists.
+ * modlin_ctx_load_tuning(ctx, NULL) reloads it, (ctx, "") clears the table, any
+ * other path loads that file (MODLIN_EINVAL if it cannot be read or parsed).
+ * modlin_ctx_set_tuning(ctx, &t) makes t's nonzero fields override the table for
+ * every later call; NULL removes the override.
+ * modlin_tuning_select applies the same selection to a table file without a
+ * context (no GPU needed); bm != 0 selects a Berlekamp-Massey entry. */
+int modlin_ctx_set_tuning(modlin_ctx_t *ctx, const modlin_tuning_t *t);
+int modlin_ctx_load_tuning(modlin_ctx_t *ctx, const char *path);
+int modlin_tuning_select(const char *path, uint64_t p, uint64_t max_n, uint64_t count, int bm, modlin_tuning_t *out);
 
 /* --- Hot paths ------------------------------------------------------------- */
 int modlin_rank(modlin_ctx_t *ctx, const modlin_field_t *f, uint64_t *m, uint64_t n, modlin_progress_fn progress,

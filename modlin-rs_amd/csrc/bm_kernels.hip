@@ -11,9 +11,6 @@
 //     (fp.rs:292-294, 301-303); on a length change (2l <= i, fp.rs:287-299) the old
 //     c[0..l] is first copied to t and b/t swap roles by pointer.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
 
 #include <algorithm>
 
@@ -192,7 +189,9 @@ __global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldPa
     const uint32_t lo = min(n, g * span), hi = min(n, lo + span);
     __shared__ uint64_t red[NT / 64];
     __shared__ uint64_t sh_d;
+    __shared__ uint32_t sh_abort;
     unsigned int bars = 0;
+    if (tid == 0) sh_abort = 0;
     const bool tagged = f.p <= P61;  // partials below 2^61 leave 3 bits for a step tag
 
     for (uint32_t j = lo + tid; j < hi; j += NT) {
@@ -258,6 +257,7 @@ __global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldPa
                     while ((x & ~P61) != tag) {
                         if (++spins == BM_SPIN_LIMIT) {
                             __hip_atomic_store(&cs->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            sh_abort = 1;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
@@ -281,6 +281,9 @@ __global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldPa
             }
         }
         __syncthreads();
+        // a wait that ran out ends the kernel: the other workgroups then run out
+        // too, at their next exchange, and the host reports the error
+        if (sh_abort || __hip_atomic_load(&cs->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         const uint64_t d = sh_d;
         bytes += 8ull + 16ull * l;
         if (d == 0) {  // fp.rs:279-282
@@ -412,12 +415,8 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     if (count == 1 && maxn >= coop_min) {
         BmCoop *cs = (BmCoop *)(base + off_coop);
         MDL_HIP(hipMemsetAsync(cs, 0, sizeof(BmCoop), st));
-        uint32_t G = (uint32_t)std::min(ctx->cu_count, 1024);
-        if (const char *env = getenv("MODLIN_TUNE")) {  // "bm=G" (experiments)
-            unsigned v;
-            const char *q = strstr(env, "bm=");
-            if (q && sscanf(q + 3, "%u", &v) == 1 && v >= 1 && v <= 1024) G = v;
-        }
+        const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
+        const uint32_t G = tu.bm_coop_wgs ? tu.bm_coop_wgs : (uint32_t)std::min(ctx->cu_count, 1024);
         void *args[] = {(void *)&ds, (void *)&f, (void *)&cs};
         hipError_t e;
         if (f.mode == MODE_M61)
@@ -427,7 +426,13 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         else
             e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_GENERIC>, dim3(G), dim3(BM_COOP_THREADS), args, 0,
                                            st);
-        MDL_HIP(e);
+        if (e != hipSuccess) {
+            // refused (another process may hold the device's cooperative-launch
+            // resources): the one-workgroup kernel below gives the same result
+            (void)hipGetLastError();
+            ctx->stats.bm_coop = 0;
+            goto single;
+        }
         if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
         uint32_t timeout = 0;
         rc = copy_d2h(ctx, &timeout, &cs->timeout, sizeof(timeout));
@@ -448,6 +453,7 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         }
         return MODLIN_OK;
     }
+single:
     const bool wide = count == 1 && maxn >= 4096;
 #define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
     if (f.mode == MODE_M61) {

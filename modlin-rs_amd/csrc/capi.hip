@@ -75,7 +75,33 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
         delete ctx;
         return MODLIN_EHIP;
     }
+    // launch parameters from tuning.txt next to the library, if one was written
+    std::vector<mdl::TuneEntry> table;
+    std::string err;
+    if (mdl::parse_tuning(mdl::default_tuning_path(), table, err)) ctx->tune_table.swap(table);
     *out = ctx;
+    return MODLIN_OK;
+}
+
+int modlin_ctx_set_tuning(modlin_ctx_t *ctx, const modlin_tuning_t *t) {
+    if (!ctx) return MODLIN_EINVAL;
+    if (t && ((t->check_every & 1) || t->panel > 64 || t->bm_coop_wgs > 1024)) {
+        ctx->err = "tuning: check_every must be even, panel <= 64, bm_coop_wgs <= 1024";
+        return MODLIN_EINVAL;
+    }
+    ctx->tune_override = t ? *t : modlin_tuning_t{};
+    return MODLIN_OK;
+}
+
+int modlin_ctx_load_tuning(modlin_ctx_t *ctx, const char *path) {
+    if (!ctx) return MODLIN_EINVAL;
+    if (path && !*path) {
+        ctx->tune_table.clear();
+        return MODLIN_OK;
+    }
+    std::vector<mdl::TuneEntry> table;
+    if (!mdl::parse_tuning(path ? std::string(path) : mdl::default_tuning_path(), table, ctx->err)) return MODLIN_EINVAL;
+    ctx->tune_table.swap(table);
     return MODLIN_OK;
 }
 

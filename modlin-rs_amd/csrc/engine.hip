@@ -131,7 +131,10 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     ctx->stats = modlin_stats_t{};
     if (count == 0) return MODLIN_OK;
     const FieldParams f = make_params(p);
-    uint32_t b = (opts && opts->panel) ? opts->panel : (uint32_t)BMAX;
+    uint64_t maxn = 0;
+    for (uint64_t i = 0; i < count; i++) maxn = std::max<uint64_t>(maxn, mats[i].n);
+    const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, false);
+    uint32_t b = (opts && opts->panel) ? opts->panel : (tu.panel ? tu.panel : (uint32_t)BMAX);
     if (b > (uint32_t)BMAX) b = BMAX;
     const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
     const uint32_t method = opts ? opts->method : (uint32_t)MODLIN_METHOD_AUTO;
@@ -243,19 +246,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     MDL_HIP(hipGetLastError());
 
     const uint32_t nmat = (uint32_t)count;
-    rl.grid_gx = 1024;
-    rl.grid_near = (uint32_t)ctx->cu_count * 2;
-    rl.grid_far = (uint32_t)ctx->cu_count * (use_mfma ? 4 : 8);  // multiples of 8 (XCD numbering)
+    rl.grid_gx = tu.gather_wgs ? tu.gather_wgs : 1024;
+    rl.grid_near = (uint32_t)ctx->cu_count * (tu.near_per_cu ? tu.near_per_cu : 2);
+    // far grid: a multiple of 8 (XCD numbering)
+    rl.grid_far = ((uint32_t)ctx->cu_count * (tu.far_per_cu ? tu.far_per_cu : (use_mfma ? 4 : 8)) + 7) / 8 * 8;
     rl.far_items = use_mfma ? 1u : 0u;
-    int check_every = 8;
-    if (const char *env = getenv("MODLIN_TUNE")) {  // "near=N,far=N,gx=N,every=N" (experiments)
-        unsigned v;
-        const char *q;
-        if ((q = strstr(env, "near=")) && sscanf(q + 5, "%u", &v) == 1 && v) rl.grid_near = (v + 7) / 8 * 8;
-        if ((q = strstr(env, "far=")) && sscanf(q + 4, "%u", &v) == 1 && v) rl.grid_far = (v + 7) / 8 * 8;
-        if ((q = strstr(env, "gx=")) && sscanf(q + 3, "%u", &v) == 1 && v) rl.grid_gx = v;
-        if ((q = strstr(env, "every=")) && sscanf(q + 6, "%u", &v) == 1 && v) check_every = (int)((v + 1) / 2 * 2);
-    }
+    const int check_every = tu.check_every ? (int)tu.check_every : 8;
     std::vector<hipEvent_t> evs;
     if (profile) {
         evs.resize(4 * check_every);

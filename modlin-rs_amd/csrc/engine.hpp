@@ -8,9 +8,13 @@
 
 #include "../../include/modlin.h"
 #include "fp_dev.hpp"
+#include "tuning.hpp"
 
 struct modlin_ctx {
     int device = 0;
+    // launch parameters: explicit override (nonzero fields) over the table
+    modlin_tuning_t tune_override{};
+    std::vector<mdl::TuneEntry> tune_table;
     hipStream_t stream = nullptr;
     std::string err;
     modlin_stats_t stats{};

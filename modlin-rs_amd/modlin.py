@@ -44,6 +44,7 @@ EXPORTS = [
     "modlin_prng_fill", "modlin_prng_name", "modlin_prng_fill_device",
     "modlin_util_combine_rows", "modlin_util_lfsr",
     "modlin_dev_alloc", "modlin_dev_free", "modlin_memcpy_h2d", "modlin_memcpy_d2h", "modlin_synchronize",
+    "modlin_ctx_set_tuning", "modlin_ctx_load_tuning", "modlin_tuning_select",
 ]
 
 
@@ -60,6 +61,13 @@ class _MatDesc(ctypes.Structure):
 class _Opts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_uint32), ("panel", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("coop_min", ctypes.c_uint32)]
+
+
+TUNING_FIELDS = ("far_per_cu", "near_per_cu", "gather_wgs", "check_every", "panel", "bm_coop_wgs")
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint32) for name in TUNING_FIELDS] + [("reserved", ctypes.c_uint32 * 2)]
 
 
 class _Stats(ctypes.Structure):
@@ -116,6 +124,9 @@ def load() -> ctypes.CDLL:
     lib.modlin_memcpy_h2d.argtypes = [vp, vp, vp, u64]
     lib.modlin_memcpy_d2h.argtypes = [vp, vp, vp, u64]
     lib.modlin_synchronize.argtypes = [vp]
+    lib.modlin_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(_Tuning)]
+    lib.modlin_ctx_load_tuning.argtypes = [vp, ctypes.c_char_p]
+    lib.modlin_tuning_select.argtypes = [ctypes.c_char_p, u64, u64, u64, ctypes.c_int, ctypes.POINTER(_Tuning)]
     _lib = lib
     return lib
 
@@ -186,6 +197,25 @@ class Context:
         if rc == MODLIN_EINVAL:
             raise ValueError(f"{what}: {msg}")
         raise ModlinError(f"{what}: status {rc}: {msg}")
+
+    # --- launch parameters (no effect on results) ------------------------
+    def set_tuning(self, **fields):
+        """Overrides the tuning table for later calls with the given nonzero
+        fields (names in TUNING_FIELDS); no arguments removes the override."""
+        if not fields:
+            self._check(self._lib.modlin_ctx_set_tuning(self._h, None), "set_tuning")
+            return
+        t = _Tuning()
+        for k, v in fields.items():
+            if k not in TUNING_FIELDS:
+                raise ValueError(f"unknown tuning field {k}")
+            setattr(t, k, int(v))
+        self._check(self._lib.modlin_ctx_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
+
+    def load_tuning(self, path: "str | None" = None):
+        """Loads a tuning table (None: tuning.txt next to the library; "": none)."""
+        self._check(self._lib.modlin_ctx_load_tuning(self._h, None if path is None else path.encode()),
+                    "load_tuning")
 
     def stats(self) -> dict:
         s = _Stats()
@@ -365,3 +395,13 @@ def lfsr(p: int, coef: np.ndarray, init: np.ndarray, n: int) -> np.ndarray:
     if rc != MODLIN_OK:
         raise ValueError("lfsr: bad arguments")
     return out[:n]
+
+
+def tuning_select(path: str, p: int, max_n: int, count: int, bm: bool = False) -> dict:
+    """The launch parameters a call (p, largest n, batch size) would take from the
+    tuning table at `path` (zero: built-in default).  Needs no GPU."""
+    t = _Tuning()
+    rc = load().modlin_tuning_select(path.encode(), p, max_n, count, 1 if bm else 0, ctypes.byref(t))
+    if rc != MODLIN_OK:
+        raise ValueError(f"cannot use tuning table {path}")
+    return {k: getattr(t, k) for k in TUNING_FIELDS}

@@ -175,3 +175,38 @@ def test_graph_replay_matches_direct(ctx, method):
         for _ in range(2):
             assert ctx.rank_batch(f, mats, method=method) == want
             assert ctx.stats()["graph"] == 1.0
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("tuning", [dict(far_per_cu=2, near_per_cu=1, check_every=4),
+                                    dict(far_per_cu=8, near_per_cu=4, gather_wgs=64, check_every=16),
+                                    dict(panel=48), dict(panel=16, far_per_cu=6)])
+def test_launch_parameters_do_not_change_ranks(ctx, method, tuning):
+    """Every tuning-table setting gives the oracle's ranks (mixed sizes, deficient
+    MIXMAX members, a planted corank)."""
+    mats = workloads.c2(count=12, max_side=600) + [workloads.c3(520, M61)]
+    print("tuning", tuning, "method", method, flush=True)
+    ctx.set_tuning(**tuning)
+    try:
+        got = _rank_all(ctx, M61, mats, method)
+    finally:
+        ctx.set_tuning()
+    for i, (a, g) in enumerate(zip(mats, got)):
+        assert g == oracle.rank(M61, a, a.shape[0]), (i, tuning)
+
+
+def test_tuning_table_loaded_and_cleared(ctx, tmp_path):
+    table = tmp_path / "tuning.txt"
+    table.write_text("rank m61 1024 many 2 1 256 4 32 0\n")
+    ctx.load_tuning(str(table))
+    try:
+        mats = workloads.c2(count=6, max_side=900)
+        got = _rank_all(ctx, M61, mats, modlin.METHOD_MFMA)
+        assert got == [oracle.rank(M61, a, a.shape[0]) for a in mats]
+    finally:
+        try:
+            ctx.load_tuning(None)   # the library's own table, if one is installed
+        except ValueError:
+            ctx.load_tuning("")
+    with pytest.raises(ValueError):
+        ctx.load_tuning(str(tmp_path / "missing.txt"))
