@@ -122,7 +122,9 @@ typedef struct modlin_tuning {
     uint32_t check_every; /* panel iterations between host checks, even (default 8) */
     uint32_t panel;       /* panel width 1..64 (default 64; modlin_opts.panel overrides) */
     uint32_t bm_coop_wgs; /* workgroups of the cooperative BM, 1..1024 (default: one per CU) */
-    uint32_t reserved[2];
+    uint32_t bm_threads;  /* threads per sequence of the batched BM: 256, 512 or 1024 (default:
+                             the most that keep the whole batch resident) */
+    uint32_t reserved[1];
 } modlin_tuning_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */
@@ -137,7 +139,7 @@ const char *modlin_ctx_last_error(const modlin_ctx_t *ctx);
 int modlin_ctx_last_stats(const modlin_ctx_t *ctx, modlin_stats_t *out);
 
 /* Tuning.  A table file has one entry per line,
- *     <rank|bm> <mode> <side> <count> far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs
+ *     <rank|bm> <mode> <side> <count> far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs bm_threads
  * with mode m61 (p = 2^61-1), m31 (p = 2^31-1) or gL (other p, L int8 digits per
  * element), side the power of two at or above the largest n of a call (at least
  * 256), count 1 or many; '#' starts a comment.  A call uses the entry of its kind,

@@ -8,8 +8,8 @@ tuning.txt from the library's directory when it is created.
 
 Search: coordinate descent from the built-in defaults, two passes over the
 parameters (far-update workgroups per CU, near-update workgroups per CU, gather
-grid, panel iterations per host check, panel width; workgroups of the
-cooperative Berlekamp-Massey).  Each candidate is timed as the best of `reps`
+grid, panel iterations per host check, panel width; threads per sequence of
+the batched Berlekamp-Massey, workgroups of the cooperative one).  Each candidate is timed as the best of `reps`
 runs after one warm-up run.  Launch parameters never change a result: every
 candidate's ranks (or linear complexity) must equal those of the untuned run,
 or the tuner stops with an error.
@@ -38,6 +38,7 @@ RANK_SPACE = {
     "panel": [48, 64],
 }
 BM_SPACE = {"bm_coop_wgs": [128, 256, 512]}
+BM_BATCH_SPACE = {"bm_threads": [256, 512, 1024]}
 
 
 def side_bucket(n: int) -> int:
@@ -133,12 +134,15 @@ def main():
     small = W.c5_rank(count=256)
     jobs.append((Job(ctx, "rank", 65521, [ctx.to_device(m) for m in small], "256 x 2048^2 u32, p = 65521"),
                  RANK_SPACE))
+    seqs, _ = W.c4()
+    jobs.append((Job(ctx, "bm", M61, [ctx.to_device(s, square=False) for s in seqs],
+                     "1024 BM sequences, n in [1000, 20000]"), BM_BATCH_SPACE))
     if not args.quick:
         jobs.append((Job(ctx, "bm", M61, [ctx.to_device(W.c5_bm(), square=False)], "200000-step BM, MIXMAX-256"),
                      BM_SPACE))
 
     lines = ["# Launch parameters chosen by modlin-rs_amd/autotune.py on one MI355X.",
-             "# kind mode side count far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs"]
+             "# kind mode side count far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs bm_threads"]
     for job, space in jobs:
         params, t0, t1 = tune(job, space, args.reps, log)
         lines.append(f"{entry_line(job, params)}   # {job.label}: {t0 * 1e3:.2f} -> {t1 * 1e3:.2f} ms")

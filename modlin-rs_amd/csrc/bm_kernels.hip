@@ -1,15 +1,23 @@
 // Berlekamp-Massey over F_p on gfx950 (reference: src/fp.rs:257-308).
 //
-// k_bm_batch: one workgroup per sequence.  Each step i
-//   * the discrepancy d = s[i] + sum_{j=1..l} c[j] s[i-j] (fp.rs:274-278) is a
-//     block-wide dot product: every lane accumulates its strided terms lazily in
-//     128 bits (one Barrett reduction per safe_batch products), then the partials
-//     are added mod p across the wavefront with DPP/shuffles and across waves in
-//     LDS.  Addition mod p is associative, so d equals the reference's ordered sum.
+// k_bm_batch: one workgroup per sequence.  Thread tid owns the indices
+// j = tid (mod NT) of c and t in both phases of a step, so it only ever reads c
+// values it wrote itself and a step needs one workgroup barrier (the one inside the
+// discrepancy reduction).  Each step i
+//   * the discrepancy d = sum_{j=0..l} c[j] s[i-j] (fp.rs:274-278; c[0] is 1 in the
+//     reference and a nonzero scale here, see below) is a block-wide dot product:
+//     every lane accumulates its terms lazily in 128 bits, four loads in flight at
+//     a time (one reduction per safe_batch products), the partials are added mod p
+//     across the wavefront with shuffles and across waves in LDS.  Addition mod p is
+//     associative, so the zero test sees the reference's sum;
 //   * d = 0 only advances m (fp.rs:279-282);
-//   * otherwise c[m..m+jmax) -= (d / bb) b[0..jmax) is a coalesced streaming axpy
-//     (fp.rs:292-294, 301-303); on a length change (2l <= i, fp.rs:287-299) the old
-//     c[0..l] is first copied to t and b/t swap roles by pointer.
+//   * otherwise c <- bb c - d x^m b over j in [0, max(l+1, m+jmax)) (fp.rs:292-294,
+//     301-303 without the inverse of bb: the inversionless update scales c and every
+//     later discrepancy by nonzero factors, so the zero tests, the length changes
+//     and l are the reference's); on a length change (2l <= i, fp.rs:287-299) the
+//     old c[0..l] is first copied to t, and b/t swap roles by pointer.  b is read
+//     shifted by m (other threads' indices): the barrier of the next step orders
+//     those reads after the copy, and the copy of a later length change after them.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
 
 #include <algorithm>
@@ -22,6 +30,7 @@ namespace mdl {
 struct BmSeq {
     const uint64_t *s;
     uint64_t *c, *b, *t;
+    uint64_t *u;  // fourth state buffer (blocked cooperative kernel only)
     uint32_t n;
     uint32_t lc;
     unsigned long long bytes;  // algorithmic bytes: sum_i 8 + 16 l_i + [d_i != 0] 24 jmax_i + [change] 16 (old_l + 1)
@@ -50,70 +59,110 @@ __device__ __forceinline__ uint64_t block_sum_mod(uint64_t v, uint64_t p, uint64
 
 template <int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
+    constexpr int NW = NT / 64;
+    constexpr uint32_t U = 4;  // loads in flight per lane
     BmSeq &Q = seqs[blockIdx.x];
     const uint32_t n = Q.n;
-    const uint64_t *s = Q.s;
-    uint64_t *c = Q.c, *b = Q.b, *t = Q.t;
+    const gptr<const uint64_t> s = to_global(Q.s);
+    gptr<uint64_t> c = to_global(Q.c), b = to_global(Q.b), t = to_global(Q.t);
     const uint64_t p = f.p, sb = f.safe_batch;
-    const uint32_t tid = threadIdx.x;
-    __shared__ uint64_t red[NT / 64];
-    __shared__ uint64_t sh_inv;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    __shared__ uint64_t red[2][NW];
 
-    for (uint32_t j = tid; j < n; j += NT) {
-        c[j] = 0;
-        b[j] = 0;
-        t[j] = 0;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        c[0] = 1 % p;
-        b[0] = 1 % p;
-    }
-    __syncthreads();
+    // c = 1, b = 1 (fp.rs:267-270); entries of b and t at or above their length
+    // are never read, so only c needs clearing
+    for (uint32_t j = tid; j < n; j += NT) c[j] = j == 0 ? 1 % p : 0;
+    if (tid == 0) b[0] = 1 % p;
     uint32_t l = 0, bl = 1, m = 1;
-    uint64_t bb_inv = 1 % p;
+    uint64_t bb = 1 % p;
     unsigned long long bytes = 0;  // kept in a register, stored once at the end
     for (uint32_t i = 0; i < n; i++) {
         uint64_t ah = 0, al = 0, part = 0, cnt = 0;
-        for (uint32_t j = 1 + tid; j <= l; j += NT) {
-            uint64_t h, lo;
-            mul_wide(c[j], s[i - j], h, lo);
-            add128(ah, al, h, lo);
-            if (++cnt == sb) {
+        uint32_t j = tid;
+        for (; j + (U - 1) * NT <= l; j += U * NT) {
+            uint64_t cv[U], sv[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                cv[u] = c[j + u * NT];
+                sv[u] = s[i - j - u * NT];
+            }
+            if (cnt + U > sb) {
                 part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
                 ah = al = 0;
                 cnt = 0;
             }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                uint64_t h, lo;
+                mul_wide(cv[u], sv[u], h, lo);
+                add128(ah, al, h, lo);
+            }
+            cnt += U;
+        }
+        for (; j <= l; j += NT) {
+            uint64_t h, lo;
+            mul_wide(c[j], s[i - j], h, lo);
+            if (cnt == sb) {
+                part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
+                ah = al = 0;
+                cnt = 0;
+            }
+            add128(ah, al, h, lo);
+            cnt++;
         }
         if (cnt) part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
-        uint64_t d = add_mod(block_sum_mod<NT>(part, p, red), s[i], p);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part = add_mod(part, __shfl_xor(part, off, 64), p);
+        // red[i & 1] is rewritten two steps later, after every thread has passed
+        // the next step's barrier and so has read it
+        if (lane == 0) red[i & 1][wid] = part;
+        __syncthreads();
+        uint64_t d = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) d = add_mod(d, red[i & 1][w], p);
         bytes += 8ull + 16ull * l;
         if (d == 0) {
             m++;
             continue;
         }
-        const uint64_t coef = mul_mod_t<MODE>(f, d, bb_inv);
         const uint32_t jmax = min(bl, n - m);
-        if (2 * l <= i) {
-            const uint32_t old_l = l;
-            bytes += 24ull * jmax + 16ull * (old_l + 1);
-            for (uint32_t j = tid; j <= old_l; j += NT) t[j] = c[j];
-            if (tid == 0) sh_inv = inv_mod_bin(d, p);
-            __syncthreads();
-            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod_t<MODE>(f, coef, b[j]), p);
-            uint64_t *tmp = b;
+        const bool change = 2 * l <= i;
+        const uint32_t old_l = l;
+        bytes += 24ull * jmax + (change ? 16ull * (old_l + 1) : 0);
+        const uint32_t top = max(l + 1, m + jmax), a1 = m + jmax;
+        j = tid;
+        for (; j + (U - 1) * NT < top; j += U * NT) {
+            uint64_t cv[U], yv[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t jj = j + u * NT;
+                cv[u] = c[jj];
+                yv[u] = (jj >= m && jj < a1) ? b[jj - m] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t jj = j + u * NT;
+                if (change && jj <= old_l) t[jj] = cv[u];
+                c[jj] = ff_step_t<MODE>(f, bb, cv[u], d, yv[u]);
+            }
+        }
+        for (; j < top; j += NT) {
+            const uint64_t cv = c[j];
+            const uint64_t y = (j >= m && j < a1) ? b[j - m] : 0;
+            if (change && j <= old_l) t[j] = cv;
+            c[j] = ff_step_t<MODE>(f, bb, cv, d, y);
+        }
+        if (change) {
+            const gptr<uint64_t> tmp = b;
             b = t;
             t = tmp;
             bl = old_l + 1;
-            bb_inv = sh_inv;
+            bb = d;
             l = i + 1 - old_l;
             m = 1;
         } else {
-            bytes += 24ull * jmax;
-            for (uint32_t j = tid; j < jmax; j += NT) c[j + m] = sub_mod(c[j + m], mul_mod_t<MODE>(f, coef, b[j]), p);
             m++;
         }
-        __syncthreads();
     }
     if (tid == 0) {
         Q.lc = l;
@@ -156,6 +205,12 @@ __device__ __forceinline__ uint64_t ld_dev(gptr<const uint64_t> a) {
 }
 __device__ __forceinline__ void st_dev(gptr<uint64_t> a, uint64_t v) {
     __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lane_bcast64(uint64_t v, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ void coop_barrier(BmCoop *cs, unsigned int target) {
@@ -341,6 +396,318 @@ __global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldPa
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_bm_block: one long sequence on the whole GPU, BM_BK steps per exchange.
+//
+// With C, B the connection polynomials and m, l, bl, bb the scalar state at the
+// start of a block of steps i0 .. i0+K-1, every c and b of the block is a
+// combination  c_t = U_t C + V_t x^m B  with polynomials U_t, V_t of degree < K,
+// and the discrepancy of step i0+t is  d_t = [x^(i0+t)] (c_t S),  S = sum s_j x^j
+// (the reference's d = sum_{j<=l} c[j] s[i-j], fp.rs:274-278, with c[0] scaled;
+// c has no terms above l).  So a block needs only the 2K numbers
+//     Ec(r) = [x^(i0+r)] (C S),   Eb(r) = [x^(i0+r)] (x^m B S),   r < K,
+// sums over the coefficients of C and B that every workgroup forms for the
+// indices it owns.  One exchange adds them up (reduce-scatter then all-gather
+// through tagged words, as in k_bm_coop); then one wavefront replays the K steps
+// exactly as the reference does (fp.rs:271-305, inversionless): lane r holds the
+// discrepancy of step i0+r of the current c and of the current x^m b,
+//     d = e(t);  e <- bb e - d eb;  eb <- x eb, or x e on a length change,
+// and lane a holds coefficient a of U, V (and of the b combinations).  Last, each
+// workgroup forms the new C = U C + V x^m B (and B, if it changed) for its own
+// indices, from windows of the old C and B.  Addition mod p is associative and
+// every product is exact, so d, the length changes and l are the reference's.
+//
+// Ownership: indices of c and b in chunks of BM_BW, chunk q G + g to workgroup g,
+// kept in LDS (sCo, sBo) and mirrored to HBM for the other workgroups' windows.
+// The HBM copies are double-buffered: a block reads one and writes the other, and
+// a workgroup only writes after the exchange, i.e. after every workgroup has
+// finished the previous block's reads.  Cross-workgroup data uses device-scope
+// loads and stores; each wave waits for its stores before its workgroup
+// publishes.  Needs p < 2^61 (three tag bits) and at most BM_BQ chunks per
+// workgroup; k_bm_coop covers the rest.
+// ---------------------------------------------------------------------------
+constexpr uint32_t BM_BK = 64;  // steps per block (one wavefront replays them)
+constexpr uint32_t BM_BW = 64;  // chunk width
+constexpr uint32_t BM_BQ = 16;  // chunks per workgroup at most
+constexpr uint32_t BM_BLOCK_THREADS = 512;
+constexpr uint32_t BM_BWIN = BM_BW + BM_BK - 1;
+constexpr uint32_t BM_BLOCK_SPIN_LIMIT = 1u << 22;  // polls of one wait (seconds)
+
+struct BmBlockX {
+    unsigned int timeout;  // set if a wait ran out
+    unsigned int pad[63];
+    uint64_t part[2][2 * BM_BK][1024];  // per block parity, value, workgroup
+    uint64_t tot[2][2 * BM_BK];         // per block parity, value
+};
+
+__device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint32_t lane) {
+    const uint64_t y = __shfl_up(v, 1, 64);
+    return lane == 0 ? 0 : y;
+}
+
+// Polls a tagged word until it carries `tag`; false (and the error flag set) if
+// the wait ran out.
+__device__ __forceinline__ bool poll_tag(gptr<const uint64_t> a, uint64_t tag, uint64_t &val, BmBlockX *x) {
+    uint64_t v = ld_dev(a);
+    uint32_t spins = 0;
+    while ((v & ~P61) != tag) {
+        if (++spins == BM_BLOCK_SPIN_LIMIT) {
+            __hip_atomic_store(&x->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_dev(a);
+    }
+    val = v & P61;
+    return true;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, FieldParams f, BmBlockX *x) {
+    constexpr uint32_t NT = BM_BLOCK_THREADS, K = BM_BK, W = BM_BW, QM = BM_BQ, WIN = BM_BWIN, V2 = 2 * K;
+    constexpr uint32_t NW = NT / 64;
+    BmSeq &Q = *seq;
+    const uint32_t n = Q.n, G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const gptr<const uint64_t> s = to_global(Q.s);
+    const gptr<uint64_t> Cg[2] = {to_global(Q.c), to_global(Q.t)}, Bg[2] = {to_global(Q.b), to_global(Q.u)};
+    const uint64_t p = f.p, sb = f.safe_batch;
+    // chunks q = 0 .. nq-1 of this workgroup start at (q G + g) W
+    const uint32_t nchunks = (n + W - 1) / W;
+    const uint32_t nq = nchunks > g ? (nchunks - g + G - 1) / G : 0;  // <= QM (host)
+    __shared__ uint64_t sCo[QM * W], sBo[QM * W];
+    __shared__ uint64_t stage[QM * 2 * WIN];
+    __shared__ uint64_t sPoly[4][K];
+    __shared__ uint64_t sRed[NT / V2][V2];
+    __shared__ uint64_t sE[V2];
+    __shared__ uint64_t sWave[NW];
+    __shared__ uint64_t sState[6];
+    __shared__ uint32_t sAbort;
+
+    for (uint32_t k = tid; k < QM * W; k += NT) {
+        sCo[k] = 0;
+        sBo[k] = 0;
+    }
+    if (tid == 0) sAbort = 0;
+    __syncthreads();
+    if (g == 0 && tid == 0) {  // c = 1, b = 1 (fp.rs:267-270)
+        sCo[0] = 1 % p;
+        sBo[0] = 1 % p;
+        st_dev(Cg[0], 1 % p);
+        st_dev(Bg[0], 1 % p);
+    }
+    uint32_t l = 0, bl = 1, m = 1, cc = 0, bc = 0;
+    uint64_t bb = 1 % p;
+    unsigned long long bytes = 0;
+    for (uint32_t i0 = 0, blk = 0; i0 < n; i0 += K, blk++) {
+        const uint32_t Kp = min(K, n - i0);
+        const uint32_t par = blk & 1;
+        const uint64_t tag = (uint64_t)(((blk >> 1) + 1) & 7) << 61;
+        // ---- 1. this workgroup's parts of Ec(r) and Eb(r)
+        // chunks holding a coefficient of C (index <= l) or of B (index < bl)
+        const uint32_t top_e = max(l, bl - 1);
+        uint32_t nqa = 0;
+        while (nqa < nq && (nqa * G + g) * W <= top_e) nqa++;
+        // s windows: chunk q at c0 needs s[i0 + r - j], j in [c0, c0+W), r < K, for
+        // Ec and the same shifted by m for Eb: u = r - (j - c0) + W - 1 in [0, WIN)
+        for (uint32_t k = tid; k < nqa * 2 * WIN; k += NT) {
+            const uint32_t q = k / (2 * WIN), rem = k - q * 2 * WIN, which = rem / WIN, u = rem - which * WIN;
+            const uint32_t c0 = (q * G + g) * W;
+            const int64_t xi = (int64_t)i0 - (int64_t)c0 - (int64_t)(W - 1) + (int64_t)u - (which ? (int64_t)m : 0);
+            stage[k] = (xi >= 0 && xi < (int64_t)n) ? s[(uint32_t)xi] : 0;
+        }
+        __syncthreads();
+        {
+            const uint32_t v = tid & (V2 - 1), part = tid / V2, which = v / K, r = v & (K - 1);
+            constexpr uint32_t OP = W / (NT / V2);  // offsets per part
+            const uint64_t *own = which ? sBo : sCo;
+            uint64_t ah = 0, al = 0, acc = 0, cnt = 0;
+            for (uint32_t q = 0; q < nqa; q++) {
+                const uint64_t *win = stage + q * 2 * WIN + which * WIN + r + W - 1;
+#pragma unroll 4
+                for (uint32_t o = part * OP; o < part * OP + OP; o++) {
+                    if (cnt == sb) {
+                        acc = add_mod(acc, reduce_t<MODE>(f, ah, al), p);
+                        ah = al = 0;
+                        cnt = 0;
+                    }
+                    uint64_t h, lo;
+                    mul_wide(own[q * W + o], win[-(int32_t)o], h, lo);
+                    add128(ah, al, h, lo);
+                    cnt++;
+                }
+            }
+            if (cnt) acc = add_mod(acc, reduce_t<MODE>(f, ah, al), p);
+            sRed[part][v] = acc;
+        }
+        // every wave's HBM stores of the previous block are complete before publishing
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid < V2) {
+            uint64_t v = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < NT / V2; k++) v = add_mod(v, sRed[k][tid], p);
+            st_dev(to_global(&x->part[par][tid][g]), v | tag);
+        }
+        // ---- 2. reduce-scatter: workgroup g adds up values v = g, g + G, ...
+        for (uint32_t v = g; v < V2; v += G) {
+            uint64_t acc = 0;
+            for (uint32_t w = tid; w < G; w += NT) {
+                uint64_t y = 0;
+                if (!poll_tag(to_global(&x->part[par][v][w]), tag, y, x)) sAbort = 1;
+                acc = add_mod(acc, y, p);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc = add_mod(acc, __shfl_xor(acc, off, 64), p);
+            if (lane == 0) sWave[wid] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t t2 = 0;
+                for (uint32_t k = 0; k < NW; k++) t2 = add_mod(t2, sWave[k], p);
+                st_dev(to_global(&x->tot[par][v]), t2 | tag);
+            }
+            __syncthreads();
+        }
+        // ---- all-gather
+        if (tid < V2) {
+            uint64_t y = 0;
+            if (!poll_tag(to_global(&x->tot[par][tid]), tag, y, x)) sAbort = 1;
+            sE[tid] = y;
+        }
+        __syncthreads();
+        if (sAbort || __hip_atomic_load(&x->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        // ---- 3. the K steps (wavefront 0; every workgroup replays the same steps)
+        const uint32_t l0 = l, bl0 = bl, m0 = m;
+        if (wid == 0) {
+            uint64_t e = sE[lane], eb = sE[K + lane];
+            uint64_t U = lane == 0 ? 1 % p : 0, V = 0, Up = 0, Vp = lane == 0 ? 1 % p : 0, Ub = 0, Vb = 0;
+            uint32_t bch = 0;
+            for (uint32_t t = 0; t < Kp; t++) {
+                const uint32_t i = i0 + t;
+                const uint64_t d = lane_bcast64(e, t);
+                bytes += 8ull + 16ull * l;
+                if (d == 0) {  // fp.rs:279-282
+                    m++;
+                    eb = shift_up1(eb, lane);
+                    Up = shift_up1(Up, lane);
+                    Vp = shift_up1(Vp, lane);
+                    continue;
+                }
+                const uint32_t jmax = min(bl, n - m);
+                const bool change = 2 * l <= i;  // fp.rs:287-299
+                bytes += 24ull * jmax + (change ? 16ull * (l + 1) : 0);
+                const uint64_t ne = ff_step_t<MODE>(f, bb, e, d, eb);
+                const uint64_t nU = ff_step_t<MODE>(f, bb, U, d, Up);
+                const uint64_t nV = ff_step_t<MODE>(f, bb, V, d, Vp);
+                if (change) {
+                    eb = shift_up1(e, lane);
+                    Up = shift_up1(U, lane);
+                    Vp = shift_up1(V, lane);
+                    Ub = U;
+                    Vb = V;
+                    bch = 1;
+                    const uint32_t old_l = l;
+                    bl = old_l + 1;
+                    bb = d;
+                    l = i + 1 - old_l;
+                    m = 1;
+                } else {
+                    eb = shift_up1(eb, lane);
+                    Up = shift_up1(Up, lane);
+                    Vp = shift_up1(Vp, lane);
+                    m++;
+                }
+                e = ne;
+                U = nU;
+                V = nV;
+            }
+            sPoly[0][lane] = U;
+            sPoly[1][lane] = V;
+            sPoly[2][lane] = Ub;
+            sPoly[3][lane] = Vb;
+            if (lane == 0) {
+                sState[0] = l;
+                sState[1] = bl;
+                sState[2] = m;
+                sState[3] = bb;
+                sState[4] = bch;
+                sState[5] = bytes;
+            }
+        }
+        __syncthreads();
+        l = (uint32_t)sState[0];
+        bl = (uint32_t)sState[1];
+        m = (uint32_t)sState[2];
+        bb = sState[3];
+        const bool bch = sState[4] != 0;
+        // ---- 4. new C (and B) on this workgroup's chunks
+        const uint32_t top_a = bch ? max(l, bl - 1) : l;
+        uint32_t nqb = 0;
+        while (nqb < nq && (nqb * G + g) * W <= top_a) nqb++;
+        // windows: old C over [c0 - K + 1, c0 + W) and old x^m0 B over the same range
+        for (uint32_t k = tid; k < nqb * 2 * WIN; k += NT) {
+            const uint32_t q = k / (2 * WIN), rem = k - q * 2 * WIN, which = rem / WIN, u = rem - which * WIN;
+            const int64_t j = (int64_t)((q * G + g) * W) - (int64_t)(K - 1) + (int64_t)u;
+            uint64_t y = 0;
+            if (which == 0) {
+                if (u >= K - 1)
+                    y = sCo[q * W + (u - (K - 1))];  // own chunk (zero above l0)
+                else if (j >= 0 && j <= (int64_t)l0)
+                    y = ld_dev(Cg[cc] + (uint32_t)j);
+            } else {
+                const int64_t jb = j - (int64_t)m0;
+                if (jb >= 0 && jb < (int64_t)bl0) y = ld_dev(Bg[bc] + (uint32_t)jb);
+            }
+            stage[k] = y;
+        }
+        __syncthreads();
+        for (uint32_t q = wid; q < nqb; q += NW) {
+            const uint32_t j = (q * G + g) * W + lane;
+            const uint64_t *cw = stage + q * 2 * WIN + lane + K - 1;  // cw[-a] = C[j - a]
+            const uint64_t *bw = cw + WIN;                            // bw[-a] = B[j - m0 - a]
+            uint64_t ch = 0, cl = 0, ca = 0, bh = 0, blo = 0, ba = 0, cnt = 0;
+            for (uint32_t a = 0; a < K; a++) {
+                if (cnt + 2 > sb) {
+                    ca = add_mod(ca, reduce_t<MODE>(f, ch, cl), p);
+                    ch = cl = 0;
+                    if (bch) {
+                        ba = add_mod(ba, reduce_t<MODE>(f, bh, blo), p);
+                        bh = blo = 0;
+                    }
+                    cnt = 0;
+                }
+                const uint64_t xc = cw[-(int32_t)a], xb = bw[-(int32_t)a];
+                uint64_t h, lo;
+                mul_wide(sPoly[0][a], xc, h, lo);
+                add128(ch, cl, h, lo);
+                mul_wide(sPoly[1][a], xb, h, lo);
+                add128(ch, cl, h, lo);
+                if (bch) {
+                    mul_wide(sPoly[2][a], xc, h, lo);
+                    add128(bh, blo, h, lo);
+                    mul_wide(sPoly[3][a], xb, h, lo);
+                    add128(bh, blo, h, lo);
+                }
+                cnt += 2;
+            }
+            ca = add_mod(ca, reduce_t<MODE>(f, ch, cl), p);
+            sCo[q * W + lane] = ca;
+            if (j < n) st_dev(Cg[cc ^ 1] + j, ca);
+            if (bch) {
+                ba = add_mod(ba, reduce_t<MODE>(f, bh, blo), p);
+                sBo[q * W + lane] = ba;
+                if (j < n) st_dev(Bg[bc ^ 1] + j, ba);
+            }
+        }
+        cc ^= 1;
+        if (bch) bc ^= 1;
+        __syncthreads();
+    }
+    if (g == 0 && tid == 0) {
+        Q.lc = l;
+        Q.bytes = (unsigned long long)sState[5];
+    }
+}
+
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 __global__ void k_widen_seq(const uint32_t *src, uint64_t *dst, size_t count) {
@@ -371,18 +738,24 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         offS[i] = cur;
         cur = align_up(cur + n * 8, 256);
         offC[i] = cur;
-        cur = align_up(cur + 3 * n * 8, 256);
+        cur = align_up(cur + (count == 1 ? 4 : 3) * n * 8, 256);
     }
     const size_t off_stage = cur;
     cur = align_up(cur + maxn * 4, 256);
     const size_t off_coop = cur;
-    cur = align_up(cur + sizeof(BmCoop), 256);
+    cur = align_up(cur + (count == 1 ? std::max(sizeof(BmCoop), sizeof(BmBlockX)) : sizeof(BmCoop)), 256);
     int rc = arena_reserve(ctx, cur);
     if (rc) return rc;
     char *base = (char *)ctx->arena;
     hipStream_t st = ctx->stream;
+    // longest sequences first: workgroups are dispatched in index order, so when
+    // a batch does not fit on the device at once the long ones start first
+    std::vector<uint64_t> ord(count);
+    for (uint64_t i = 0; i < count; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint64_t a, uint64_t b2) { return seqs[a].n > seqs[b2].n; });
     std::vector<BmSeq> hs(count);
-    for (uint64_t i = 0; i < count; i++) {
+    for (uint64_t k = 0; k < count; k++) {
+        const uint64_t i = ord[k];
         const uint64_t n = seqs[i].n;
         uint64_t *sd = (uint64_t *)(base + offS[i]);
         uint64_t *cd = (uint64_t *)(base + offC[i]);
@@ -397,13 +770,14 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         }
         if (seqs[i].elem == MODLIN_ELEM_U32)
             hipLaunchKernelGGL(k_widen_seq, dim3(256), dim3(256), 0, st, (const uint32_t *)dst, sd, (size_t)n);
-        hs[i].s = sd;
-        hs[i].c = cd;
-        hs[i].b = cd + n;
-        hs[i].t = cd + 2 * n;
-        hs[i].n = (uint32_t)n;
-        hs[i].lc = 0;
-        hs[i].bytes = 0;
+        hs[k].s = sd;
+        hs[k].c = cd;
+        hs[k].b = cd + n;
+        hs[k].t = cd + 2 * n;
+        hs[k].u = count == 1 ? cd + 3 * n : nullptr;
+        hs[k].n = (uint32_t)n;
+        hs[k].lc = 0;
+        hs[k].bytes = 0;
     }
     BmSeq *ds = (BmSeq *)base;
     rc = copy_h2d(ctx, ds, hs.data(), count * sizeof(BmSeq));
@@ -413,10 +787,48 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     // per CU, at most 1024 partials).
     const uint32_t coop_min = (opts && opts->coop_min) ? opts->coop_min : BM_COOP_MIN;
     if (count == 1 && maxn >= coop_min) {
+        const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
+        const uint32_t G0 = tu.bm_coop_wgs ? tu.bm_coop_wgs : (uint32_t)std::min(ctx->cu_count, 1024);
+        // blocked kernel: p < 2^61 (tagged words) and at most BM_BQ chunks per workgroup
+        const uint64_t nchunks = (maxn + BM_BW - 1) / BM_BW;
+        const uint32_t Gb = (uint32_t)std::max<uint64_t>(G0, (nchunks + BM_BQ - 1) / BM_BQ);
+        if (p <= P61 && Gb <= 1024) {
+            const void *kb = f.mode == MODE_M61   ? (const void *)k_bm_block<MODE_M61>
+                             : f.mode == MODE_M31 ? (const void *)k_bm_block<MODE_M31>
+                                                  : (const void *)k_bm_block<MODE_GENERIC>;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb, BM_BLOCK_THREADS, 0) != hipSuccess) per_cu = 0;
+            if ((uint64_t)Gb <= (uint64_t)per_cu * (uint64_t)std::max(ctx->cu_count, 0)) {
+                BmBlockX *bx = (BmBlockX *)(base + off_coop);
+                MDL_HIP(hipMemsetAsync(bx, 0, sizeof(BmBlockX), st));
+                void *bargs[] = {(void *)&ds, (void *)&f, (void *)&bx};
+                if (hipLaunchCooperativeKernel(kb, dim3(Gb), dim3(BM_BLOCK_THREADS), bargs, 0, st) == hipSuccess) {
+                    if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+                    uint32_t timeout = 0;
+                    rc = copy_d2h(ctx, &timeout, &bx->timeout, sizeof(timeout));
+                    if (rc) return rc;
+                    if (timeout) {
+                        ctx->err = "blocked cooperative Berlekamp-Massey: an exchange wait ran out";
+                        return MODLIN_EHIP;
+                    }
+                    rc = copy_d2h(ctx, hs.data(), ds, sizeof(BmSeq));
+                    if (rc) return rc;
+                    lcs[0] = hs[0].lc;
+                    ctx->stats.bm_bytes = (double)hs[0].bytes;
+                    ctx->stats.bm_coop = (double)Gb;
+                    if (profile) {
+                        float ms = 0;
+                        MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
+                        ctx->stats.bm_ms = ms;
+                    }
+                    return MODLIN_OK;
+                }
+                (void)hipGetLastError();  // refused: the kernels below give the same result
+            }
+        }
         BmCoop *cs = (BmCoop *)(base + off_coop);
         MDL_HIP(hipMemsetAsync(cs, 0, sizeof(BmCoop), st));
-        const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
-        const uint32_t G = tu.bm_coop_wgs ? tu.bm_coop_wgs : (uint32_t)std::min(ctx->cu_count, 1024);
+        const uint32_t G = G0;
         void *args[] = {(void *)&ds, (void *)&f, (void *)&cs};
         hipError_t e;
         if (f.mode == MODE_M61)
@@ -454,24 +866,36 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         return MODLIN_OK;
     }
 single:
-    const bool wide = count == 1 && maxn >= 4096;
+    // threads per sequence: the tuning table's, else as many as keep the whole
+    // batch resident (about 32 waves per CU), 256 at least
+    const modlin_tuning_t tb = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
+    const uint64_t cus = (uint64_t)std::max(ctx->cu_count, 1);
+    const uint32_t nt = tb.bm_threads ? tb.bm_threads : count <= 2 * cus ? 1024 : count <= 4 * cus ? 512 : 256;
 #define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
+#define MDL_BM_NT(MODE)              \
+    if (nt == 1024)                   \
+        MDL_BM(1024, MODE);           \
+    else if (nt == 512)               \
+        MDL_BM(512, MODE);            \
+    else                              \
+        MDL_BM(256, MODE);
     if (f.mode == MODE_M61) {
-        if (wide) MDL_BM(1024, MODE_M61); else MDL_BM(256, MODE_M61);
+        MDL_BM_NT(MODE_M61)
     } else if (f.mode == MODE_M31) {
-        if (wide) MDL_BM(1024, MODE_M31); else MDL_BM(256, MODE_M31);
+        MDL_BM_NT(MODE_M31)
     } else {
-        if (wide) MDL_BM(1024, MODE_GENERIC); else MDL_BM(256, MODE_GENERIC);
+        MDL_BM_NT(MODE_GENERIC)
     }
+#undef MDL_BM_NT
 #undef MDL_BM
     if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
     MDL_HIP(hipGetLastError());
     rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));
     if (rc) return rc;
     double bytes = 0;
-    for (uint64_t i = 0; i < count; i++) {
-        lcs[i] = hs[i].lc;
-        bytes += (double)hs[i].bytes;
+    for (uint64_t k = 0; k < count; k++) {
+        lcs[ord[k]] = hs[k].lc;
+        bytes += (double)hs[k].bytes;
     }
     ctx->stats.bm_bytes = bytes;
     if (profile) {

@@ -18,6 +18,8 @@ struct TuneEntry {
     modlin_tuning_t t;
 };
 
+// 0 (default) or a thread count the batched BM kernel is built for.
+inline bool bm_threads_ok(uint32_t v) { return v == 0 || v == 256 || v == 512 || v == 1024; }
 // Reads a table; false (with err set) if the file is unreadable or a line is malformed.
 bool parse_tuning(const std::string &path, std::vector<TuneEntry> &out, std::string &err);
 // tuning.txt in the directory of this library.

@@ -11,12 +11,13 @@ import pytest
 from conftest import modlin
 
 M61 = (1 << 61) - 1
-TABLE = """# kind mode side count far near gather every panel bm
-rank m61 4096 many 4 2 1024 8 64 0
-rank m61 8192 1 8 1 512 16 64 0
-rank m61 16384 1 2 2 2048 8 48 0
-rank g3 2048 many 6 2 1024 8 64 0   # 65521
-bm m61 262144 1 0 0 0 0 0 128
+TABLE = """# kind mode side count far near gather every panel bm_coop bm_threads
+rank m61 4096 many 4 2 1024 8 64 0 0
+rank m61 8192 1 8 1 512 16 64 0 0
+rank m61 16384 1 2 2 2048 8 48 0 0
+rank g3 2048 many 6 2 1024 8 64 0 0   # 65521
+bm m61 262144 1 0 0 0 0 0 128 0
+bm m61 16384 many 0 0 0 0 0 0 512
 """
 
 
@@ -51,13 +52,17 @@ def test_bm_entries(table):
     t = modlin.tuning_select(table, M61, 200000, 1, bm=True)
     assert t["bm_coop_wgs"] == 128
     assert modlin.tuning_select(table, M61, 200000, 1)["bm_coop_wgs"] == 0
+    t = modlin.tuning_select(table, M61, 20000, 1024, bm=True)
+    assert (t["bm_threads"], t["bm_coop_wgs"]) == (512, 0)
 
 
-@pytest.mark.parametrize("bad", ["rank m61 4096 many 4 2 1024 8 64",          # missing field
-                                 "rank m61 3000 1 4 2 1024 8 64 0",            # side not a power of two
-                                 "rank m61 4096 1 4 2 1024 7 64 0",            # odd check_every
-                                 "rank m61 4096 some 4 2 1024 8 64 0",         # count
-                                 "tile m61 4096 1 4 2 1024 8 64 0"])           # kind
+@pytest.mark.parametrize("bad", ["rank m61 4096 many 4 2 1024 8 64 0",        # missing field
+                                 "rank m61 3000 1 4 2 1024 8 64 0 0",          # side not a power of two
+                                 "rank m61 4096 1 4 2 1024 7 64 0 0",          # odd check_every
+                                 "rank m61 4096 some 4 2 1024 8 64 0 0",       # count
+                                 "bm m61 4096 many 0 0 0 0 0 0 384",           # bm_threads
+                                 "rank m61 4096 1 4 2 1024 8 64 0 0 1",        # extra field
+                                 "tile m61 4096 1 4 2 1024 8 64 0 0"])         # kind
 def test_malformed_tables_rejected(tmp_path, bad):
     p = tmp_path / "bad.txt"
     p.write_text(bad + "\n")
