@@ -440,9 +440,18 @@ struct BmBlockX {
     uint64_t tot[2][2 * BM_BK];         // per block parity, value
 };
 
-__device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint32_t lane) {
-    const uint64_t y = __shfl_up(v, 1, 64);
-    return lane == 0 ? 0 : y;
+// A value every lane holds alike (read from LDS), as a scalar: branches on it stay
+// uniform instead of running both sides under an exec mask.
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
+
+// Lane r gets lane r-1's value, lane 0 gets 0 (DPP whole-wave shift right).
+__device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint32_t) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x138, 0xf, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Polls a tagged word until it carries `tag`; false (and the error flag set) if
@@ -462,15 +471,58 @@ __device__ __forceinline__ bool poll_tag(gptr<const uint64_t> a, uint64_t tag, u
     return true;
 }
 
+// Records of the steps of a block, written by wavefront 0 of k_bm_block: the step's
+// discrepancy d (< 2^61) with its kind in bits 62-63 (0: d = 0, 1: update,
+// 2: update with a length change).  *prog, written with release order every few
+// steps, says how many records are there; followers wait for record t with
+// acquire loads.
+__device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *prog, uint32_t t, uint32_t &avail) {
+    for (uint32_t spins = 0; t >= avail && spins < BM_BLOCK_SPIN_LIMIT; spins++) {
+        avail = uni32(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (t >= avail) __builtin_amdgcn_s_sleep(0);
+    }
+    return uni64(rec[t]);
+}
+
+// Replays the steps of one block for one polynomial pair (P, P'): P <- bb P - d P',
+// P' <- x P' (or x P on a length change, when Pb keeps the old P and bb becomes d).
+template <int MODE>
+__device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp, uint32_t lane, uint64_t bb,
+                                               uint64_t P, uint64_t Pp, uint32_t *prog, const uint64_t *rec,
+                                               uint64_t *outP, uint64_t *outPb) {
+    uint64_t Pb = 0;
+    uint32_t avail = 0;
+    for (uint32_t t = 0; t < Kp; t++) {
+        const uint64_t r = bm_record(rec, prog, t, avail);
+        const uint32_t fl = (uint32_t)(r >> 62);
+        if (fl == 0) {
+            Pp = shift_up1(Pp, lane);
+            continue;
+        }
+        const uint64_t d = r & P61;
+        const uint64_t nP = ff_step_t<MODE>(f, bb, P, d, Pp);
+        if (fl == 2) {
+            Pp = shift_up1(P, lane);
+            Pb = P;
+            bb = d;
+        } else {
+            Pp = shift_up1(Pp, lane);
+        }
+        P = nP;
+    }
+    outP[lane] = P;
+    outPb[lane] = Pb;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, FieldParams f, BmBlockX *x) {
     constexpr uint32_t NT = BM_BLOCK_THREADS, K = BM_BK, W = BM_BW, QM = BM_BQ, WIN = BM_BWIN, V2 = 2 * K;
     constexpr uint32_t NW = NT / 64;
     BmSeq &Q = *seq;
-    const uint32_t n = Q.n, G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t n = Q.n, G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = uni32(tid >> 6);
     const gptr<const uint64_t> s = to_global(Q.s);
     const gptr<uint64_t> Cg[2] = {to_global(Q.c), to_global(Q.t)}, Bg[2] = {to_global(Q.b), to_global(Q.u)};
-    const uint64_t p = f.p, sb = f.safe_batch;
+    const uint64_t p = f.p;
     // chunks q = 0 .. nq-1 of this workgroup start at (q G + g) W
     const uint32_t nchunks = (n + W - 1) / W;
     const uint32_t nq = nchunks > g ? (nchunks - g + G - 1) / G : 0;  // <= QM (host)
@@ -480,8 +532,10 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
     __shared__ uint64_t sRed[NT / V2][V2];
     __shared__ uint64_t sE[V2];
     __shared__ uint64_t sWave[NW];
-    __shared__ uint64_t sState[6];
-    __shared__ uint32_t sAbort;
+    __shared__ uint64_t sRec[K];
+    __shared__ uint64_t sJunk[2 * 64];
+    __shared__ uint64_t sState[7];
+    __shared__ uint32_t sAbort, sProg;
 
     for (uint32_t k = tid; k < QM * W; k += NT) {
         sCo[k] = 0;
@@ -518,26 +572,17 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
         __syncthreads();
         {
             const uint32_t v = tid & (V2 - 1), part = tid / V2, which = v / K, r = v & (K - 1);
-            constexpr uint32_t OP = W / (NT / V2);  // offsets per part
+            constexpr uint32_t OP = W / (NT / V2);  // offsets per part (even)
             const uint64_t *own = which ? sBo : sCo;
-            uint64_t ah = 0, al = 0, acc = 0, cnt = 0;
+            MacSum<MODE> acc;
             for (uint32_t q = 0; q < nqa; q++) {
                 const uint64_t *win = stage + q * 2 * WIN + which * WIN + r + W - 1;
+                const uint64_t *ow = own + q * W;
 #pragma unroll 4
-                for (uint32_t o = part * OP; o < part * OP + OP; o++) {
-                    if (cnt == sb) {
-                        acc = add_mod(acc, reduce_t<MODE>(f, ah, al), p);
-                        ah = al = 0;
-                        cnt = 0;
-                    }
-                    uint64_t h, lo;
-                    mul_wide(own[q * W + o], win[-(int32_t)o], h, lo);
-                    add128(ah, al, h, lo);
-                    cnt++;
-                }
+                for (uint32_t o = part * OP; o < part * OP + OP; o += 2)
+                    acc.add2(f, ow[o], win[-(int32_t)o], ow[o + 1], win[-(int32_t)o - 1]);
             }
-            if (cnt) acc = add_mod(acc, reduce_t<MODE>(f, ah, al), p);
-            sRed[part][v] = acc;
+            sRed[part][v] = acc.value(f);
         }
         // every wave's HBM stores of the previous block are complete before publishing
         __builtin_amdgcn_s_waitcnt(0);
@@ -573,72 +618,92 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
             if (!poll_tag(to_global(&x->tot[par][tid]), tag, y, x)) sAbort = 1;
             sE[tid] = y;
         }
+        if (tid == 0) sProg = 0;
         __syncthreads();
-        if (sAbort || __hip_atomic_load(&x->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        // ---- 3. the K steps (wavefront 0; every workgroup replays the same steps)
+        if (uni32(sAbort) != 0 || uni32(__hip_atomic_load(&x->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+            break;
+        // ---- 3. the K steps.  Wavefront 0 replays the discrepancies (lane r: step
+        // i0+r of the current c and of the current x^m b) with the length test and
+        // writes one record per step; wavefronts 1 and 2 follow it with the
+        // coefficients of U (C's part) and V (B's part), wavefront 3 with the rest of
+        // the reference's scalar state (m, bl) and the byte count.
         const uint32_t l0 = l, bl0 = bl, m0 = m;
         if (wid == 0) {
+            // for 2^61 - 1, e and eb are kept in [0, p + 7] and only the broadcast d
+            // is reduced (on the scalar unit)
             uint64_t e = sE[lane], eb = sE[K + lane];
-            uint64_t U = lane == 0 ? 1 % p : 0, V = 0, Up = 0, Vp = lane == 0 ? 1 % p : 0, Ub = 0, Vb = 0;
-            uint32_t bch = 0;
+            __asm__ volatile("" : "+v"(e), "+v"(eb));  // loaded before the loop: no LDS wait at its head
             for (uint32_t t = 0; t < Kp; t++) {
                 const uint32_t i = i0 + t;
-                const uint64_t d = lane_bcast64(e, t);
-                bytes += 8ull + 16ull * l;
+                uint64_t d = lane_bcast64(e, t);
+                if (MODE == MODE_M61 && d >= P61) d -= P61;
+                uint64_t fl = 0;
                 if (d == 0) {  // fp.rs:279-282
-                    m++;
                     eb = shift_up1(eb, lane);
-                    Up = shift_up1(Up, lane);
-                    Vp = shift_up1(Vp, lane);
+                } else {
+                    const uint64_t ne = MODE == MODE_M61 ? ff_step_m61_lazy(bb, e, d, eb) : ff_step_t<MODE>(f, bb, e, d, eb);
+                    if (2 * l <= i) {  // length change, fp.rs:287-299
+                        eb = shift_up1(e, lane);
+                        l = i + 1 - l;
+                        bb = d;
+                        fl = 2;
+                    } else {
+                        eb = shift_up1(eb, lane);
+                        fl = 1;
+                    }
+                    e = ne;
+                }
+                // lane 0 writes the record, the other lanes their own junk slots: no
+                // branch on the lane, so the loop's control stays scalar
+                *(lane == 0 ? &sRec[t] : &sJunk[lane]) = d | (fl << 62);
+                if ((t & 7) == 7 || t + 1 == Kp)
+                    __hip_atomic_store(lane == 0 ? &sProg : (uint32_t *)&sJunk[64 + lane], t + 1, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if (lane == 0) sState[3] = bb;
+        } else if (wid == 1) {
+            bm_replay_poly<MODE>(f, Kp, lane, bb, lane == 0 ? 1 % p : 0, 0, &sProg, sRec, sPoly[0], sPoly[2]);
+        } else if (wid == 2) {
+            bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3]);
+        } else if (wid == 3) {
+            uint32_t avail = 0, upd = 0, bch = 0;
+            for (uint32_t t = 0; t < Kp; t++) {
+                const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, avail) >> 62);
+                const uint32_t i = i0 + t;
+                bytes += 8ull + 16ull * l;
+                if (fl == 0) {
+                    m++;
                     continue;
                 }
                 const uint32_t jmax = min(bl, n - m);
-                const bool change = 2 * l <= i;  // fp.rs:287-299
-                bytes += 24ull * jmax + (change ? 16ull * (l + 1) : 0);
-                const uint64_t ne = ff_step_t<MODE>(f, bb, e, d, eb);
-                const uint64_t nU = ff_step_t<MODE>(f, bb, U, d, Up);
-                const uint64_t nV = ff_step_t<MODE>(f, bb, V, d, Vp);
-                if (change) {
-                    eb = shift_up1(e, lane);
-                    Up = shift_up1(U, lane);
-                    Vp = shift_up1(V, lane);
-                    Ub = U;
-                    Vb = V;
-                    bch = 1;
+                bytes += 24ull * jmax + (fl == 2 ? 16ull * (l + 1) : 0);
+                upd = 1;
+                if (fl == 2) {
                     const uint32_t old_l = l;
                     bl = old_l + 1;
-                    bb = d;
                     l = i + 1 - old_l;
                     m = 1;
+                    bch = 1;
                 } else {
-                    eb = shift_up1(eb, lane);
-                    Up = shift_up1(Up, lane);
-                    Vp = shift_up1(Vp, lane);
                     m++;
                 }
-                e = ne;
-                U = nU;
-                V = nV;
             }
-            sPoly[0][lane] = U;
-            sPoly[1][lane] = V;
-            sPoly[2][lane] = Ub;
-            sPoly[3][lane] = Vb;
             if (lane == 0) {
                 sState[0] = l;
                 sState[1] = bl;
                 sState[2] = m;
-                sState[3] = bb;
                 sState[4] = bch;
                 sState[5] = bytes;
+                sState[6] = upd;
             }
         }
         __syncthreads();
-        l = (uint32_t)sState[0];
-        bl = (uint32_t)sState[1];
-        m = (uint32_t)sState[2];
-        bb = sState[3];
-        const bool bch = sState[4] != 0;
+        l = uni32((uint32_t)sState[0]);
+        bl = uni32((uint32_t)sState[1]);
+        m = uni32((uint32_t)sState[2]);
+        bb = uni64(sState[3]);
+        const bool bch = uni32((uint32_t)sState[4]) != 0;
+        if (uni32((uint32_t)sState[6]) == 0) continue;  // no update in this block: C and B stand
         // ---- 4. new C (and B) on this workgroup's chunks
         const uint32_t top_a = bch ? max(l, bl - 1) : l;
         uint32_t nqb = 0;
@@ -660,42 +725,50 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
             stage[k] = y;
         }
         __syncthreads();
-        for (uint32_t q = wid; q < nqb; q += NW) {
-            const uint32_t j = (q * G + g) * W + lane;
+        // work items (chunk, tap range): with fewer chunks than wavefronts each
+        // chunk's K taps are split over wpc wavefronts and the parts added in LDS
+        const uint32_t wpc = nqb >= 5 ? 1 : nqb >= 3 ? 2 : nqb == 2 ? 4 : 8;
+        const uint32_t taps = K / wpc;
+        uint64_t *sAcc = stage + 8 * 2 * WIN;  // free when wpc > 1 (nqb <= 4): [item][2][64]
+        for (uint32_t it = wid; it < nqb * wpc; it += NW) {
+            const uint32_t q = it / wpc, a0 = (it - q * wpc) * taps;
             const uint64_t *cw = stage + q * 2 * WIN + lane + K - 1;  // cw[-a] = C[j - a]
             const uint64_t *bw = cw + WIN;                            // bw[-a] = B[j - m0 - a]
-            uint64_t ch = 0, cl = 0, ca = 0, bh = 0, blo = 0, ba = 0, cnt = 0;
-            for (uint32_t a = 0; a < K; a++) {
-                if (cnt + 2 > sb) {
-                    ca = add_mod(ca, reduce_t<MODE>(f, ch, cl), p);
-                    ch = cl = 0;
-                    if (bch) {
-                        ba = add_mod(ba, reduce_t<MODE>(f, bh, blo), p);
-                        bh = blo = 0;
-                    }
-                    cnt = 0;
-                }
+            MacSum<MODE> cs, bs;
+            for (uint32_t a = a0; a < a0 + taps; a++) {
                 const uint64_t xc = cw[-(int32_t)a], xb = bw[-(int32_t)a];
-                uint64_t h, lo;
-                mul_wide(sPoly[0][a], xc, h, lo);
-                add128(ch, cl, h, lo);
-                mul_wide(sPoly[1][a], xb, h, lo);
-                add128(ch, cl, h, lo);
-                if (bch) {
-                    mul_wide(sPoly[2][a], xc, h, lo);
-                    add128(bh, blo, h, lo);
-                    mul_wide(sPoly[3][a], xb, h, lo);
-                    add128(bh, blo, h, lo);
-                }
-                cnt += 2;
+                cs.add2(f, sPoly[0][a], xc, sPoly[1][a], xb);
+                if (bch) bs.add2(f, sPoly[2][a], xc, sPoly[3][a], xb);
             }
-            ca = add_mod(ca, reduce_t<MODE>(f, ch, cl), p);
-            sCo[q * W + lane] = ca;
-            if (j < n) st_dev(Cg[cc ^ 1] + j, ca);
-            if (bch) {
-                ba = add_mod(ba, reduce_t<MODE>(f, bh, blo), p);
-                sBo[q * W + lane] = ba;
-                if (j < n) st_dev(Bg[bc ^ 1] + j, ba);
+            const uint64_t cv = cs.value(f), bv = bch ? bs.value(f) : 0;
+            if (wpc == 1) {
+                const uint32_t j = (q * G + g) * W + lane;
+                sCo[q * W + lane] = cv;
+                if (j < n) st_dev(Cg[cc ^ 1] + j, cv);
+                if (bch) {
+                    sBo[q * W + lane] = bv;
+                    if (j < n) st_dev(Bg[bc ^ 1] + j, bv);
+                }
+            } else {
+                sAcc[it * 128 + lane] = cv;
+                sAcc[it * 128 + 64 + lane] = bv;
+            }
+        }
+        if (wpc > 1) {
+            __syncthreads();
+            for (uint32_t q = wid; q < nqb; q += NW) {
+                uint64_t cv = 0, bv = 0;
+                for (uint32_t k = 0; k < wpc; k++) {
+                    cv = add_mod(cv, sAcc[(q * wpc + k) * 128 + lane], p);
+                    bv = add_mod(bv, sAcc[(q * wpc + k) * 128 + 64 + lane], p);
+                }
+                const uint32_t j = (q * G + g) * W + lane;
+                sCo[q * W + lane] = cv;
+                if (j < n) st_dev(Cg[cc ^ 1] + j, cv);
+                if (bch) {
+                    sBo[q * W + lane] = bv;
+                    if (j < n) st_dev(Bg[bc ^ 1] + j, bv);
+                }
             }
         }
         cc ^= 1;

@@ -228,18 +228,75 @@ MDL_HD uint64_t reduce_t(const FieldParams &f, uint64_t th, uint64_t tl) {  // a
     if (MODE == MODE_M31) return fold_m31(th, tl);
     return reduce128(f, th, tl);
 }
+// Lazy sums of products mod p.  add2(a, b, c, d) adds a b + c d (all < p); any
+// number of calls stays exact, and value() is the canonical residue.
+//   MODE_M61: with 31-bit halves (a = a1 2^31 + a0),
+//     a b == a0 b0 + a1 (2 b1) + (a0 b1 + a1 b0) 2^31   (2^62 == 2 mod 2^61 - 1),
+//   so s collects a0 b0 + 2 a1 b1 (< 1.5 2^62 per product) and t the cross terms
+//   (< 2^62 per product): four 32 x 32 -> 64 products, no 128-bit carries.  Both
+//   are folded to < 2^61 + 8 after every pair, so a pair never overflows 64 bits.
+//   Other modes: 128-bit sums of full products, reduced every safe_batch products.
+template <int MODE>
+struct MacSum {
+    uint64_t s = 0, t = 0, r = 0;
+    uint64_t cnt = 0;
+    MDL_HD void add2(const FieldParams &f, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+        if constexpr (MODE == MODE_M61) {
+            const uint32_t a0 = (uint32_t)a & 0x7fffffffu, a1 = (uint32_t)(a >> 31);
+            const uint32_t b0 = (uint32_t)b & 0x7fffffffu, b1 = (uint32_t)(b >> 31);
+            const uint32_t c0 = (uint32_t)c & 0x7fffffffu, c1 = (uint32_t)(c >> 31);
+            const uint32_t d0 = (uint32_t)d & 0x7fffffffu, d1 = (uint32_t)(d >> 31);
+            s += (uint64_t)a0 * b0 + (uint64_t)a1 * (2u * b1) + (uint64_t)c0 * d0 + (uint64_t)c1 * (2u * d1);
+            t += (uint64_t)a0 * b1 + (uint64_t)a1 * b0 + (uint64_t)c0 * d1 + (uint64_t)c1 * d0;
+            s = (s & P61) + (s >> 61);
+            t = (t & P61) + (t >> 61);
+        } else {
+            if (cnt + 2 > f.safe_batch) {  // safe_batch >= 4 for p < 2^63
+                r = add_mod(r, reduce_t<MODE>(f, s, t), f.p);
+                s = t = 0;
+                cnt = 0;
+            }
+            uint64_t h, l;
+            mul_wide(a, b, h, l);
+            add128(s, t, h, l);
+            mul_wide(c, d, h, l);
+            add128(s, t, h, l);
+            cnt += 2;
+        }
+    }
+    MDL_HD uint64_t value(const FieldParams &f) const {
+        if constexpr (MODE == MODE_M61) {
+            // t 2^31 = (t >> 30) 2^61 + (t mod 2^30) 2^31 == (t >> 30) + (t mod 2^30) 2^31
+            return canon_m61(s + (t >> 30) + ((t & 0x3fffffffull) << 31));  // < 2^63
+        } else {
+            return add_mod(r, reduce_t<MODE>(f, s, t), f.p);
+        }
+    }
+};
+
 // b x + d y mod 2^61 - 1 for x, y, b, d < 2^61, b and d given split into 31-bit
 // halves (b0 = b mod 2^31, b1 = b >> 31).  Both products share the two sums
 //   s = x0 b0 + x1 2 b1 + y0 d0 + y1 2 d1 < 3 2^62,   t = x0 b1 + x1 b0 + y0 d1 + y1 d0 < 2^63
 // (2^62 == 2), so b x + d y == s + t 2^31 and a single fold reduces them.
-MDL_HD uint64_t lin2_m61(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
+// lin2_m61_lazy: the same, also for x, y < 2^61 + 8 (then x1, y1 <= 2^30, and s <
+// 1.5 2^63, t <= 2^63 still hold), result in [0, p + 7], congruent, not reduced.
+MDL_HD uint64_t lin2_m61_lazy(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
     const uint32_t x0 = (uint32_t)x & 0x7fffffffu, x1 = (uint32_t)(x >> 31);
     const uint32_t y0 = (uint32_t)y & 0x7fffffffu, y1 = (uint32_t)(y >> 31);
     const uint64_t s = (uint64_t)x0 * b0 + (uint64_t)x1 * (2u * b1) + (uint64_t)y0 * d0 + (uint64_t)y1 * (2u * d1);
     const uint64_t t = (uint64_t)x0 * b1 + (uint64_t)x1 * b0 + (uint64_t)y0 * d1 + (uint64_t)y1 * d0;
-    uint64_t r = s + (t >> 30) + ((t & 0x3fffffffull) << 31);  // < 2^64
-    r = (r & P61) + (r >> 61);                                   // <= p + 7
+    const uint64_t r = s + (t >> 30) + ((t & 0x3fffffffull) << 31);  // < 2^64
+    return (r & P61) + (r >> 61);                                      // <= p + 7
+}
+MDL_HD uint64_t lin2_m61(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
+    const uint64_t r = lin2_m61_lazy(x, y, b0, b1, d0, d1);
     return r >= P61 ? r - P61 : r;
+}
+// pi x - e y mod 2^61 - 1 for pi, e < p and x, y < 2^61 + 8, in [0, p + 7].
+MDL_HD uint64_t ff_step_m61_lazy(uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
+    const uint64_t ne = P61 - e;
+    return lin2_m61_lazy(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)ne & 0x7fffffffu,
+                         (uint32_t)(ne >> 31));
 }
 
 template <int MODE>

@@ -1,6 +1,8 @@
 // The GPU field arithmetic (modlin-rs_amd/csrc/fp_dev.hpp) on the CPU, against
 // exact 128-bit remainders: the fused Mersenne step lin2_m61 (b x + d y), the
-// fraction-free step ff_step_t in every reduction mode, and mul_mod_t.  Inputs are
+// fraction-free step ff_step_t in every reduction mode (and ff_step_m61_lazy on
+// operands up to p + 7), mul_mod_t, and the lazy
+// sums of products MacSum (long runs of add2, every mode).  Inputs are
 // random canonical residues plus the corners 0, 1, p - 1, 2^31 - 1, 2^31, 2^32.
 // Built and run by tests/test_limb.py; prints "bad=0" and exits 0 on success.
 #include <stdint.h>
@@ -38,6 +40,14 @@ static void run(uint64_t p, int n) {
             bad++;
         }
         if (MODE == MODE_M61) {
+            // lazily reduced operands x + k p, y + k' p <= p + 7 (k, k' in {0, 1} when
+            // x, y <= 7): result congruent and at most p + 7
+            const uint64_t xl = x <= 7 ? x + P61 : x, yl = y <= 7 ? y + P61 : y;
+            const uint64_t z = ff_step_m61_lazy(pi, xl, e, yl);
+            if (z > P61 + 7 || z % P61 != want) {
+                if (bad < 8) printf("ff_step_m61_lazy mismatch\n");
+                bad++;
+            }
             const uint64_t l = lin2_m61(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)e & 0x7fffffffu,
                                         (uint32_t)(e >> 31));
             if (l != (uint64_t)(((u128)pi * x + (u128)e * y) % p)) {
@@ -48,7 +58,35 @@ static void run(uint64_t p, int n) {
     }
 }
 
+// MacSum over runs of up to 300 pairs, every residue drawn from the corners and at
+// random (long runs of p - 1 stress the 2^61 - 1 folds and the safe_batch cut).
+template <int MODE>
+static void run_sum(uint64_t p, int n) {
+    const FieldParams f = make_params(p);
+    for (int it = 0; it < n; it++) {
+        MacSum<MODE> acc;
+        u128 want = 0;
+        const int len = it % 7 == 0 ? 300 : (int)(rng() % 40);
+        const bool worst = it % 5 == 0;
+        for (int k = 0; k < len; k++) {
+            const uint64_t a = worst ? p - 1 : pick(p, (int)(rng() % 12)), b = worst ? p - 1 : pick(p, (int)(rng() % 12));
+            const uint64_t c = pick(p, (int)(rng() % 12)), d = worst ? p - 1 : pick(p, (int)(rng() % 12));
+            acc.add2(f, a, b, c, d);
+            want = (want + (u128)a * b % p + (u128)c * d % p) % p;
+        }
+        if (acc.value(f) != (uint64_t)want) {
+            if (bad < 8) printf("MacSum mismatch p=%llu len=%d\n", (unsigned long long)p, len);
+            bad++;
+        }
+    }
+}
+
 int main() {
+    run_sum<MODE_M61>(P61, 20000);
+    run_sum<MODE_M31>(P31, 5000);
+    run_sum<MODE_GENERIC>(4611686018427387847ull, 5000);
+    run_sum<MODE_GENERIC>(9223372036854775783ull, 5000);
+    run_sum<MODE_GENERIC>(65521ull, 5000);
     run<MODE_M61>(P61, 2000000);
     run<MODE_M31>(P31, 500000);
     const uint64_t primes[] = {2ull, 3ull, 65521ull, 4294967291ull, 1099511627689ull, 4611686018427387847ull,
