@@ -155,7 +155,8 @@ def bm_long(ctx, steps):
     fx = fixture("c5_bm.json")
     dev[0].free()
     return {"n": int(seq.size), "steps_per_s": seq.size / dt, "ms": dt * 1e3, "us_per_step": dt * 1e6 / seq.size,
-            "kernel_ms": st["bm_ms"], "lc": lc[0], "lc_match_fixture": (fx["lc"] == lc[0]) if fx else None}
+            "kernel_ms": st["bm_ms"], "workgroups": st["bm_coop"], "lc": lc[0],
+            "lc_match_fixture": (fx["lc"] == lc[0]) if fx else None}
 
 
 def rank_batch_u32(ctx, steps):

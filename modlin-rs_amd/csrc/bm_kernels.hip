@@ -6,9 +6,9 @@
 // discrepancy reduction).  Each step i
 //   * the discrepancy d = sum_{j=0..l} c[j] s[i-j] (fp.rs:274-278; c[0] is 1 in the
 //     reference and a nonzero scale here, see below) is a block-wide dot product:
-//     every lane accumulates its terms lazily in 128 bits, four loads in flight at
-//     a time (one reduction per safe_batch products), the partials are added mod p
-//     across the wavefront with shuffles and across waves in LDS.  Addition mod p is
+//     every lane accumulates its terms lazily (MacSum), four loads in flight at a
+//     time, the partials are added mod p across the wavefront with shuffles and
+//     across waves in LDS.  Addition mod p is
 //     associative, so the zero test sees the reference's sum;
 //   * d = 0 only advances m (fp.rs:279-282);
 //   * otherwise c <- bb c - d x^m b over j in [0, max(l+1, m+jmax)) (fp.rs:292-294,
@@ -35,6 +35,13 @@ struct BmSeq {
     uint32_t lc;
     unsigned long long bytes;  // algorithmic bytes: sum_i 8 + 16 l_i + [d_i != 0] 24 jmax_i + [change] 16 (old_l + 1)
 };
+
+// A value every lane holds alike (read from LDS), as a scalar: branches on it stay
+// uniform instead of running both sides under an exec mask.
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
 
 template <int NT>
 __device__ __forceinline__ uint64_t block_sum_mod(uint64_t v, uint64_t p, uint64_t *red) {
@@ -65,8 +72,8 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     const uint32_t n = Q.n;
     const gptr<const uint64_t> s = to_global(Q.s);
     gptr<uint64_t> c = to_global(Q.c), b = to_global(Q.b), t = to_global(Q.t);
-    const uint64_t p = f.p, sb = f.safe_batch;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t p = f.p;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = uni32(tid >> 6);
     __shared__ uint64_t red[2][NW];
 
     // c = 1, b = 1 (fp.rs:267-270); entries of b and t at or above their length
@@ -77,7 +84,7 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     uint64_t bb = 1 % p;
     unsigned long long bytes = 0;  // kept in a register, stored once at the end
     for (uint32_t i = 0; i < n; i++) {
-        uint64_t ah = 0, al = 0, part = 0, cnt = 0;
+        MacSum<MODE> acc;
         uint32_t j = tid;
         for (; j + (U - 1) * NT <= l; j += U * NT) {
             uint64_t cv[U], sv[U];
@@ -86,31 +93,12 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
                 cv[u] = c[j + u * NT];
                 sv[u] = s[i - j - u * NT];
             }
-            if (cnt + U > sb) {
-                part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
-                ah = al = 0;
-                cnt = 0;
-            }
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                uint64_t h, lo;
-                mul_wide(cv[u], sv[u], h, lo);
-                add128(ah, al, h, lo);
-            }
-            cnt += U;
+            for (uint32_t u = 0; u < U; u += 2) acc.add2(f, cv[u], sv[u], cv[u + 1], sv[u + 1]);
         }
-        for (; j <= l; j += NT) {
-            uint64_t h, lo;
-            mul_wide(c[j], s[i - j], h, lo);
-            if (cnt == sb) {
-                part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
-                ah = al = 0;
-                cnt = 0;
-            }
-            add128(ah, al, h, lo);
-            cnt++;
-        }
-        if (cnt) part = add_mod(part, reduce_t<MODE>(f, ah, al), p);
+        for (; j + NT <= l; j += 2 * NT) acc.add2(f, c[j], s[i - j], c[j + NT], s[i - j - NT]);
+        if (j <= l) acc.add2(f, c[j], s[i - j], 0, 0);
+        uint64_t part = acc.value(f);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) part = add_mod(part, __shfl_xor(part, off, 64), p);
         // red[i & 1] is rewritten two steps later, after every thread has passed
@@ -439,13 +427,6 @@ struct BmBlockX {
     uint64_t part[2][2 * BM_BK][1024];  // per block parity, value, workgroup
     uint64_t tot[2][2 * BM_BK];         // per block parity, value
 };
-
-// A value every lane holds alike (read from LDS), as a scalar: branches on it stay
-// uniform instead of running both sides under an exec mask.
-__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-__device__ __forceinline__ uint64_t uni64(uint64_t v) {
-    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
-}
 
 // Lane r gets lane r-1's value, lane 0 gets 0 (DPP whole-wave shift right).
 __device__ __forceinline__ uint64_t shift_up1(uint64_t v, uint32_t) {
