@@ -141,7 +141,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = plan->gx_total;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
     const uint64_t p = f.p;
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = far ? plan->far_total : plan->near_total;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
 
     // operand prefetch registers: the next tile's digit chunks
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
     const uint32_t total = plan->far_total;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint32_t wr = wave * 16;
     const uint32_t koff = 16 * (lane >> 4);
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {

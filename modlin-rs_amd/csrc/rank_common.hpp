@@ -103,6 +103,13 @@ constexpr uint32_t KSTEP = 64;         // int8 K per MFMA step (one digit plane 
 // Pointers into global memory.  Loads and stores through them compile to global_*
 // instructions; through a generic pointer they are flat_*, whose completion also
 // holds up every wait for LDS results.
+// A value every lane holds alike (read from LDS), as a scalar: branches on it stay
+// uniform instead of running both sides under an exec mask.
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni32((uint32_t)(v >> 32)) << 32) | uni32((uint32_t)v);
+}
+
 template <typename T>
 using gptr = __attribute__((address_space(1))) T *;
 template <typename T>

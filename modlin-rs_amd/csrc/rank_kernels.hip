@@ -91,7 +91,7 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     // own column if v = j mod 16); lane = block row.  A step reads column j (for the
     // ballot, and as this lane's multiplier e), and speculatively its own columns of
     // rows `lane` and k, all at once; the pivot comes by readlane.
-    const uint32_t lane = tid & 63, wave = tid >> 6;
+    const uint32_t lane = tid & 63, wave = uni32(tid >> 6);
     uint32_t k = 0;
     for (uint32_t j = 0; j < w && k < Bt; j++) {
         uint64_t colv = U[lane][j];
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     __shared__ uint64_t Ur[BMAX][BMAX + 1];  // U[r][t], t > r (padded: the back substitution reads columns)
     __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
     __shared__ uint32_t swq[BMAX];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint64_t *eta = S.Eta;
     const uint64_t p = f.p;
     {

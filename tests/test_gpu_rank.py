@@ -197,7 +197,7 @@ def test_launch_parameters_do_not_change_ranks(ctx, method, tuning):
 
 def test_tuning_table_loaded_and_cleared(ctx, tmp_path):
     table = tmp_path / "tuning.txt"
-    table.write_text("rank m61 1024 many 2 1 256 4 32 0\n")
+    table.write_text("rank m61 1024 many 2 1 256 4 32 0 0\n")
     ctx.load_tuning(str(table))
     try:
         mats = workloads.c2(count=6, max_side=900)
