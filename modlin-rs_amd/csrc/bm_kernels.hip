@@ -15,7 +15,10 @@
 //     301-303 without the inverse of bb: the inversionless update scales c and every
 //     later discrepancy by nonzero factors, so the zero tests, the length changes
 //     and l are the reference's); on a length change (2l <= i, fp.rs:287-299) the
-//     old c[0..l] is first copied to t, and b/t swap roles by pointer.  b is read
+//     old c[0..l] is first copied to t, and b/t swap roles by pointer.  The update
+//     loop also forms the thread's part of the next step's dot product from the
+//     values it has just written, so after an update the next step skips its own
+//     pass over c.  b is read
 //     shifted by m (other threads' indices): the barrier of the next step orders
 //     those reads after the copy, and the copy of a later length change after them.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
@@ -76,22 +79,29 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     uint32_t l = 0, bl = 1, m = 1;
     uint64_t bb = 1 % p;
     unsigned long long bytes = 0;  // kept in a register, stored once at the end
+    MacSum<MODE> nxt;    // this thread's part of the next discrepancy, formed by an update
+    bool fused = false;  // nxt holds it (the previous step updated c)
     for (uint32_t i = 0; i < n; i++) {
-        MacSum<MODE> acc;
         uint32_t j = tid;
-        for (; j + (U - 1) * NT <= l; j += U * NT) {
-            uint64_t cv[U], sv[U];
+        uint64_t part;
+        if (fused) {
+            part = nxt.value(f);
+        } else {
+            MacSum<MODE> acc;
+            for (; j + (U - 1) * NT <= l; j += U * NT) {
+                uint64_t cv[U], sv[U];
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                cv[u] = c[j + u * NT];
-                sv[u] = s[i - j - u * NT];
+                for (uint32_t u = 0; u < U; u++) {
+                    cv[u] = c[j + u * NT];
+                    sv[u] = s[i - j - u * NT];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u += 2) acc.add2(f, cv[u], sv[u], cv[u + 1], sv[u + 1]);
             }
-#pragma unroll
-            for (uint32_t u = 0; u < U; u += 2) acc.add2(f, cv[u], sv[u], cv[u + 1], sv[u + 1]);
+            for (; j + NT <= l; j += 2 * NT) acc.add2(f, c[j], s[i - j], c[j + NT], s[i - j - NT]);
+            if (j <= l) acc.add2(f, c[j], s[i - j], 0, 0);
+            part = acc.value(f);
         }
-        for (; j + NT <= l; j += 2 * NT) acc.add2(f, c[j], s[i - j], c[j + NT], s[i - j - NT]);
-        if (j <= l) acc.add2(f, c[j], s[i - j], 0, 0);
-        uint64_t part = acc.value(f);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) part = add_mod(part, __shfl_xor(part, off, 64), p);
         // red[i & 1] is rewritten two steps later, after every thread has passed
@@ -104,6 +114,7 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
         bytes += 8ull + 16ull * l;
         if (d == 0) {
             m++;
+            fused = false;
             continue;
         }
         const uint32_t jmax = min(bl, n - m);
@@ -111,28 +122,41 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
         const uint32_t old_l = l;
         bytes += 24ull * jmax + (change ? 16ull * (old_l + 1) : 0);
         const uint32_t top = max(l + 1, m + jmax), a1 = m + jmax;
+        // The update also forms this thread's part of the next discrepancy,
+        // sum_j c'[j] s[i+1-j]: it covers every j < top, c' is zero above the next
+        // l (and below top it is exactly zero there), and c is zero from top on.
+        const uint32_t inext = i + 1 < n ? i + 1 : 0;  // s index bound (no next step: s[0..0], unused)
+        nxt = MacSum<MODE>();
         j = tid;
         for (; j + (U - 1) * NT < top; j += U * NT) {
-            uint64_t cv[U], yv[U];
+            uint64_t cv[U], yv[U], sv[U], nv[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 const uint32_t jj = j + u * NT;
                 cv[u] = c[jj];
                 yv[u] = (jj >= m && jj < a1) ? b[jj - m] : 0;
+                sv[u] = jj <= inext ? s[inext - jj] : 0;
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 const uint32_t jj = j + u * NT;
                 if (change && jj <= old_l) t[jj] = cv[u];
-                c[jj] = ff_step_t<MODE>(f, bb, cv[u], d, yv[u]);
+                nv[u] = ff_step_t<MODE>(f, bb, cv[u], d, yv[u]);
+                c[jj] = nv[u];
             }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u += 2) nxt.add2(f, nv[u], sv[u], nv[u + 1], sv[u + 1]);
         }
         for (; j < top; j += NT) {
             const uint64_t cv = c[j];
             const uint64_t y = (j >= m && j < a1) ? b[j - m] : 0;
+            const uint64_t sv = j <= inext ? s[inext - j] : 0;
             if (change && j <= old_l) t[j] = cv;
-            c[j] = ff_step_t<MODE>(f, bb, cv, d, y);
+            const uint64_t nv = ff_step_t<MODE>(f, bb, cv, d, y);
+            c[j] = nv;
+            nxt.add2(f, nv, sv, 0, 0);
         }
+        fused = true;
         if (change) {
             const gptr<uint64_t> tmp = b;
             b = t;

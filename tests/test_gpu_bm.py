@@ -115,3 +115,42 @@ def test_cooperative_equals_single_workgroup(ctx):
     single = ctx.lc_batch(f, [s], coop_min=1 << 31)
     assert ctx.stats()["bm_coop"] == 0
     assert coop == single == [oracle.lc(M61, s)]
+
+
+@pytest.mark.parametrize("wgs", [1, 3, 40, 130])
+def test_blocked_cooperative_workgroup_counts(ctx, wgs):
+    """The blocked cooperative kernel with few workgroups: each one adds up several
+    of the 128 exchanged values and owns up to 16 chunks of c and b (the count is
+    raised to what the chunks need)."""
+    rng = np.random.default_rng(5 + wgs)
+    f = modlin.Field(M61)
+    coef = rng.integers(1, M61, size=300, dtype=np.uint64)
+    init = rng.integers(0, M61, size=300, dtype=np.uint64)
+    cases = [rng.integers(0, M61, size=3000, dtype=np.uint64), modlin.lfsr(M61, coef, init, 2500),
+             np.zeros(1500, dtype=np.uint64)]
+    ctx.set_tuning(bm_coop_wgs=wgs)
+    try:
+        for s in cases:
+            print("blocked BM", wgs, s.size, flush=True)
+            got = ctx.lc_batch(f, [s], coop_min=1000)
+            assert ctx.stats()["bm_coop"] >= wgs
+            assert got[0] == oracle.lc(M61, s)
+    finally:
+        ctx.set_tuning()
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_batch_thread_counts(ctx, threads):
+    """The batched kernel at every thread count it is built for, on a mixed batch."""
+    rng = np.random.default_rng(threads)
+    f = modlin.Field(M61)
+    seqs = [rng.integers(0, M61, size=int(n), dtype=np.uint64) for n in rng.integers(1, 3000, size=12)]
+    coef = rng.integers(1, M61, size=40, dtype=np.uint64)
+    seqs.append(modlin.lfsr(M61, coef, rng.integers(0, M61, size=40, dtype=np.uint64), 2000))
+    seqs.append(np.zeros(700, dtype=np.uint64))
+    ctx.set_tuning(bm_threads=threads)
+    try:
+        print("batch BM threads", threads, flush=True)
+        assert ctx.lc_batch(f, seqs) == [oracle.lc(M61, s) for s in seqs]
+    finally:
+        ctx.set_tuning()
