@@ -469,8 +469,8 @@ __device__ __forceinline__ bool poll_tag(gptr<const uint64_t> a, uint64_t tag, u
     return true;
 }
 
-// Records of the steps of a block, written by wavefront 0 of k_bm_block: the step's
-// discrepancy d (< 2^61) with its kind in bits 62-63 (0: d = 0, 1: update,
+// Records of the steps of a block, written by wavefront 0 of bm_replay_block: the
+// step's discrepancy d (< p < 2^62) with its kind in bits 62-63 (0: d = 0, 1: update,
 // 2: update with a length change).  *prog, written with release order every few
 // steps, says how many records are there; followers wait for record t with
 // acquire loads.
@@ -497,7 +497,7 @@ __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp
             Pp = shift_up1(Pp, lane);
             continue;
         }
-        const uint64_t d = r & P61;
+        const uint64_t d = r & ((1ull << 62) - 1);  // d < p < 2^62
         const uint64_t nP = ff_step_t<MODE>(f, bb, P, d, Pp);
         if (fl == 2) {
             Pp = shift_up1(P, lane);
@@ -510,6 +510,93 @@ __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp
     }
     outP[lane] = P;
     outPb[lane] = Pb;
+}
+
+// The replay of one block of K steps (k_bm_block, k_bm_batch_blk).  Wavefront 0
+// replays the discrepancies (lane r: step i0+r of the current c and of the current
+// x^m b) with the length test and writes one record per step; wavefronts 1 and 2
+// follow it with the coefficients of U (C's part) and V (B's part), wavefront 3
+// with the rest of the reference's scalar state (m, bl) and the byte count.  The
+// block-start state (l, bl, m, bb) is the same in every thread; the new state goes
+// to sState: l, bl, m, bb, whether B changed, the byte count, whether c changed.
+template <int MODE>
+__device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n, uint32_t i0, uint32_t Kp,
+                                                uint32_t lane, uint32_t wid, uint32_t l, uint32_t bl, uint32_t m,
+                                                uint64_t bb, unsigned long long &bytes, const uint64_t *sE,
+                                                uint64_t (*sPoly)[BM_BK], uint64_t *sRec, uint64_t *sJunk,
+                                                uint32_t *sProg_, uint64_t *sState) {
+    constexpr uint32_t K = BM_BK;
+    const uint64_t p = f.p;
+    uint32_t &sProg = *sProg_;
+    if (wid == 0) {
+        // for 2^61 - 1, e and eb are kept in [0, p + 7] and only the broadcast d
+        // is reduced (on the scalar unit)
+        uint64_t e = sE[lane], eb = sE[K + lane];
+        __asm__ volatile("" : "+v"(e), "+v"(eb));  // loaded before the loop: no LDS wait at its head
+        for (uint32_t t = 0; t < Kp; t++) {
+            const uint32_t i = i0 + t;
+            uint64_t d = lane_bcast64(e, t);
+            if (MODE == MODE_M61 && d >= P61) d -= P61;
+            uint64_t fl = 0;
+            if (d == 0) {  // fp.rs:279-282
+                eb = shift_up1(eb, lane);
+            } else {
+                const uint64_t ne = MODE == MODE_M61 ? ff_step_m61_lazy(bb, e, d, eb) : ff_step_t<MODE>(f, bb, e, d, eb);
+                if (2 * l <= i) {  // length change, fp.rs:287-299
+                    eb = shift_up1(e, lane);
+                    l = i + 1 - l;
+                    bb = d;
+                    fl = 2;
+                } else {
+                    eb = shift_up1(eb, lane);
+                    fl = 1;
+                }
+                e = ne;
+            }
+            // lane 0 writes the record, the other lanes their own junk slots: no
+            // branch on the lane, so the loop's control stays scalar
+            *(lane == 0 ? &sRec[t] : &sJunk[lane]) = d | (fl << 62);
+            if ((t & 7) == 7 || t + 1 == Kp)
+                __hip_atomic_store(lane == 0 ? &sProg : (uint32_t *)&sJunk[64 + lane], t + 1, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (lane == 0) sState[3] = bb;
+    } else if (wid == 1) {
+        bm_replay_poly<MODE>(f, Kp, lane, bb, lane == 0 ? 1 % p : 0, 0, &sProg, sRec, sPoly[0], sPoly[2]);
+    } else if (wid == 2) {
+        bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3]);
+    } else if (wid == 3) {
+        uint32_t avail = 0, upd = 0, bch = 0;
+        for (uint32_t t = 0; t < Kp; t++) {
+            const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, avail) >> 62);
+            const uint32_t i = i0 + t;
+            bytes += 8ull + 16ull * l;
+            if (fl == 0) {
+                m++;
+                continue;
+            }
+            const uint32_t jmax = min(bl, n - m);
+            bytes += 24ull * jmax + (fl == 2 ? 16ull * (l + 1) : 0);
+            upd = 1;
+            if (fl == 2) {
+                const uint32_t old_l = l;
+                bl = old_l + 1;
+                l = i + 1 - old_l;
+                m = 1;
+                bch = 1;
+            } else {
+                m++;
+            }
+        }
+        if (lane == 0) {
+            sState[0] = l;
+            sState[1] = bl;
+            sState[2] = m;
+            sState[4] = bch;
+            sState[5] = bytes;
+            sState[6] = upd;
+        }
+    }
 }
 
 template <int MODE>
@@ -620,81 +707,9 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
         __syncthreads();
         if (uni32(sAbort) != 0 || uni32(__hip_atomic_load(&x->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
             break;
-        // ---- 3. the K steps.  Wavefront 0 replays the discrepancies (lane r: step
-        // i0+r of the current c and of the current x^m b) with the length test and
-        // writes one record per step; wavefronts 1 and 2 follow it with the
-        // coefficients of U (C's part) and V (B's part), wavefront 3 with the rest of
-        // the reference's scalar state (m, bl) and the byte count.
+        // ---- 3. the K steps (wavefronts 0-3)
         const uint32_t l0 = l, bl0 = bl, m0 = m;
-        if (wid == 0) {
-            // for 2^61 - 1, e and eb are kept in [0, p + 7] and only the broadcast d
-            // is reduced (on the scalar unit)
-            uint64_t e = sE[lane], eb = sE[K + lane];
-            __asm__ volatile("" : "+v"(e), "+v"(eb));  // loaded before the loop: no LDS wait at its head
-            for (uint32_t t = 0; t < Kp; t++) {
-                const uint32_t i = i0 + t;
-                uint64_t d = lane_bcast64(e, t);
-                if (MODE == MODE_M61 && d >= P61) d -= P61;
-                uint64_t fl = 0;
-                if (d == 0) {  // fp.rs:279-282
-                    eb = shift_up1(eb, lane);
-                } else {
-                    const uint64_t ne = MODE == MODE_M61 ? ff_step_m61_lazy(bb, e, d, eb) : ff_step_t<MODE>(f, bb, e, d, eb);
-                    if (2 * l <= i) {  // length change, fp.rs:287-299
-                        eb = shift_up1(e, lane);
-                        l = i + 1 - l;
-                        bb = d;
-                        fl = 2;
-                    } else {
-                        eb = shift_up1(eb, lane);
-                        fl = 1;
-                    }
-                    e = ne;
-                }
-                // lane 0 writes the record, the other lanes their own junk slots: no
-                // branch on the lane, so the loop's control stays scalar
-                *(lane == 0 ? &sRec[t] : &sJunk[lane]) = d | (fl << 62);
-                if ((t & 7) == 7 || t + 1 == Kp)
-                    __hip_atomic_store(lane == 0 ? &sProg : (uint32_t *)&sJunk[64 + lane], t + 1, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            if (lane == 0) sState[3] = bb;
-        } else if (wid == 1) {
-            bm_replay_poly<MODE>(f, Kp, lane, bb, lane == 0 ? 1 % p : 0, 0, &sProg, sRec, sPoly[0], sPoly[2]);
-        } else if (wid == 2) {
-            bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3]);
-        } else if (wid == 3) {
-            uint32_t avail = 0, upd = 0, bch = 0;
-            for (uint32_t t = 0; t < Kp; t++) {
-                const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, avail) >> 62);
-                const uint32_t i = i0 + t;
-                bytes += 8ull + 16ull * l;
-                if (fl == 0) {
-                    m++;
-                    continue;
-                }
-                const uint32_t jmax = min(bl, n - m);
-                bytes += 24ull * jmax + (fl == 2 ? 16ull * (l + 1) : 0);
-                upd = 1;
-                if (fl == 2) {
-                    const uint32_t old_l = l;
-                    bl = old_l + 1;
-                    l = i + 1 - old_l;
-                    m = 1;
-                    bch = 1;
-                } else {
-                    m++;
-                }
-            }
-            if (lane == 0) {
-                sState[0] = l;
-                sState[1] = bl;
-                sState[2] = m;
-                sState[4] = bch;
-                sState[5] = bytes;
-                sState[6] = upd;
-            }
-        }
+        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState);
         __syncthreads();
         l = uni32((uint32_t)sState[0]);
         bl = uni32((uint32_t)sState[1]);
@@ -779,6 +794,130 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_bm_batch_blk: the blocked form of k_bm_block inside one workgroup per
+// sequence (batches).  A block of K steps costs one pass over c and b for the E
+// sums, the replay of the K steps (bm_replay_block) and one pass forming the new C
+// and B, with no barrier between steps: about 6 products per coefficient and step
+// where k_bm_batch needs 3, but on independent data, not a latency chain.  C and B
+// live in HBM, double-buffered (one pass reads one copy and writes the other);
+// both passes run over the coefficients in rounds of NT, staged through LDS.
+// Needs p < 2^62 (the replay records hold the step's kind in bits 62-63).
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, FieldParams f) {
+    constexpr uint32_t NT = BM_BLOCK_THREADS, K = BM_BK, V2 = 2 * K, R = NT, SW = R + K - 1, NP = NT / V2;
+    BmSeq &Q = seqs[blockIdx.x];
+    const uint32_t n = Q.n, tid = threadIdx.x, lane = tid & 63, wid = uni32(tid >> 6);
+    const gptr<const uint64_t> s = to_global(Q.s);
+    const gptr<uint64_t> Cg[2] = {to_global(Q.c), to_global(Q.t)}, Bg[2] = {to_global(Q.b), to_global(Q.u)};
+    const uint64_t p = f.p;
+    __shared__ uint64_t sCr[R], sBr[R];  // E pass: this round's coefficients of C and B
+    __shared__ uint64_t sW[2][SW];       // E pass: s windows; application: windows of C and x^m B
+    __shared__ uint64_t sPoly[4][K];
+    __shared__ uint64_t sRed[NP][V2];
+    __shared__ uint64_t sE[V2];
+    __shared__ uint64_t sRec[K];
+    __shared__ uint64_t sJunk[2 * 64];
+    __shared__ uint64_t sState[7];
+    __shared__ uint32_t sProg;
+
+    if (tid == 0) {  // c = 1, b = 1 (fp.rs:267-270); coefficients above l (of b: at or above bl) are never read
+        Cg[0][0] = 1 % p;
+        Bg[0][0] = 1 % p;
+    }
+    __syncthreads();
+    uint32_t l = 0, bl = 1, m = 1, cc = 0, bc = 0;
+    uint64_t bb = 1 % p;
+    unsigned long long bytes = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += K) {
+        const uint32_t Kp = min(K, n - i0);
+        // ---- E sums: thread (v, part), v = value (Ec(r), then Eb(r)), part = a
+        // quarter of each round's coefficients; the window index of coefficient o
+        // of the round for lag r is r - o + R - 1.
+        {
+            const uint32_t v = tid & (V2 - 1), part = tid / V2, which = v / K, r = v & (K - 1);
+            const uint64_t *own = which ? sBr : sCr;
+            const uint64_t *win = sW[which] + r + R - 1;
+            MacSum<MODE> acc;
+            const uint32_t top_e = max(l, bl - 1);
+            for (uint32_t j0 = 0; j0 <= top_e; j0 += R) {
+                for (uint32_t k = tid; k < R; k += NT) {
+                    const uint32_t j = j0 + k;
+                    sCr[k] = j <= l ? Cg[cc][j] : 0;
+                    sBr[k] = j < bl ? Bg[bc][j] : 0;
+                }
+                for (uint32_t k = tid; k < 2 * SW; k += NT) {
+                    const uint32_t w = k / SW, u = k - w * SW;
+                    const int64_t xi = (int64_t)i0 - (int64_t)j0 - (int64_t)(R - 1) + (int64_t)u - (w ? (int64_t)m : 0);
+                    sW[w][u] = (xi >= 0 && xi < (int64_t)n) ? s[(uint32_t)xi] : 0;
+                }
+                __syncthreads();
+#pragma unroll 4
+                for (uint32_t o = part * (R / NP); o < (part + 1) * (R / NP); o += 2)
+                    acc.add2(f, own[o], win[-(int32_t)o], own[o + 1], win[-(int32_t)o - 1]);
+                __syncthreads();
+            }
+            sRed[part][v] = acc.value(f);
+        }
+        __syncthreads();
+        if (tid < V2) {
+            uint64_t v = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < NP; k++) v = add_mod(v, sRed[k][tid], p);
+            sE[tid] = v;
+        }
+        if (tid == 0) sProg = 0;
+        __syncthreads();
+        // ---- the K steps (wavefronts 0-3)
+        const uint32_t l0 = l, bl0 = bl, m0 = m;
+        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState);
+        __syncthreads();
+        l = uni32((uint32_t)sState[0]);
+        bl = uni32((uint32_t)sState[1]);
+        m = uni32((uint32_t)sState[2]);
+        bb = uni64(sState[3]);
+        const bool bch = uni32((uint32_t)sState[4]) != 0;
+        if (uni32((uint32_t)sState[6]) == 0) continue;  // no update in this block: C and B stand
+        // ---- new C (and B): thread tid forms coefficient j0 + tid of each round
+        // from the windows C[j - a], B[j - m0 - a], a < K
+        const uint32_t top_a = bch ? max(l, bl - 1) : l;
+        for (uint32_t j0 = 0; j0 <= top_a; j0 += R) {
+            for (uint32_t k = tid; k < 2 * SW; k += NT) {
+                const uint32_t w = k / SW, u = k - w * SW;
+                const int64_t j = (int64_t)j0 - (int64_t)(K - 1) + (int64_t)u - (w ? (int64_t)m0 : 0);
+                uint64_t y = 0;
+                if (w == 0) {
+                    if (j >= 0 && j <= (int64_t)l0) y = Cg[cc][(uint32_t)j];
+                } else {
+                    if (j >= 0 && j < (int64_t)bl0) y = Bg[bc][(uint32_t)j];
+                }
+                sW[w][u] = y;
+            }
+            __syncthreads();
+            const uint64_t *cw = sW[0] + tid + K - 1, *bw = sW[1] + tid + K - 1;
+            MacSum<MODE> cs, bs;
+            for (uint32_t a = 0; a < K; a++) {
+                const uint64_t xc = cw[-(int32_t)a], xb = bw[-(int32_t)a];
+                cs.add2(f, sPoly[0][a], xc, sPoly[1][a], xb);
+                if (bch) bs.add2(f, sPoly[2][a], xc, sPoly[3][a], xb);
+            }
+            const uint32_t j = j0 + tid;
+            if (j < n) {
+                Cg[cc ^ 1][j] = cs.value(f);
+                if (bch) Bg[bc ^ 1][j] = bs.value(f);
+            }
+            __syncthreads();
+        }
+        cc ^= 1;
+        if (bch) bc ^= 1;
+    }
+    if (tid == 0) {
+        Q.lc = l;
+        Q.bytes = (unsigned long long)sState[5];
+    }
+}
+
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 __global__ void k_widen_seq(const uint32_t *src, uint64_t *dst, size_t count) {
@@ -809,7 +948,7 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         offS[i] = cur;
         cur = align_up(cur + n * 8, 256);
         offC[i] = cur;
-        cur = align_up(cur + (count == 1 ? 4 : 3) * n * 8, 256);
+        cur = align_up(cur + 4 * n * 8, 256);  // c, b, t and a fourth buffer (blocked kernels)
     }
     const size_t off_stage = cur;
     cur = align_up(cur + maxn * 4, 256);
@@ -845,7 +984,7 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         hs[k].c = cd;
         hs[k].b = cd + n;
         hs[k].t = cd + 2 * n;
-        hs[k].u = count == 1 ? cd + 3 * n : nullptr;
+        hs[k].u = cd + 3 * n;
         hs[k].n = (uint32_t)n;
         hs[k].lc = 0;
         hs[k].bytes = 0;
@@ -937,6 +1076,17 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         return MODLIN_OK;
     }
 single:
+    // Blocks of K steps per pass (k_bm_batch_blk) unless the caller asks for the
+    // one-step-per-pass form (MODLIN_METHOD_PLAIN) or p >= 2^62.
+    if ((!opts || opts->method != MODLIN_METHOD_PLAIN) && p < (1ull << 62)) {
+        if (f.mode == MODE_M61)
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M61>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
+        else if (f.mode == MODE_M31)
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M31>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
+        else
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_GENERIC>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds,
+                               f);
+    } else {
     // threads per sequence: the tuning table's, else as many as keep the whole
     // batch resident (about 32 waves per CU), 256 at least
     const modlin_tuning_t tb = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
@@ -959,6 +1109,7 @@ single:
     }
 #undef MDL_BM_NT
 #undef MDL_BM
+    }
     if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
     MDL_HIP(hipGetLastError());
     rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));

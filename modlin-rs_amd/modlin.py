@@ -274,12 +274,15 @@ class Context:
                                                 _u64p(out)), "rank_batch")
         return [int(x) for x in out[:len(mats)]]
 
-    def lc_batch(self, field: Field, seqs: Sequence, profile: bool = False, coop_min: int = 0) -> list[int]:
+    def lc_batch(self, field: Field, seqs: Sequence, profile: bool = False, coop_min: int = 0,
+                 method: int = METHOD_AUTO) -> list[int]:
         """Linear complexities of a batch.  A batch of one sequence of length at
-        least coop_min (0: the library default) runs cooperatively on the whole GPU."""
+        least coop_min (0: the library default) runs cooperatively on the whole GPU;
+        method=METHOD_PLAIN runs the one-step-per-pass kernel instead of blocks of
+        64 steps (the result is the same)."""
         arr, keep = self._descs(seqs, False)
         out = np.zeros(max(1, len(seqs)), dtype=np.uint64)
-        opts = _Opts(0, 0, FLAG_PROFILE if profile else 0, coop_min)
+        opts = _Opts(method, 0, FLAG_PROFILE if profile else 0, coop_min)
         self._check(self._lib.modlin_lc_batch(self._h, ctypes.byref(field._f), arr, len(seqs), ctypes.byref(opts),
                                               _u64p(out)), "lc_batch")
         return [int(x) for x in out[:len(seqs)]]

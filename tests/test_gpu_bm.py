@@ -151,6 +151,29 @@ def test_batch_thread_counts(ctx, threads):
     ctx.set_tuning(bm_threads=threads)
     try:
         print("batch BM threads", threads, flush=True)
-        assert ctx.lc_batch(f, seqs) == [oracle.lc(M61, s) for s in seqs]
+        assert ctx.lc_batch(f, seqs, method=modlin.METHOD_PLAIN) == [oracle.lc(M61, s) for s in seqs]
     finally:
         ctx.set_tuning()
+
+
+@pytest.mark.parametrize("p", [2, 3, 65521, (1 << 31) - 1, M61, (1 << 62) - 57, (1 << 63) - 25])
+def test_blocked_batch_equals_plain(ctx, p):
+    """Blocks of 64 steps per pass (the default for p < 2^62) and one step per pass
+    give the oracle's linear complexities: sequences shorter than a block, lengths
+    around block multiples, recurrences whose order crosses block boundaries,
+    constant, zero and single-spike sequences."""
+    rng = np.random.default_rng(p % 1000 + 17)
+    f = modlin.Field(p)
+    seqs = [rng.integers(0, p, size=int(n), dtype=np.uint64) for n in (1, 2, 63, 64, 65, 127, 128, 129, 700, 2500)]
+    for L in (1, 5, 64, 65, 130):
+        coef = rng.integers(1, p, size=L, dtype=np.uint64)
+        seqs.append(modlin.lfsr(p, coef, rng.integers(0, p, size=L, dtype=np.uint64), 3 * L + 400))
+    seqs.append(np.full(900, 1 % p, dtype=np.uint64))
+    seqs.append(np.zeros(333, dtype=np.uint64))
+    spike = np.zeros(600, dtype=np.uint64)
+    spike[450] = 1 % p
+    seqs.append(spike)
+    print("blocked vs plain BM", p, len(seqs), flush=True)
+    want = [oracle.lc(p, s) for s in seqs]
+    assert ctx.lc_batch(f, seqs) == want
+    assert ctx.lc_batch(f, seqs, method=modlin.METHOD_PLAIN) == want
