@@ -89,7 +89,8 @@ typedef struct modlin_seq_desc {
                                    context's captured hipGraph of it */
 
 typedef struct modlin_opts {
-    uint32_t method; /* modlin_method */
+    uint32_t method; /* modlin_method.  BM: PLAIN runs one step per pass over c; AUTO and MFMA
+                        run blocks of 64 steps (p < 2^62).  The result is the same. */
     uint32_t panel;  /* panel width (0 = tuned default, else 1..64) */
     uint32_t flags;  /* MODLIN_FLAG_* */
     uint32_t coop_min; /* BM: a batch of one sequence at least this long runs cooperatively on
