@@ -62,8 +62,8 @@ def mode_of(p: int) -> str:
 class Job:
     """A workload class: a callable that runs it once and returns its results."""
 
-    def __init__(self, ctx, kind, p, items, label):
-        self.ctx, self.kind, self.p, self.items, self.label = ctx, kind, p, items, label
+    def __init__(self, ctx, kind, p, items, label, method=M.METHOD_AUTO):
+        self.ctx, self.kind, self.p, self.items, self.label, self.method = ctx, kind, p, items, label, method
         self.field = M.Field(p)
         self.max_n = max(it.n for it in items)
         self.count = len(items)
@@ -71,7 +71,7 @@ class Job:
     def run(self):
         if self.kind == "rank":
             return self.ctx.rank_batch(self.field, self.items)
-        return self.ctx.lc_batch(self.field, self.items)
+        return self.ctx.lc_batch(self.field, self.items, method=self.method)
 
     def time(self, reps):
         self.run()
@@ -135,8 +135,10 @@ def main():
     jobs.append((Job(ctx, "rank", 65521, [ctx.to_device(m) for m in small], "256 x 2048^2 u32, p = 65521"),
                  RANK_SPACE))
     seqs, _ = W.c4()
+    # bm_threads sizes the one-step-per-pass kernel (method PLAIN); the default,
+    # blocked form has no launch parameter to tune
     jobs.append((Job(ctx, "bm", M61, [ctx.to_device(s, square=False) for s in seqs],
-                     "1024 BM sequences, n in [1000, 20000]"), BM_BATCH_SPACE))
+                     "1024 BM sequences, n in [1000, 20000], one step per pass", M.METHOD_PLAIN), BM_BATCH_SPACE))
     if not args.quick:
         jobs.append((Job(ctx, "bm", M61, [ctx.to_device(W.c5_bm(), square=False)], "200000-step BM, MIXMAX-256"),
                      BM_SPACE))
