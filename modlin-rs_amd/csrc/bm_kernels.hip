@@ -1,5 +1,16 @@
 // Berlekamp-Massey over F_p on gfx950 (reference: src/fp.rs:257-308).
 //
+// Which kernel runs (lc_batch):
+//   one sequence of length >= coop_min, p <= 2^61   k_bm_block   (whole GPU, 64 steps per exchange)
+//   one such sequence, p > 2^61                      k_bm_coop    (whole GPU, one exchange per step)
+//   batches (and the rest), p < 2^62                 k_bm_batch_blk (one workgroup per sequence,
+//                                                                  64 steps per pass)
+//   method PLAIN, or p >= 2^62                       k_bm_batch   (one workgroup per sequence,
+//                                                                  one step per pass)
+// All four keep the reference's per-step decisions (d = 0, length change, l) and
+// return its linear complexity; the blocked ones reach each step's discrepancy
+// through the algebra described at k_bm_block.
+//
 // k_bm_batch: one workgroup per sequence.  Thread tid owns the indices
 // j = tid (mod NT) of c and t in both phases of a step, so it only ever reads c
 // values it wrote itself and a step needs one workgroup barrier (the one inside the
@@ -8,8 +19,8 @@
 //     reference and a nonzero scale here, see below) is a block-wide dot product:
 //     every lane accumulates its terms lazily (MacSum), four loads in flight at a
 //     time, the partials are added mod p across the wavefront with shuffles and
-//     across waves in LDS.  Addition mod p is
-//     associative, so the zero test sees the reference's sum;
+//     across waves in LDS.  Addition mod p is associative, so the zero test sees
+//     the reference's sum;
 //   * d = 0 only advances m (fp.rs:279-282);
 //   * otherwise c <- bb c - d x^m b over j in [0, max(l+1, m+jmax)) (fp.rs:292-294,
 //     301-303 without the inverse of bb: the inversionless update scales c and every
@@ -18,9 +29,9 @@
 //     old c[0..l] is first copied to t, and b/t swap roles by pointer.  The update
 //     loop also forms the thread's part of the next step's dot product from the
 //     values it has just written, so after an update the next step skips its own
-//     pass over c.  b is read
-//     shifted by m (other threads' indices): the barrier of the next step orders
-//     those reads after the copy, and the copy of a later length change after them.
+//     pass over c.  b is read shifted by m (other threads' indices): the barrier of
+//     the next step orders those reads after the copy, and the copy of a later
+//     length change after them.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
 
 #include <algorithm>
@@ -33,7 +44,7 @@ namespace mdl {
 struct BmSeq {
     const uint64_t *s;
     uint64_t *c, *b, *t;
-    uint64_t *u;  // fourth state buffer (blocked cooperative kernel only)
+    uint64_t *u;  // fourth state buffer (the blocked kernels double-buffer c and b)
     uint32_t n;
     uint32_t lc;
     unsigned long long bytes;  // algorithmic bytes: sum_i 8 + 16 l_i + [d_i != 0] 24 jmax_i + [change] 16 (old_l + 1)
