@@ -202,8 +202,8 @@ def main():
                     help="also the 8192^2 matrix over 2^31-1, the 16384^2 matrices and the 256 x 2048^2 u32 batch")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: measures one GPU only; --gpus above 1 is not supported "
+              "(a batch would shard by matrix or sequence, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     method = {"auto": M.METHOD_AUTO, "plain": M.METHOD_PLAIN, "mfma": M.METHOD_MFMA}[args.method]
 
