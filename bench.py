@@ -50,11 +50,13 @@ def limb_count(p: int) -> int:
     return L
 
 
-def cpu_baseline(mats, budget_n3=4.0e10):
-    """The oracle's port of fp::rank (OpenMP over rows, as rayon does) on the
-    first matrices of the batch whose sum of n^3 stays within the budget."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
+
+
+def _rank_sample(O, mats, budget_n3):
     sample, acc = [], 0.0
     for m in mats:
         n = m.shape[0]
@@ -67,9 +69,44 @@ def cpu_baseline(mats, budget_n3=4.0e10):
         O.rank(M61, m, m.shape[0])
     dt = time.perf_counter() - t0
     cols = sum(m.shape[0] for m in sample)
-    return {"value": cols / dt, "unit": "columns/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"first {len(sample)} matrices of the batch (sum n = {cols}, sum n^3 = {acc:.3g}), "
-                      f"{dt:.1f} s"}
+    return cols / dt, f"first {len(sample)} matrices of the batch (sum n = {cols}, sum n^3 = {acc:.3g}), {dt:.1f} s"
+
+
+def cpu_baseline(mats, budget_n3=4.0e10, budget_n3_1t=4.0e9):
+    """The oracle's port of fp::rank (OpenMP over rows, as rayon does) on the
+    first matrices of the batch whose sum of n^3 stays within the budget: on all
+    host threads, and on one thread with a smaller budget."""
+    O = _oracle()
+    cores = O.num_threads()
+    v, sample = _rank_sample(O, mats, budget_n3)
+    O.set_num_threads(1)
+    try:
+        v1, sample1 = _rank_sample(O, mats, budget_n3_1t)
+    finally:
+        O.set_num_threads(cores)
+    return {"value": v, "unit": "columns/s", "cores": cores, "kind": "port", "sample": sample,
+            "single_thread": {"value": v1, "unit": "columns/s", "cores": 1, "sample": sample1}}
+
+
+def cpu_bm(seqs, long_seq, budget_s=4.0, long_prefix=20000):
+    """The oracle's port of fp::linear_complexity (one thread, as the reference) on
+    the batch's sequences in order until the time budget is spent, and on the
+    first long_prefix steps of the long sequence (the same first steps as the
+    full run, with l still small, so a lower bound of its time per step)."""
+    O = _oracle()
+    t0, steps, k = time.perf_counter(), 0, 0
+    while k < len(seqs) and time.perf_counter() - t0 < budget_s:
+        O.lc(M61, seqs[k])
+        steps += seqs[k].size
+        k += 1
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    O.lc(M61, long_seq[:long_prefix])
+    dl = time.perf_counter() - t1
+    return {"bm_batch": {"value": steps / dt, "unit": "steps/s", "cores": 1, "kind": "port",
+                         "sample": f"first {k} sequences of the batch ({steps} steps), {dt:.1f} s"},
+            "bm_long": {"value": dl * 1e6 / long_prefix, "unit": "us/step", "cores": 1, "kind": "port",
+                        "sample": f"first {long_prefix} of the 200000 steps, {dl:.2f} s"}}
 
 
 def traffic_per_launch():
@@ -275,6 +312,8 @@ def main():
         out["copy_calibration"] = copy_calibration(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mats)
+        if not args.no_extras:
+            out["cpu_baseline_bm"] = cpu_bm(W.c4()[0], W.c5_bm())
     print(json.dumps(out))
 
 
