@@ -815,6 +815,125 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
 // both passes run over the coefficients in rounds of NT, staged through LDS.
 // Needs p < 2^62 (the replay records hold the step's kind in bits 62-63).
 // ---------------------------------------------------------------------------
+// x 2^s mod 2^61 - 1 for x < 2^40, 0 <= s < 61: the bits shifted past 2^61 wrap
+// around (2^61 == 1); result < 2^61 + 2^40, congruent, not reduced.
+__device__ __forceinline__ uint64_t rot_m61(uint64_t x, uint32_t s) {
+    return s == 0 ? x : ((x << s) & P61) + (x >> (61 - s));
+}
+
+constexpr uint32_t BM_RA = 1024;           // outputs per round of the matrix-core application
+constexpr uint32_t BM_PW = BM_RA + 128;    // digit plane length of a window round
+constexpr uint32_t BM_RP = 160;            // reversed polynomial digit plane length
+
+// The application of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.
+// With balanced 8-digit expansions W^ = sum_u 256^u W_u of the window values and
+// P^ = sum_v 256^v P_v of the polynomial coefficients,
+//   c_new[j] = sum_{a<64} U[a] C[j-a] + V[a] B[j-m0-a] == sum_s 256^s Q_s[j],
+//   Q_s[j] = sum_{u+v=s} sum_a C_u[j-a] U_v[a] + B_u[j-m0-a] V_v[a],
+// and b_new alike with Ub, Vb.  A 16 x 16 output tile holds 256 consecutive
+// outputs j = j0 + 256 t + 16 r + c; with x = c - a it is the product of the
+// Hankel rows A[r][x] = W_u[j0 + 256 t + 16 r + x] (x in [-64, 64), two K chunks of
+// 64, read as aligned 16-byte runs of a forward digit plane) and the Toeplitz
+// columns B[x][c] = P_v[c - x] (16-byte runs of a reversed, zero-padded digit plane
+// of the polynomial).  The MFMAs accumulate straight into the 15 classes s = u + v:
+// |Q_s| <= 8 pairs * 2 windows * 64 taps * 2^14 = 2^24, exact in int32.  Epilogue:
+// Q_s + 2^24 >= 0 rotated by 8 s bits mod p and summed; the offsets' total koff is
+// subtracted.  Wave w forms tile w & 3 of c_new (w < 4) or of b_new (w >= 4).
+__device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uint32_t n, uint32_t top_a, uint32_t l0,
+                                  uint32_t bl0, uint32_t m0, bool bch, gptr<const uint64_t> cold,
+                                  gptr<const uint64_t> bold, gptr<uint64_t> cnew, gptr<uint64_t> bnew,
+                                  const uint64_t (*sPoly)[BM_BK], int8_t (*sPl)[8][BM_PW],
+                                  int8_t (*sRP)[8][BM_RP], uint64_t koff) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    constexpr uint32_t NT = BM_BLOCK_THREADS;
+    // reversed polynomial digit planes: sRP[poly][v][79 - a] = digit v of poly[a]
+    // (polys U, V, Ub, Vb), zero outside a in [0, 64)
+    for (uint32_t k = tid; k < 4 * 8 * BM_RP / 4; k += NT) ((int32_t *)&sRP[0][0][0])[k] = 0;
+    __syncthreads();
+    for (uint32_t k = tid; k < 4 * BM_BK; k += NT) {
+        const uint32_t poly = k / BM_BK, a = k - poly * BM_BK;
+        int64_t v = balanced(sPoly[poly][a], P61);
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+            const int8_t dg = (int8_t)(v & 0xff);
+            sRP[poly][u][79 - a] = dg;
+            v = (v - dg) >> 8;
+        }
+    }
+    const uint32_t col = lane & 15, kq = lane >> 4, tau = wid & 3, pp = wid >> 2;
+    const v4i zero = {0, 0, 0, 0};
+    for (uint32_t j0 = 0; j0 <= top_a; j0 += BM_RA) {
+        // forward digit planes of the windows: sPl[w][u][y] = digit u of C[j0 - 64 + y]
+        // (w = 0, zero above l0) or of B[j0 - 64 + y - m0] (w = 1, zero at or above bl0)
+        for (uint32_t k = tid; k < 2 * BM_PW; k += NT) {
+            const uint32_t w = k / BM_PW, y = k - w * BM_PW;
+            const int64_t x = (int64_t)j0 - 64 + (int64_t)y - (w ? (int64_t)m0 : 0);
+            uint64_t val = 0;
+            if (w == 0) {
+                if (x >= 0 && x <= (int64_t)l0) val = cold[(uint32_t)x];
+            } else {
+                if (x >= 0 && x < (int64_t)bl0) val = bold[(uint32_t)x];
+            }
+            int64_t v = balanced(val, P61);
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) {
+                const int8_t dg = (int8_t)(v & 0xff);
+                sPl[w][u][y] = dg;
+                v = (v - dg) >> 8;
+            }
+        }
+        __syncthreads();
+        if (pp == 0 || bch) {
+            v4i acc[15];
+#pragma unroll
+            for (int q = 0; q < 15; q++) acc[q] = zero;
+            const uint32_t abase = 256 * tau + 16 * col + 16 * kq;  // A row col (= r), K bytes 16 kq ..
+            const uint32_t bbase = 15 - col + 16 * kq;               // B column col
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                // reload the A runs for every v (kept in LDS, not in registers)
+                __asm__ volatile("" ::: "memory");
+                v4i b00, b01, b10, b11;  // [window][K chunk]
+                __builtin_memcpy(&b00, &sRP[2 * pp][v][bbase], 16);
+                __builtin_memcpy(&b01, &sRP[2 * pp][v][bbase + 64], 16);
+                __builtin_memcpy(&b10, &sRP[2 * pp + 1][v][bbase], 16);
+                __builtin_memcpy(&b11, &sRP[2 * pp + 1][v][bbase + 64], 16);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    v4i a00, a01, a10, a11;
+                    __builtin_memcpy(&a00, &sPl[0][u][abase], 16);
+                    __builtin_memcpy(&a01, &sPl[0][u][abase + 64], 16);
+                    __builtin_memcpy(&a10, &sPl[1][u][abase], 16);
+                    __builtin_memcpy(&a11, &sPl[1][u][abase + 64], 16);
+                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a00, b00, acc[u + v], 0, 0, 0);
+                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a01, b01, acc[u + v], 0, 0, 0);
+                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a10, b10, acc[u + v], 0, 0, 0);
+                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a11, b11, acc[u + v], 0, 0, 0);
+                }
+            }
+            // register i of this lane: row 4 kq + i of the tile, column col
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint64_t w0 = 0, w1 = 0;
+#pragma unroll
+                for (int q = 0; q < 7; q++) w0 += rot_m61((uint64_t)(uint32_t)(acc[q][i] + (1 << 24)), (8 * q) % 61);
+#pragma unroll
+                for (int q = 7; q < 15; q++) w1 += rot_m61((uint64_t)(uint32_t)(acc[q][i] + (1 << 24)), (8 * q) % 61);
+                const uint64_t val =
+                    sub_mod(add_mod(canon_m61(fold_m61_64(w0)), canon_m61(fold_m61_64(w1)), P61), koff, P61);
+                const uint32_t j = j0 + 256 * tau + 16 * (4 * kq + i) + col;
+                if (j < n) {
+                    if (pp == 0)
+                        cnew[j] = val;
+                    else
+                        bnew[j] = val;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, FieldParams f) {
     constexpr uint32_t NT = BM_BLOCK_THREADS, K = BM_BK, V2 = 2 * K, R = NT, SW = R + K - 1, NP = NT / V2;
@@ -832,6 +951,13 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
     __shared__ uint64_t sJunk[2 * 64];
     __shared__ uint64_t sState[7];
     __shared__ uint32_t sProg;
+    // matrix-core application (2^61 - 1): digit planes of the windows and of U|V|Ub|Vb
+    __shared__ __attribute__((aligned(16))) int8_t sPl[MODE == MODE_M61 ? 2 : 1][8][MODE == MODE_M61 ? BM_PW : 16];
+    __shared__ __attribute__((aligned(16))) int8_t sRPl[MODE == MODE_M61 ? 4 : 1][8][MODE == MODE_M61 ? BM_RP : 16];
+    uint64_t koff = 0;  // sum over classes s < 15 of 2^24 256^s mod p (the epilogue's offsets)
+    if constexpr (MODE == MODE_M61) {
+        for (uint32_t q = 0; q < 15; q++) koff = add_mod(koff, canon_m61(rot_m61(1ull << 24, (8 * q) % 61)), P61);
+    }
 
     if (tid == 0) {  // c = 1, b = 1 (fp.rs:267-270); coefficients above l (of b: at or above bl) are never read
         Cg[0][0] = 1 % p;
@@ -893,6 +1019,10 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
         // ---- new C (and B): thread tid forms coefficient j0 + tid of each round
         // from the windows C[j - a], B[j - m0 - a], a < K
         const uint32_t top_a = bch ? max(l, bl - 1) : l;
+        if constexpr (MODE == MODE_M61) {
+            bm_apply_mfma_m61(tid, lane, wid, n, top_a, l0, bl0, m0, bch, Cg[cc], Bg[bc], Cg[cc ^ 1], Bg[bc ^ 1], sPoly,
+                              sPl, sRPl, koff);
+        } else
         for (uint32_t j0 = 0; j0 <= top_a; j0 += R) {
             for (uint32_t k = tid; k < 2 * SW; k += NT) {
                 const uint32_t w = k / SW, u = k - w * SW;
