@@ -934,6 +934,151 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
     }
 }
 
+constexpr uint32_t BM_RE = 512;        // coefficients per round of the matrix-core E sums
+constexpr uint32_t BM_SWE = BM_RE + 80;  // reversed s digit plane length of a round
+
+// The eight signed digits (in [-128, 127], least significant first) of the balanced
+// representative of x mod 2^61 - 1 as the bytes of one word: with c = 0x80 in every
+// byte, balanced(x) + c has byte u equal to digit u + 128, so xor c gives digit u in
+// two's complement.  Equal to peeling the low byte as an int8 eight times.
+__device__ __forceinline__ uint64_t digits8_m61(uint64_t x) {
+    constexpr uint64_t c = 0x8080808080808080ull;
+    return ((uint64_t)balanced(x, P61) + c) ^ c;
+}
+
+// Eight values' digit words d[t] -> eight plane words: byte t of plane word u is byte u
+// of d[t] (an 8x8 byte transpose, two 4x4 transposes per half), stored at
+// dst[u * stride + k .. + 8] (k and stride multiples of 8).
+__device__ __forceinline__ void store_planes8(int8_t *dst, uint32_t stride, uint32_t k, const uint64_t d[8]) {
+    uint32_t q[2][2][4];  // [half of the digits][group of four values][digit within the half]
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const uint32_t a = (uint32_t)(d[4 * g] >> (32 * h)), b = (uint32_t)(d[4 * g + 1] >> (32 * h)),
+                           c = (uint32_t)(d[4 * g + 2] >> (32 * h)), e = (uint32_t)(d[4 * g + 3] >> (32 * h));
+            const uint32_t t0 = __byte_perm(a, b, 0x5140), t1 = __byte_perm(a, b, 0x7362);  // a0 b0 a1 b1 | a2 b2 a3 b3
+            const uint32_t t2 = __byte_perm(c, e, 0x5140), t3 = __byte_perm(c, e, 0x7362);
+            q[h][g][0] = __byte_perm(t0, t2, 0x5410);  // a0 b0 c0 e0
+            q[h][g][1] = __byte_perm(t0, t2, 0x7632);
+            q[h][g][2] = __byte_perm(t1, t3, 0x5410);
+            q[h][g][3] = __byte_perm(t1, t3, 0x7632);
+        }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        *(uint64_t *)(dst + u * stride + k) = (uint64_t)q[u >> 2][0][u & 3] | ((uint64_t)q[u >> 2][1][u & 3] << 32);
+}
+
+// The E sums of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.  With
+// D = x^m B (D[j] = B[j - m]) both sums read the same window of s:
+//   Ec(r) = sum_j C[j] s[i0+r-j],   Eb(r) = sum_j D[j] s[i0+r-j],
+// and with signed base-256 digits, Ec(r) == sum_{u,v} 256^(u+v) T_v[u][r] where
+// T_v[u][r] = sum_j C_u[j] s_v[i0+r-j] (rows 8 + u likewise for D).  One MFMA per
+// (digit v of s, K chunk of 64 coefficients): rows 0-7 the digit planes of C, rows
+// 8-15 those of D, columns 16 lags r of one of four lag tiles, K = j.  The A runs
+// are aligned 16-byte runs of the planes; the B operand is the Toeplitz
+// s_v[i0+r-j], per lane a 16-byte run of a reversed s digit plane at a lane-fixed
+// byte offset mod 4, put together from five aligned dwords with byte shifts.  A
+// round takes 512 coefficients, so |T| <= 512 * 2^14 = 2^23; after each round the T
+// of equal u + v in a lane (four at most, |sum| <= 2^25) go into a residue: plus
+// 2^25, rotated by 8 (u + v) bits mod p, summed, minus the offsets.  Wave w forms lag
+// tile w & 3 over the digits v of s in [4 (w >> 2), 4 (w >> 2) + 4); the two halves
+// meet in LDS.  Rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl)
+// are skipped.
+__device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uint32_t n, uint32_t i0, uint32_t l,
+                                  uint32_t bl, uint32_t m, gptr<const uint64_t> s, gptr<const uint64_t> cp,
+                                  gptr<const uint64_t> bp, int8_t *pl, uint64_t *sE) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    constexpr uint32_t RE = BM_RE, SWE = BM_SWE;
+    // the lane constants below are formed anew per call: hoisted out of the caller's
+    // loop they stay live through the application and cost a block of occupancy
+    __asm__ volatile("" : "+v"(lane), "+v"(tid));
+    int8_t *AP = pl, *SP = pl + 16 * RE;  // AP[16][RE]: planes of C, then of D; SP[8][SWE]
+    const uint32_t rho = wid & 3, vh = wid >> 2, col = lane & 15, kq = lane >> 4;
+    const uint32_t zb = 63 - 16 * rho - col + 16 * kq, sh = zb & 3;
+    const v4i zero = {0, 0, 0, 0};
+    // this lane's offsets per round: 2^25 256^(4 (kq & 1) + 4 vh + c), c < 7
+    uint64_t koff = 0;
+    for (uint32_t c = 0; c < 7; c++)
+        koff = add_mod(koff, canon_m61(rot_m61(1ull << 25, (8 * (4 * (kq & 1) + 4 * vh + c)) % 61)), P61);
+    uint64_t epart = 0;
+    const uint32_t top_e = max(l, m + bl - 1);
+    for (uint32_t j0 = 0; j0 <= top_e; j0 += RE) {
+        if (j0 > l && j0 + RE <= m) continue;  // uniform: neither C nor D reaches this round
+        // digit planes, eight values per thread: AP[u][k] of C[j0 + k] (zero above l),
+        // AP[8 + u][k] of D[j0 + k]; SP[v][z] of s[i0 - j0 + 63 - z] (zero outside [0, n))
+        if (tid < 2 * RE / 8) {
+            const uint32_t w = tid / (RE / 8), k = 8 * (tid % (RE / 8));
+            uint64_t d[8];
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                const uint32_t j = j0 + k + t;
+                uint64_t x = 0;
+                if (w == 0) {
+                    if (j <= l) x = cp[j];
+                } else {
+                    if (j >= m && j - m < bl) x = bp[j - m];
+                }
+                d[t] = digits8_m61(x);
+            }
+            store_planes8(AP + w * 8 * RE, RE, k, d);
+        } else if (tid < 2 * RE / 8 + SWE / 8) {
+            const uint32_t z = 8 * (tid - 2 * RE / 8);
+            const int64_t x0 = (int64_t)i0 - (int64_t)j0 + 63 - (int64_t)z;
+            uint64_t d[8];
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                const int64_t x = x0 - (int64_t)t;
+                d[t] = digits8_m61((x >= 0 && x < (int64_t)n) ? s[(uint32_t)x] : 0);
+            }
+            store_planes8(SP, SWE, z, d);
+        }
+        __syncthreads();
+        v4i acc[4] = {zero, zero, zero, zero};
+#pragma unroll 1
+        for (uint32_t kc = 0; kc < 8; kc++) {
+            __asm__ volatile("" ::: "memory");  // keeps the loads of one chunk from being hoisted over the others
+            v4i af;
+            __builtin_memcpy(&af, AP + col * RE + 64 * kc + 16 * kq, 16);
+#pragma unroll
+            for (uint32_t v = 0; v < 4; v++) {
+                // bytes zb + 64 kc .. + 16 of plane 4 vh + v: five aligned dwords, shifted
+                const uint32_t *q = (const uint32_t *)(SP + (4 * vh + v) * SWE + zb + 64 * kc - sh);
+                const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+                v4i bf;
+                bf[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
+                bf[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
+                bf[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
+                bf[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
+                acc[v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[v], 0, 0, 0);
+            }
+        }
+        {  // rows 4 kq + i (digit u = 4 (kq & 1) + i of C, or of D); class c = i + v
+            int32_t cl[7];
+#pragma unroll
+            for (int c = 0; c < 7; c++) cl[c] = 0;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) cl[i + v] += acc[v][i];
+            uint64_t w = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < 7; c++)
+                w = fold_m61_64(
+                    w + rot_m61((uint64_t)(uint32_t)(cl[c] + (1 << 25)), (8 * (4 * (kq & 1) + 4 * vh + c)) % 61));
+            epart = add_mod(epart, sub_mod(canon_m61(w), koff, P61), P61);
+        }
+        __syncthreads();
+    }
+    // lanes kq and kq ^ 1 hold digits 0-3 and 4-7 of the same sum (Ec for kq < 2, Eb
+    // for kq >= 2); then the two digit halves of s, from the waves w and w ^ 4
+    epart = add_mod(epart, __shfl_xor(epart, 16, 64), P61);
+    uint64_t *red = (uint64_t *)pl;  // the planes are free after the last round's barrier
+    if ((kq & 1) == 0) red[vh * 2 * BM_BK + (kq >> 1) * BM_BK + 16 * rho + col] = epart;
+    __syncthreads();
+    if (tid < 2 * BM_BK) sE[tid] = add_mod(red[tid], red[2 * BM_BK + tid], P61);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, FieldParams f) {
     constexpr uint32_t NT = BM_BLOCK_THREADS, K = BM_BK, V2 = 2 * K, R = NT, SW = R + K - 1, NP = NT / V2;
@@ -972,7 +1117,9 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
         // ---- E sums: thread (v, part), v = value (Ec(r), then Eb(r)), part = a
         // quarter of each round's coefficients; the window index of coefficient o
         // of the round for lag r is r - o + R - 1.
-        {
+        if constexpr (MODE == MODE_M61) {
+            bm_esums_mfma_m61(tid, lane, wid, n, i0, l, bl, m, s, Cg[cc], Bg[bc], &sPl[0][0][0], sE);
+        } else {
             const uint32_t v = tid & (V2 - 1), part = tid / V2, which = v / K, r = v & (K - 1);
             const uint64_t *own = which ? sBr : sCr;
             const uint64_t *win = sW[which] + r + R - 1;
@@ -996,13 +1143,13 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
                 __syncthreads();
             }
             sRed[part][v] = acc.value(f);
-        }
-        __syncthreads();
-        if (tid < V2) {
-            uint64_t v = 0;
+            __syncthreads();
+            if (tid < V2) {
+                uint64_t v = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < NP; k++) v = add_mod(v, sRed[k][tid], p);
-            sE[tid] = v;
+                for (uint32_t k = 0; k < NP; k++) v = add_mod(v, sRed[k][tid], p);
+                sE[tid] = v;
+            }
         }
         if (tid == 0) sProg = 0;
         __syncthreads();
