@@ -825,6 +825,38 @@ constexpr uint32_t BM_RA = 1024;           // outputs per round of the matrix-co
 constexpr uint32_t BM_PW = BM_RA + 128;    // digit plane length of a window round
 constexpr uint32_t BM_RP = 160;            // reversed polynomial digit plane length
 
+// The eight signed digits (in [-128, 127], least significant first) of the balanced
+// representative of x mod 2^61 - 1 as the bytes of one word: with c = 0x80 in every
+// byte, balanced(x) + c has byte u equal to digit u + 128, so xor c gives digit u in
+// two's complement.  Equal to peeling the low byte as an int8 eight times.
+__device__ __forceinline__ uint64_t digits8_m61(uint64_t x) {
+    constexpr uint64_t c = 0x8080808080808080ull;
+    return ((uint64_t)balanced(x, P61) + c) ^ c;
+}
+
+// Eight values' digit words d[t] -> eight plane words: byte t of plane word u is byte u
+// of d[t] (an 8x8 byte transpose, two 4x4 transposes per half), stored at
+// dst[u * stride + k .. + 8] (k and stride multiples of 8).
+__device__ __forceinline__ void store_planes8(int8_t *dst, uint32_t stride, uint32_t k, const uint64_t d[8]) {
+    uint32_t q[2][2][4];  // [half of the digits][group of four values][digit within the half]
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const uint32_t a = (uint32_t)(d[4 * g] >> (32 * h)), b = (uint32_t)(d[4 * g + 1] >> (32 * h)),
+                           c = (uint32_t)(d[4 * g + 2] >> (32 * h)), e = (uint32_t)(d[4 * g + 3] >> (32 * h));
+            const uint32_t t0 = __byte_perm(a, b, 0x5140), t1 = __byte_perm(a, b, 0x7362);  // a0 b0 a1 b1 | a2 b2 a3 b3
+            const uint32_t t2 = __byte_perm(c, e, 0x5140), t3 = __byte_perm(c, e, 0x7362);
+            q[h][g][0] = __byte_perm(t0, t2, 0x5410);  // a0 b0 c0 e0
+            q[h][g][1] = __byte_perm(t0, t2, 0x7632);
+            q[h][g][2] = __byte_perm(t1, t3, 0x5410);
+            q[h][g][3] = __byte_perm(t1, t3, 0x7632);
+        }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        *(uint64_t *)(dst + u * stride + k) = (uint64_t)q[u >> 2][0][u & 3] | ((uint64_t)q[u >> 2][1][u & 3] << 32);
+}
+
 // The application of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.
 // With balanced 8-digit expansions W^ = sum_u 256^u W_u of the window values and
 // P^ = sum_v 256^v P_v of the polynomial coefficients,
@@ -846,6 +878,7 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
                                   int8_t (*sRP)[8][BM_RP], uint64_t koff) {
     typedef int v4i __attribute__((ext_vector_type(4)));
     constexpr uint32_t NT = BM_BLOCK_THREADS;
+    __asm__ volatile("" : "+v"(lane), "+v"(tid));  // lane constants per call (as in the E sums)
     // reversed polynomial digit planes: sRP[poly][v][79 - a] = digit v of poly[a]
     // (polys U, V, Ub, Vb), zero outside a in [0, 64)
     for (uint32_t k = tid; k < 4 * 8 * BM_RP / 4; k += NT) ((int32_t *)&sRP[0][0][0])[k] = 0;
@@ -865,22 +898,24 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
     for (uint32_t j0 = 0; j0 <= top_a; j0 += BM_RA) {
         // forward digit planes of the windows: sPl[w][u][y] = digit u of C[j0 - 64 + y]
         // (w = 0, zero above l0) or of B[j0 - 64 + y - m0] (w = 1, zero at or above bl0)
-        for (uint32_t k = tid; k < 2 * BM_PW; k += NT) {
-            const uint32_t w = k / BM_PW, y = k - w * BM_PW;
-            const int64_t x = (int64_t)j0 - 64 + (int64_t)y - (w ? (int64_t)m0 : 0);
-            uint64_t val = 0;
-            if (w == 0) {
-                if (x >= 0 && x <= (int64_t)l0) val = cold[(uint32_t)x];
-            } else {
-                if (x >= 0 && x < (int64_t)bl0) val = bold[(uint32_t)x];
-            }
-            int64_t v = balanced(val, P61);
+        uint32_t st = tid;
+        __asm__ volatile("" : "+v"(st));  // the staging indices per round, not held through the products
+        if (st < 2 * BM_PW / 8) {  // eight values per thread
+            const uint32_t w = st / (BM_PW / 8), y = 8 * (st % (BM_PW / 8));
+            const int64_t x0 = (int64_t)j0 - 64 + (int64_t)y - (w ? (int64_t)m0 : 0);
+            uint64_t d[8];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; u++) {
-                const int8_t dg = (int8_t)(v & 0xff);
-                sPl[w][u][y] = dg;
-                v = (v - dg) >> 8;
+            for (uint32_t t = 0; t < 8; t++) {
+                const int64_t x = x0 + (int64_t)t;
+                uint64_t val = 0;
+                if (w == 0) {
+                    if (x >= 0 && x <= (int64_t)l0) val = cold[(uint32_t)x];
+                } else {
+                    if (x >= 0 && x < (int64_t)bl0) val = bold[(uint32_t)x];
+                }
+                d[t] = digits8_m61(val);
             }
+            store_planes8(&sPl[w][0][0], BM_PW, y, d);
         }
         __syncthreads();
         if (pp == 0 || bch) {
@@ -937,38 +972,6 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
 constexpr uint32_t BM_RE = 512;        // coefficients per round of the matrix-core E sums
 constexpr uint32_t BM_SWE = BM_RE + 80;  // reversed s digit plane length of a round
 
-// The eight signed digits (in [-128, 127], least significant first) of the balanced
-// representative of x mod 2^61 - 1 as the bytes of one word: with c = 0x80 in every
-// byte, balanced(x) + c has byte u equal to digit u + 128, so xor c gives digit u in
-// two's complement.  Equal to peeling the low byte as an int8 eight times.
-__device__ __forceinline__ uint64_t digits8_m61(uint64_t x) {
-    constexpr uint64_t c = 0x8080808080808080ull;
-    return ((uint64_t)balanced(x, P61) + c) ^ c;
-}
-
-// Eight values' digit words d[t] -> eight plane words: byte t of plane word u is byte u
-// of d[t] (an 8x8 byte transpose, two 4x4 transposes per half), stored at
-// dst[u * stride + k .. + 8] (k and stride multiples of 8).
-__device__ __forceinline__ void store_planes8(int8_t *dst, uint32_t stride, uint32_t k, const uint64_t d[8]) {
-    uint32_t q[2][2][4];  // [half of the digits][group of four values][digit within the half]
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-#pragma unroll
-        for (int g = 0; g < 2; g++) {
-            const uint32_t a = (uint32_t)(d[4 * g] >> (32 * h)), b = (uint32_t)(d[4 * g + 1] >> (32 * h)),
-                           c = (uint32_t)(d[4 * g + 2] >> (32 * h)), e = (uint32_t)(d[4 * g + 3] >> (32 * h));
-            const uint32_t t0 = __byte_perm(a, b, 0x5140), t1 = __byte_perm(a, b, 0x7362);  // a0 b0 a1 b1 | a2 b2 a3 b3
-            const uint32_t t2 = __byte_perm(c, e, 0x5140), t3 = __byte_perm(c, e, 0x7362);
-            q[h][g][0] = __byte_perm(t0, t2, 0x5410);  // a0 b0 c0 e0
-            q[h][g][1] = __byte_perm(t0, t2, 0x7632);
-            q[h][g][2] = __byte_perm(t1, t3, 0x5410);
-            q[h][g][3] = __byte_perm(t1, t3, 0x7632);
-        }
-#pragma unroll
-    for (int u = 0; u < 8; u++)
-        *(uint64_t *)(dst + u * stride + k) = (uint64_t)q[u >> 2][0][u & 3] | ((uint64_t)q[u >> 2][1][u & 3] << 32);
-}
-
 // The E sums of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.  With
 // D = x^m B (D[j] = B[j - m]) both sums read the same window of s:
 //   Ec(r) = sum_j C[j] s[i0+r-j],   Eb(r) = sum_j D[j] s[i0+r-j],
@@ -1003,37 +1006,59 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         koff = add_mod(koff, canon_m61(rot_m61(1ull << 25, (8 * (4 * (kq & 1) + 4 * vh + c)) % 61)), P61);
     uint64_t epart = 0;
     const uint32_t top_e = max(l, m + bl - 1);
-    for (uint32_t j0 = 0; j0 <= top_e; j0 += RE) {
-        if (j0 > l && j0 + RE <= m) continue;  // uniform: neither C nor D reaches this round
-        // digit planes, eight values per thread: AP[u][k] of C[j0 + k] (zero above l),
-        // AP[8 + u][k] of D[j0 + k]; SP[v][z] of s[i0 - j0 + 63 - z] (zero outside [0, n))
-        if (tid < 2 * RE / 8) {
-            const uint32_t w = tid / (RE / 8), k = 8 * (tid % (RE / 8));
-            uint64_t d[8];
+    // rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl) are skipped
+    auto next_round = [&](uint32_t j) {
+        while (j <= top_e && j > l && j + RE <= m) j += RE;
+        return j;
+    };
+    // this thread's eight values of a round: AP rows of C (tid < 64) or D (tid < 128)
+    // at j0 + k, or SP columns (tid < 202) of s[i0 - j0 + 63 - z]; fetched a round ahead
+    uint32_t st = tid;
+    __asm__ volatile("" : "+v"(st));
+    const uint32_t sw = st < RE / 8 ? 0 : st < 2 * RE / 8 ? 1 : st < 2 * RE / 8 + SWE / 8 ? 2 : 3;
+    const uint32_t sk = sw < 2 ? 8 * (st % (RE / 8)) : 8 * (st - 2 * RE / 8);
+    auto fetch = [&](uint32_t j0, uint64_t raw[8]) {
+#pragma unroll
+        for (uint32_t t = 0; t < 8; t++) raw[t] = 0;
+        if (sw == 0) {
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
-                const uint32_t j = j0 + k + t;
-                uint64_t x = 0;
-                if (w == 0) {
-                    if (j <= l) x = cp[j];
-                } else {
-                    if (j >= m && j - m < bl) x = bp[j - m];
-                }
-                d[t] = digits8_m61(x);
+                const uint32_t j = j0 + sk + t;
+                if (j <= l) raw[t] = cp[j];
             }
-            store_planes8(AP + w * 8 * RE, RE, k, d);
-        } else if (tid < 2 * RE / 8 + SWE / 8) {
-            const uint32_t z = 8 * (tid - 2 * RE / 8);
-            const int64_t x0 = (int64_t)i0 - (int64_t)j0 + 63 - (int64_t)z;
-            uint64_t d[8];
+        } else if (sw == 1) {
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                const uint32_t j = j0 + sk + t;
+                if (j >= m && j - m < bl) raw[t] = bp[j - m];
+            }
+        } else if (sw == 2) {
+            const int64_t x0 = (int64_t)i0 - (int64_t)j0 + 63 - (int64_t)sk;
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
                 const int64_t x = x0 - (int64_t)t;
-                d[t] = digits8_m61((x >= 0 && x < (int64_t)n) ? s[(uint32_t)x] : 0);
+                if (x >= 0 && x < (int64_t)n) raw[t] = s[(uint32_t)x];
             }
-            store_planes8(SP, SWE, z, d);
+        }
+    };
+    uint64_t raw[8];
+    uint32_t j0 = next_round(0);
+    fetch(j0, raw);
+    while (j0 <= top_e) {
+        // digit planes: AP[u][k] of C[j0 + k] (zero above l), AP[8 + u][k] of D[j0 + k];
+        // SP[v][z] of s[i0 - j0 + 63 - z] (zero outside [0, n))
+        if (sw < 3) {
+            uint64_t d[8];
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) d[t] = digits8_m61(raw[t]);
+            if (sw < 2)
+                store_planes8(AP + sw * 8 * RE, RE, sk, d);
+            else
+                store_planes8(SP, SWE, sk, d);
         }
         __syncthreads();
+        const uint32_t jn = next_round(j0 + RE);
+        if (jn <= top_e) fetch(jn, raw);  // in flight during the products
         v4i acc[4] = {zero, zero, zero, zero};
 #pragma unroll 1
         for (uint32_t kc = 0; kc < 8; kc++) {
@@ -1069,6 +1094,7 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
             epart = add_mod(epart, sub_mod(canon_m61(w), koff, P61), P61);
         }
         __syncthreads();
+        j0 = jn;
     }
     // lanes kq and kq ^ 1 hold digits 0-3 and 4-7 of the same sum (Ec for kq < 2, Eb
     // for kq >= 2); then the two digit halves of s, from the waves w and w ^ 4
