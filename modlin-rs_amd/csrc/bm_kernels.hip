@@ -924,15 +924,25 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
             for (int q = 0; q < 15; q++) acc[q] = zero;
             const uint32_t abase = 256 * tau + 16 * col + 16 * kq;  // A row col (= r), K bytes 16 kq ..
             const uint32_t bbase = 15 - col + 16 * kq;               // B column col
+            const uint32_t bsh = bbase & 3;
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 // reload the A runs for every v (kept in LDS, not in registers)
                 __asm__ volatile("" ::: "memory");
-                v4i b00, b01, b10, b11;  // [window][K chunk]
-                __builtin_memcpy(&b00, &sRP[2 * pp][v][bbase], 16);
-                __builtin_memcpy(&b01, &sRP[2 * pp][v][bbase + 64], 16);
-                __builtin_memcpy(&b10, &sRP[2 * pp + 1][v][bbase], 16);
-                __builtin_memcpy(&b11, &sRP[2 * pp + 1][v][bbase + 64], 16);
+                // [window][K chunk]; bytes bbase .. + 16 (bbase mod 4 is fixed per lane)
+                // from five aligned dwords, shifted
+                auto run = [&](const int8_t *row) {
+                    const uint32_t *q = (const uint32_t *)(row - bsh);
+                    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+                    v4i r;
+                    r[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, bsh);
+                    r[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, bsh);
+                    r[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, bsh);
+                    r[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, bsh);
+                    return r;
+                };
+                const v4i b00 = run(&sRP[2 * pp][v][bbase]), b01 = run(&sRP[2 * pp][v][bbase + 64]);
+                const v4i b10 = run(&sRP[2 * pp + 1][v][bbase]), b11 = run(&sRP[2 * pp + 1][v][bbase + 64]);
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     v4i a00, a01, a10, a11;
