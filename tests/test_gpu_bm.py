@@ -161,7 +161,7 @@ def test_blocked_batch_equals_plain(ctx, p):
     """Blocks of 64 steps per pass (the default for p < 2^62) and one step per pass
     give the oracle's linear complexities: sequences shorter than a block, lengths
     around block multiples, recurrences whose order crosses block boundaries,
-    constant, zero and single-spike sequences."""
+    constant, zero and single-spike sequences, and long low-order runs."""
     rng = np.random.default_rng(p % 1000 + 17)
     f = modlin.Field(p)
     seqs = [rng.integers(0, p, size=int(n), dtype=np.uint64) for n in (1, 2, 63, 64, 65, 127, 128, 129, 700, 2500)]
@@ -173,6 +173,13 @@ def test_blocked_batch_equals_plain(ctx, p):
     spike = np.zeros(600, dtype=np.uint64)
     spike[450] = 1 % p
     seqs.append(spike)
+    # low order over a long run: x^m B moves thousands of coefficients above C, so
+    # whole rounds of coefficients between the two hold neither
+    coef = rng.integers(1, p, size=5, dtype=np.uint64)
+    seqs.append(modlin.lfsr(p, coef, rng.integers(0, p, size=5, dtype=np.uint64), 3000))
+    late = np.zeros(3000, dtype=np.uint64)
+    late[2800] = 1 % p
+    seqs.append(late)
     print("blocked vs plain BM", p, len(seqs), flush=True)
     want = [oracle.lc(p, s) for s in seqs]
     assert ctx.lc_batch(f, seqs) == want
