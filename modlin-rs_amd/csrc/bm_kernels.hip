@@ -925,35 +925,43 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
             const uint32_t abase = 256 * tau + 16 * col + 16 * kq;  // A row col (= r), K bytes 16 kq ..
             const uint32_t bbase = 15 - col + 16 * kq;               // B column col
             const uint32_t bsh = bbase & 3;
+            // [window][K chunk]; bytes bbase .. + 16 (bbase mod 4 is fixed per lane)
+            // from five aligned dwords, shifted
+            auto run = [&](const int8_t *row) {
+                const uint32_t *q = (const uint32_t *)(row - bsh);
+                const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+                v4i r;
+                r[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, bsh);
+                r[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, bsh);
+                r[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, bsh);
+                r[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, bsh);
+                return r;
+            };
+            // two digits v of the polynomials at a time: each A run serves both
 #pragma unroll
-            for (int v = 0; v < 8; v++) {
-                // reload the A runs for every v (kept in LDS, not in registers)
+            for (int v = 0; v < 8; v += 2) {
+                // reload the A runs for every pair (kept in LDS, not in registers)
                 __asm__ volatile("" ::: "memory");
-                // [window][K chunk]; bytes bbase .. + 16 (bbase mod 4 is fixed per lane)
-                // from five aligned dwords, shifted
-                auto run = [&](const int8_t *row) {
-                    const uint32_t *q = (const uint32_t *)(row - bsh);
-                    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
-                    v4i r;
-                    r[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, bsh);
-                    r[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, bsh);
-                    r[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, bsh);
-                    r[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, bsh);
-                    return r;
-                };
-                const v4i b00 = run(&sRP[2 * pp][v][bbase]), b01 = run(&sRP[2 * pp][v][bbase + 64]);
-                const v4i b10 = run(&sRP[2 * pp + 1][v][bbase]), b11 = run(&sRP[2 * pp + 1][v][bbase + 64]);
+                v4i b[2][4];  // [digit][window * 2 + K chunk]
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    b[h][0] = run(&sRP[2 * pp][v + h][bbase]);
+                    b[h][1] = run(&sRP[2 * pp][v + h][bbase + 64]);
+                    b[h][2] = run(&sRP[2 * pp + 1][v + h][bbase]);
+                    b[h][3] = run(&sRP[2 * pp + 1][v + h][bbase + 64]);
+                }
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    v4i a00, a01, a10, a11;
-                    __builtin_memcpy(&a00, &sPl[0][u][abase], 16);
-                    __builtin_memcpy(&a01, &sPl[0][u][abase + 64], 16);
-                    __builtin_memcpy(&a10, &sPl[1][u][abase], 16);
-                    __builtin_memcpy(&a11, &sPl[1][u][abase + 64], 16);
-                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a00, b00, acc[u + v], 0, 0, 0);
-                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a01, b01, acc[u + v], 0, 0, 0);
-                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a10, b10, acc[u + v], 0, 0, 0);
-                    acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a11, b11, acc[u + v], 0, 0, 0);
+                    v4i a[4];
+                    __builtin_memcpy(&a[0], &sPl[0][u][abase], 16);
+                    __builtin_memcpy(&a[1], &sPl[0][u][abase + 64], 16);
+                    __builtin_memcpy(&a[2], &sPl[1][u][abase], 16);
+                    __builtin_memcpy(&a[3], &sPl[1][u][abase + 64], 16);
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int x = 0; x < 4; x++)
+                            acc[u + v + h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[x], b[h][x], acc[u + v + h], 0, 0, 0);
                 }
             }
             // register i of this lane: row 4 kq + i of the tile, column col
