@@ -1000,9 +1000,10 @@ constexpr uint32_t BM_SWE = BM_RE + 80;  // reversed s digit plane length of a r
 // are aligned 16-byte runs of the planes; the B operand is the Toeplitz
 // s_v[i0+r-j], per lane a 16-byte run of a reversed s digit plane at a lane-fixed
 // byte offset mod 4, put together from five aligned dwords with byte shifts.  A
-// round takes 512 coefficients, so |T| <= 512 * 2^14 = 2^23; after each round the T
-// of equal u + v in a lane (four at most, |sum| <= 2^25) go into a residue: plus
-// 2^25, rotated by 8 (u + v) bits mod p, summed, minus the offsets.  Wave w forms lag
+// round takes 512 coefficients, so |T| grows by at most 512 * 2^14 = 2^23 a round;
+// every 8 rounds (|T| <= 2^26) and at the end the T of equal u + v in a lane (four
+// at most, |sum| <= 2^28) go into a residue: plus 2^28, rotated by 8 (u + v) bits
+// mod p, summed, minus the offsets.  Wave w forms lag
 // tile w & 3 over the digits v of s in [4 (w >> 2), 4 (w >> 2) + 4); the two halves
 // meet in LDS.  Rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl)
 // are skipped.
@@ -1018,11 +1019,30 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
     const uint32_t rho = wid & 3, vh = wid >> 2, col = lane & 15, kq = lane >> 4;
     const uint32_t zb = 63 - 16 * rho - col + 16 * kq, sh = zb & 3;
     const v4i zero = {0, 0, 0, 0};
-    // this lane's offsets per round: 2^25 256^(4 (kq & 1) + 4 vh + c), c < 7
+    // this lane's offsets per fold: 2^28 256^(4 (kq & 1) + 4 vh + c), c < 7
     uint64_t koff = 0;
     for (uint32_t c = 0; c < 7; c++)
-        koff = add_mod(koff, canon_m61(rot_m61(1ull << 25, (8 * (4 * (kq & 1) + 4 * vh + c)) % 61)), P61);
+        koff = add_mod(koff, canon_m61(rot_m61(1ull << 28, (8 * (4 * (kq & 1) + 4 * vh + c)) % 61)), P61);
     uint64_t epart = 0;
+    v4i acc[4] = {zero, zero, zero, zero};
+    uint32_t folds = 0;  // rounds in acc since the last fold (at most 8: |T| <= 2^26)
+    auto fold = [&]() {  // rows 4 kq + i (digit u = 4 (kq & 1) + i of C, or of D); class c = i + v
+        int32_t cl[7];
+#pragma unroll
+        for (int c = 0; c < 7; c++) cl[c] = 0;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) cl[i + v] += acc[v][i];
+        uint64_t w = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 7; c++)
+            w = fold_m61_64(
+                w + rot_m61((uint64_t)(uint32_t)(cl[c] + (1 << 28)), (8 * (4 * (kq & 1) + 4 * vh + c)) % 61));
+        epart = add_mod(epart, sub_mod(canon_m61(w), koff, P61), P61);
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] = zero;
+    };
     const uint32_t top_e = max(l, m + bl - 1);
     // rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl) are skipped
     auto next_round = [&](uint32_t j) {
@@ -1077,7 +1097,6 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         __syncthreads();
         const uint32_t jn = next_round(j0 + RE);
         if (jn <= top_e) fetch(jn, raw);  // in flight during the products
-        v4i acc[4] = {zero, zero, zero, zero};
 #pragma unroll 1
         for (uint32_t kc = 0; kc < 8; kc++) {
             __asm__ volatile("" ::: "memory");  // keeps the loads of one chunk from being hoisted over the others
@@ -1096,24 +1115,14 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
                 acc[v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[v], 0, 0, 0);
             }
         }
-        {  // rows 4 kq + i (digit u = 4 (kq & 1) + i of C, or of D); class c = i + v
-            int32_t cl[7];
-#pragma unroll
-            for (int c = 0; c < 7; c++) cl[c] = 0;
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) cl[i + v] += acc[v][i];
-            uint64_t w = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < 7; c++)
-                w = fold_m61_64(
-                    w + rot_m61((uint64_t)(uint32_t)(cl[c] + (1 << 25)), (8 * (4 * (kq & 1) + 4 * vh + c)) % 61));
-            epart = add_mod(epart, sub_mod(canon_m61(w), koff, P61), P61);
+        if (++folds == 8) {
+            fold();
+            folds = 0;
         }
         __syncthreads();
         j0 = jn;
     }
+    fold();
     // lanes kq and kq ^ 1 hold digits 0-3 and 4-7 of the same sum (Ec for kq < 2, Eb
     // for kq >= 2); then the two digit halves of s, from the waves w and w ^ 4
     epart = add_mod(epart, __shfl_xor(epart, 16, 64), P61);
