@@ -879,45 +879,55 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
     typedef int v4i __attribute__((ext_vector_type(4)));
     constexpr uint32_t NT = BM_BLOCK_THREADS;
     __asm__ volatile("" : "+v"(lane), "+v"(tid));  // lane constants per call (as in the E sums)
+    // this thread's eight window values of a round (threads < 2 BM_PW / 8): digit
+    // planes sPl[w][u][y .. y + 8] of C[j0 - 64 + y] (w = 0, zero above l0) or of
+    // B[j0 - 64 + y - m0] (w = 1, zero at or above bl0); fetched a round ahead
+    uint32_t st = tid;
+    __asm__ volatile("" : "+v"(st));  // the staging indices per call, not hoisted out of the step loop
+    const uint32_t sw = st < BM_PW / 8 ? 0 : st < 2 * BM_PW / 8 ? 1 : 2, sy = 8 * (st % (BM_PW / 8));
+    auto fetch = [&](uint32_t j0, uint64_t raw[8]) {
+#pragma unroll
+        for (uint32_t t = 0; t < 8; t++) raw[t] = 0;
+        const int64_t x0 = (int64_t)j0 - 64 + (int64_t)sy;
+        if (sw == 0) {
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                const int64_t x = x0 + (int64_t)t;
+                if (x >= 0 && x <= (int64_t)l0) raw[t] = cold[(uint32_t)x];
+            }
+        } else if (sw == 1) {
+#pragma unroll
+            for (uint32_t t = 0; t < 8; t++) {
+                const int64_t x = x0 + (int64_t)t - (int64_t)m0;
+                if (x >= 0 && x < (int64_t)bl0) raw[t] = bold[(uint32_t)x];
+            }
+        }
+    };
+    uint64_t raw[8];
+    fetch(0, raw);
     // reversed polynomial digit planes: sRP[poly][v][79 - a] = digit v of poly[a]
-    // (polys U, V, Ub, Vb), zero outside a in [0, 64)
+    // (polys U, V, Ub, Vb), zero outside a in [0, 64); thread k < 32 takes a = 8 (k & 7) ..
+    // + 8 of poly k >> 3, stored reversed at 72 - 8 (k & 7)
     for (uint32_t k = tid; k < 4 * 8 * BM_RP / 4; k += NT) ((int32_t *)&sRP[0][0][0])[k] = 0;
     __syncthreads();
-    for (uint32_t k = tid; k < 4 * BM_BK; k += NT) {
-        const uint32_t poly = k / BM_BK, a = k - poly * BM_BK;
-        int64_t v = balanced(sPoly[poly][a], P61);
+    if (st < 4 * BM_BK / 8) {
+        const uint32_t poly = st >> 3, a0 = 8 * (st & 7);
+        uint64_t d[8];
 #pragma unroll
-        for (uint32_t u = 0; u < 8; u++) {
-            const int8_t dg = (int8_t)(v & 0xff);
-            sRP[poly][u][79 - a] = dg;
-            v = (v - dg) >> 8;
-        }
+        for (uint32_t t = 0; t < 8; t++) d[t] = digits8_m61(sPoly[poly][a0 + 7 - t]);
+        store_planes8(&sRP[poly][0][0], BM_RP, 72 - a0, d);
     }
     const uint32_t col = lane & 15, kq = lane >> 4, tau = wid & 3, pp = wid >> 2;
     const v4i zero = {0, 0, 0, 0};
     for (uint32_t j0 = 0; j0 <= top_a; j0 += BM_RA) {
-        // forward digit planes of the windows: sPl[w][u][y] = digit u of C[j0 - 64 + y]
-        // (w = 0, zero above l0) or of B[j0 - 64 + y - m0] (w = 1, zero at or above bl0)
-        uint32_t st = tid;
-        __asm__ volatile("" : "+v"(st));  // the staging indices per round, not held through the products
-        if (st < 2 * BM_PW / 8) {  // eight values per thread
-            const uint32_t w = st / (BM_PW / 8), y = 8 * (st % (BM_PW / 8));
-            const int64_t x0 = (int64_t)j0 - 64 + (int64_t)y - (w ? (int64_t)m0 : 0);
+        if (sw < 2) {
             uint64_t d[8];
 #pragma unroll
-            for (uint32_t t = 0; t < 8; t++) {
-                const int64_t x = x0 + (int64_t)t;
-                uint64_t val = 0;
-                if (w == 0) {
-                    if (x >= 0 && x <= (int64_t)l0) val = cold[(uint32_t)x];
-                } else {
-                    if (x >= 0 && x < (int64_t)bl0) val = bold[(uint32_t)x];
-                }
-                d[t] = digits8_m61(val);
-            }
-            store_planes8(&sPl[w][0][0], BM_PW, y, d);
+            for (uint32_t t = 0; t < 8; t++) d[t] = digits8_m61(raw[t]);
+            store_planes8(&sPl[sw][0][0], BM_PW, sy, d);
         }
         __syncthreads();
+        if (j0 + BM_RA <= top_a) fetch(j0 + BM_RA, raw);  // in flight during the products
         if (pp == 0 || bch) {
             v4i acc[15];
 #pragma unroll
