@@ -484,13 +484,19 @@ __device__ __forceinline__ bool poll_tag(gptr<const uint64_t> a, uint64_t tag, u
 // step's discrepancy d (< p < 2^62) with its kind in bits 62-63 (0: d = 0, 1: update,
 // 2: update with a length change).  *prog, written with release order every few
 // steps, says how many records are there; followers wait for record t with
-// acquire loads.
-__device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *prog, uint32_t t, uint32_t &avail) {
-    for (uint32_t spins = 0; t >= avail && spins < BM_BLOCK_SPIN_LIMIT; spins++) {
-        avail = uni32(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-        if (t >= avail) __builtin_amdgcn_s_sleep(0);
+// acquire loads.  A follower holds the block's records one per lane in rv, reloaded
+// (one LDS read for the wave) each time the count moves past what it holds, and
+// takes record t from lane t: no LDS round trip on a step.
+__device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *prog, uint32_t t, uint32_t lane,
+                                              uint32_t &avail, uint64_t &rv) {
+    if (t >= avail) {
+        for (uint32_t spins = 0; t >= avail && spins < BM_BLOCK_SPIN_LIMIT; spins++) {
+            avail = uni32(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (t >= avail) __builtin_amdgcn_s_sleep(0);
+        }
+        rv = rec[lane];  // records below avail are final; the others are not read
     }
-    return uni64(rec[t]);
+    return lane_bcast64(rv, t);
 }
 
 // Replays the steps of one block for one polynomial pair (P, P'): P <- bb P - d P',
@@ -499,10 +505,10 @@ template <int MODE>
 __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp, uint32_t lane, uint64_t bb,
                                                uint64_t P, uint64_t Pp, uint32_t *prog, const uint64_t *rec,
                                                uint64_t *outP, uint64_t *outPb) {
-    uint64_t Pb = 0;
+    uint64_t Pb = 0, rv = 0;
     uint32_t avail = 0;
     for (uint32_t t = 0; t < Kp; t++) {
-        const uint64_t r = bm_record(rec, prog, t, avail);
+        const uint64_t r = bm_record(rec, prog, t, lane, avail, rv);
         const uint32_t fl = (uint32_t)(r >> 62);
         if (fl == 0) {
             Pp = shift_up1(Pp, lane);
@@ -578,8 +584,9 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
         bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3]);
     } else if (wid == 3) {
         uint32_t avail = 0, upd = 0, bch = 0;
+        uint64_t rv = 0;
         for (uint32_t t = 0; t < Kp; t++) {
-            const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, avail) >> 62);
+            const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, lane, avail, rv) >> 62);
             const uint32_t i = i0 + t;
             bytes += 8ull + 16ull * l;
             if (fl == 0) {
