@@ -573,7 +573,7 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
             // lane 0 writes the record, the other lanes their own junk slots: no
             // branch on the lane, so the loop's control stays scalar
             *(lane == 0 ? &sRec[t] : &sJunk[lane]) = d | (fl << 62);
-            if ((t & 7) == 7 || t + 1 == Kp)
+            if ((t & 3) == 3 || t + 1 == Kp)
                 __hip_atomic_store(lane == 0 ? &sProg : (uint32_t *)&sJunk[64 + lane], t + 1, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         }
