@@ -501,6 +501,8 @@ __device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *pro
 
 // Replays the steps of one block for one polynomial pair (P, P'): P <- bb P - d P',
 // P' <- x P' (or x P on a length change, when Pb keeps the old P and bb becomes d).
+// For 2^61 - 1 the steps are lazy (values in [0, p + 7]) and P, Pb are reduced once
+// at the end.
 template <int MODE>
 __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp, uint32_t lane, uint64_t bb,
                                                uint64_t P, uint64_t Pp, uint32_t *prog, const uint64_t *rec,
@@ -515,7 +517,7 @@ __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp
             continue;
         }
         const uint64_t d = r & ((1ull << 62) - 1);  // d < p < 2^62
-        const uint64_t nP = ff_step_t<MODE>(f, bb, P, d, Pp);
+        const uint64_t nP = MODE == MODE_M61 ? ff_step_m61_lazy(bb, P, d, Pp) : ff_step_t<MODE>(f, bb, P, d, Pp);
         if (fl == 2) {
             Pp = shift_up1(P, lane);
             Pb = P;
@@ -524,6 +526,10 @@ __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp
             Pp = shift_up1(Pp, lane);
         }
         P = nP;
+    }
+    if (MODE == MODE_M61) {
+        P = P >= P61 ? P - P61 : P;
+        Pb = Pb >= P61 ? Pb - P61 : Pb;
     }
     outP[lane] = P;
     outPb[lane] = Pb;
