@@ -556,6 +556,7 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
         // is reduced (on the scalar unit)
         uint64_t e = sE[lane], eb = sE[K + lane];
         __asm__ volatile("" : "+v"(e), "+v"(eb));  // loaded before the loop: no LDS wait at its head
+        uint64_t recv = 0;  // lane t: the record of step t, stored to sRec with each progress update
         for (uint32_t t = 0; t < Kp; t++) {
             const uint32_t i = i0 + t;
             uint64_t d = lane_bcast64(e, t);
@@ -576,12 +577,16 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
                 }
                 e = ne;
             }
-            // lane 0 writes the record, the other lanes their own junk slots: no
-            // branch on the lane, so the loop's control stays scalar
-            *(lane == 0 ? &sRec[t] : &sJunk[lane]) = d | (fl << 62);
-            if ((t & 3) == 3 || t + 1 == Kp)
+            recv = lane == t ? d | (fl << 62) : recv;
+            if ((t & 3) == 3 || t + 1 == Kp) {
+                // records 0 .. t, and then the count; lanes above t store values that
+                // no follower reads before they are rewritten.  Lane 0 releases the
+                // count, the other lanes store to their junk slots: no branch on the
+                // lane, so the loop's control stays scalar
+                sRec[lane] = recv;
                 __hip_atomic_store(lane == 0 ? &sProg : (uint32_t *)&sJunk[64 + lane], t + 1, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
         if (lane == 0) sState[3] = bb;
     } else if (wid == 1) {
