@@ -4,7 +4,8 @@
 //   one sequence of length >= coop_min, p <= 2^61   k_bm_block   (whole GPU, 64 steps per exchange)
 //   one such sequence, p > 2^61                      k_bm_coop    (whole GPU, one exchange per step)
 //   batches (and the rest), p < 2^62                 k_bm_batch_blk (one workgroup per sequence,
-//                                                                  64 steps per pass)
+//                                                                  64 steps per pass; matrix
+//                                                                  cores for 2^61 - 1)
 //   method PLAIN, or p >= 2^62                       k_bm_batch   (one workgroup per sequence,
 //                                                                  one step per pass)
 // All four keep the reference's per-step decisions (d = 0, length change, l) and
@@ -830,8 +831,10 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
 // and B, with no barrier between steps: about 6 products per coefficient and step
 // where k_bm_batch needs 3, but on independent data, not a latency chain.  C and B
 // live in HBM, double-buffered (one pass reads one copy and writes the other);
-// both passes run over the coefficients in rounds of NT, staged through LDS.
-// Needs p < 2^62 (the replay records hold the step's kind in bits 62-63).
+// both passes run over the coefficients in rounds, staged through LDS.  For
+// p = 2^61 - 1 both passes are signed-digit int8 products on the matrix cores
+// (bm_esums_mfma_m61, bm_apply_mfma_m61); other moduli use lazy vector sums
+// (MacSum).  Needs p < 2^62 (the replay records hold the step's kind in bits 62-63).
 // ---------------------------------------------------------------------------
 // x 2^s mod 2^61 - 1 for x < 2^40, 0 <= s < 61: the bits shifted past 2^61 wrap
 // around (2^61 == 1); result < 2^61 + 2^40, congruent, not reduced.
@@ -1031,17 +1034,17 @@ constexpr uint32_t BM_SWE = BM_RE + 80;  // reversed s digit plane length of a r
 // round takes 512 coefficients, so |T| grows by at most 512 * 2^14 = 2^23 a round;
 // every 8 rounds (|T| <= 2^26) and at the end the T of equal u + v in a lane (four
 // at most, |sum| <= 2^28) go into a residue: plus 2^28, rotated by 8 (u + v) bits
-// mod p, summed, minus the offsets.  Wave w forms lag
-// tile w & 3 over the digits v of s in [4 (w >> 2), 4 (w >> 2) + 4); the two halves
-// meet in LDS.  Rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl)
-// are skipped.
+// mod p, summed, minus the offsets.  Wave w forms lag tile w & 3 over the digits v
+// of s in [4 (w >> 2), 4 (w >> 2) + 4); the two halves meet in LDS.  Rounds holding
+// no coefficient of C (j <= l) nor of D (m <= j < m + bl) are skipped, and the next
+// round's values are fetched from HBM during the current round's products.
 __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uint32_t n, uint32_t i0, uint32_t l,
                                   uint32_t bl, uint32_t m, gptr<const uint64_t> s, gptr<const uint64_t> cp,
                                   gptr<const uint64_t> bp, int8_t *pl, uint64_t *sE) {
     typedef int v4i __attribute__((ext_vector_type(4)));
     constexpr uint32_t RE = BM_RE, SWE = BM_SWE;
     // the lane constants below are formed anew per call: hoisted out of the caller's
-    // loop they stay live through the application and cost a block of occupancy
+    // loop they would stay live through the application's products
     __asm__ volatile("" : "+v"(lane), "+v"(tid));
     int8_t *AP = pl, *SP = pl + 16 * RE;  // AP[16][RE]: planes of C, then of D; SP[8][SWE]
     const uint32_t rho = wid & 3, vh = wid >> 2, col = lane & 15, kq = lane >> 4;
