@@ -968,32 +968,28 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
                 r[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, bsh);
                 return r;
             };
-            // two digits v of the polynomials at a time: each A run serves both
+            // the round's A runs (the windows' Hankel rows) of this lane, loaded once
+            v4i a[8][4];  // [digit u][window * 2 + K chunk]
 #pragma unroll
-            for (int v = 0; v < 8; v += 2) {
-                // reload the A runs for every pair (kept in LDS, not in registers)
-                __asm__ volatile("" ::: "memory");
-                v4i b[2][4];  // [digit][window * 2 + K chunk]
+            for (int u = 0; u < 8; u++) {
+                __builtin_memcpy(&a[u][0], &sPl[0][u][abase], 16);
+                __builtin_memcpy(&a[u][1], &sPl[0][u][abase + 64], 16);
+                __builtin_memcpy(&a[u][2], &sPl[1][u][abase], 16);
+                __builtin_memcpy(&a[u][3], &sPl[1][u][abase + 64], 16);
+            }
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    b[h][0] = run(&sRP[2 * pp][v + h][bbase]);
-                    b[h][1] = run(&sRP[2 * pp][v + h][bbase + 64]);
-                    b[h][2] = run(&sRP[2 * pp + 1][v + h][bbase]);
-                    b[h][3] = run(&sRP[2 * pp + 1][v + h][bbase + 64]);
-                }
+            for (int v = 0; v < 8; v++) {
+                __asm__ volatile("" ::: "memory");  // one digit's B runs at a time
+                v4i b[4];  // [window * 2 + K chunk]
+                b[0] = run(&sRP[2 * pp][v][bbase]);
+                b[1] = run(&sRP[2 * pp][v][bbase + 64]);
+                b[2] = run(&sRP[2 * pp + 1][v][bbase]);
+                b[3] = run(&sRP[2 * pp + 1][v][bbase + 64]);
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    v4i a[4];
-                    __builtin_memcpy(&a[0], &sPl[0][u][abase], 16);
-                    __builtin_memcpy(&a[1], &sPl[0][u][abase + 64], 16);
-                    __builtin_memcpy(&a[2], &sPl[1][u][abase], 16);
-                    __builtin_memcpy(&a[3], &sPl[1][u][abase + 64], 16);
+                for (int u = 0; u < 8; u++)
 #pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int x = 0; x < 4; x++)
-                            acc[u + v + h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[x], b[h][x], acc[u + v + h], 0, 0, 0);
-                }
+                    for (int x = 0; x < 4; x++)
+                        acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][x], b[x], acc[u + v], 0, 0, 0);
             }
             // register i of this lane: row 4 kq + i of the tile, column col
 #pragma unroll
