@@ -836,8 +836,8 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
 // (bm_esums_mfma_m61, bm_apply_mfma_m61); other moduli use lazy vector sums
 // (MacSum).  Needs p < 2^62 (the replay records hold the step's kind in bits 62-63).
 // ---------------------------------------------------------------------------
-// x 2^s mod 2^61 - 1 for x < 2^40, 0 <= s < 61: the bits shifted past 2^61 wrap
-// around (2^61 == 1); result < 2^61 + 2^40, congruent, not reduced.
+// x 2^s mod 2^61 - 1 for x < 2^42, 0 <= s < 61: the bits shifted past 2^61 wrap
+// around (2^61 == 1); result < 2^61 + 2^42, congruent, not reduced.
 __device__ __forceinline__ uint64_t rot_m61(uint64_t x, uint32_t s) {
     return s == 0 ? x : ((x << s) & P61) + (x >> (61 - s));
 }
@@ -890,8 +890,10 @@ __device__ __forceinline__ void store_planes8(int8_t *dst, uint32_t stride, uint
 // columns B[x][c] = P_v[c - x] (16-byte runs of a reversed, zero-padded digit plane
 // of the polynomial).  The MFMAs accumulate straight into the 15 classes s = u + v:
 // |Q_s| <= 8 pairs * 2 windows * 64 taps * 2^14 = 2^24, exact in int32.  Epilogue:
-// Q_s + 2^24 >= 0 rotated by 8 s bits mod p and summed; the offsets' total koff is
-// subtracted.  Wave w forms tile w & 3 of c_new (w < 4) or of b_new (w >= 4).
+// three classes at a time, T_k = Q_3k + 2^8 Q_3k+1 + 2^16 Q_3k+2 (|T_k| < 2^41, exact
+// in int64), T_k + 2^41 rotated by 24 k bits mod p and summed (5 rotations, not 15);
+// the offsets' total koff is subtracted.  Wave w forms tile w & 3 of c_new (w < 4)
+// or of b_new (w >= 4).
 __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uint32_t n, uint32_t top_a, uint32_t l0,
                                   uint32_t bl0, uint32_t m0, bool bch, gptr<const uint64_t> cold,
                                   gptr<const uint64_t> bold, gptr<uint64_t> cnew, gptr<uint64_t> bnew,
@@ -994,13 +996,14 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
             // register i of this lane: row 4 kq + i of the tile, column col
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                uint64_t w0 = 0, w1 = 0;
+                uint64_t w = 0;  // five terms below 2^61 + 2^42 each
 #pragma unroll
-                for (int q = 0; q < 7; q++) w0 += rot_m61((uint64_t)(uint32_t)(acc[q][i] + (1 << 24)), (8 * q) % 61);
-#pragma unroll
-                for (int q = 7; q < 15; q++) w1 += rot_m61((uint64_t)(uint32_t)(acc[q][i] + (1 << 24)), (8 * q) % 61);
-                const uint64_t val =
-                    sub_mod(add_mod(canon_m61(fold_m61_64(w0)), canon_m61(fold_m61_64(w1)), P61), koff, P61);
+                for (int k = 0; k < 5; k++) {
+                    const int64_t t = (int64_t)acc[3 * k][i] + ((int64_t)acc[3 * k + 1][i] << 8) +
+                                      ((int64_t)acc[3 * k + 2][i] << 16);
+                    w += rot_m61((uint64_t)(t + (1ll << 41)), (24 * k) % 61);
+                }
+                const uint64_t val = sub_mod(canon_m61(fold_m61_64(w)), koff, P61);
                 const uint32_t j = j0 + 256 * tau + 16 * (4 * kq + i) + col;
                 if (j < n) {
                     if (pp == 0)
@@ -1179,9 +1182,9 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
     // matrix-core application (2^61 - 1): digit planes of the windows and of U|V|Ub|Vb
     __shared__ __attribute__((aligned(16))) int8_t sPl[MODE == MODE_M61 ? 2 : 1][8][MODE == MODE_M61 ? BM_PW : 16];
     __shared__ __attribute__((aligned(16))) int8_t sRPl[MODE == MODE_M61 ? 4 : 1][8][MODE == MODE_M61 ? BM_RP : 16];
-    uint64_t koff = 0;  // sum over classes s < 15 of 2^24 256^s mod p (the epilogue's offsets)
+    uint64_t koff = 0;  // sum over class triples k < 5 of 2^41 2^(24 k) mod p (the epilogue's offsets)
     if constexpr (MODE == MODE_M61) {
-        for (uint32_t q = 0; q < 15; q++) koff = add_mod(koff, canon_m61(rot_m61(1ull << 24, (8 * q) % 61)), P61);
+        for (uint32_t k = 0; k < 5; k++) koff = add_mod(koff, canon_m61(rot_m61(1ull << 41, (24 * k) % 61)), P61);
     }
 
     if (tid == 0) {  // c = 1, b = 1 (fp.rs:267-270); coefficients above l (of b: at or above bl) are never read

@@ -11,12 +11,13 @@ k_bm_batch_blk (modlin-rs_amd/csrc/bm_kernels.hip) forms both products of a bloc
     most 8 rounds, then the class sums (c = u+v within a lane's four rows and a wave's
     four digits of s) offset by 2^28, rotated by 8c bits mod p, summed, minus the offsets;
   * new C: sum_{a<64} U[a] C[j-a] + V[a] B[j-m0-a] as sum_s 256^s Q_s[j] over 15
-    classes, offset 2^24, two 64-bit partial sums of rotations.
+    classes, combined three at a time in int64 (|T| < 2^41), offset 2^41, five
+    rotations in one 64-bit sum.
 This file restates that arithmetic with numpy int64 (the values the int32 accumulators
 hold) and Python integers and compares it with direct modular sums, over random residues
 and residues with many extreme digits; it checks the bounds the kernels rely on
-(|T| <= 2^26 at a fold, class sums <= 2^28 and <= 2^24, no 64-bit overflow in the
-partial sums).  A planted mistake (one rotation off by 8 bits) must fail.  The GPU
+(|T| <= 2^26 at a fold, class sums <= 2^28 and <= 2^24, triples below 2^41, no 64-bit
+overflow in the partial sums).  A planted mistake (one rotation off by 8 bits) must fail.  The GPU
 tests compare the kernel itself with the oracle.
 """
 import random
@@ -50,9 +51,11 @@ def digits_peel(x):
 
 
 def rot(x, s):
-    """rot_m61 in bm_kernels.hip: x * 2^s mod p, not reduced, for x < 2^40."""
-    assert 0 <= x < (1 << 40)
-    return x if s == 0 else ((x << s) & P) + (x >> (61 - s))
+    """rot_m61 in bm_kernels.hip: x * 2^s mod p, not reduced, for x < 2^42 (result < 2^61 + 2^42)."""
+    assert 0 <= x < (1 << 42)
+    r = x if s == 0 else ((x << s) & P) + (x >> (61 - s))
+    assert r < (1 << 61) + (1 << 42) and r % P == (x << s) % P
+    return r
 
 
 def canon(x):
@@ -209,9 +212,10 @@ def test_esums_planted_mistake():
     assert _esums_mfma(C, B, s, 60, 50, 2, 200, shift_mistake=True) != _esums_direct(C, B, s, 60, 50, 2, 200)
 
 
-def _apply_mfma(C, B, U, V, l0, bl0, m0, n):
-    """bm_apply_mfma_m61's arithmetic for new C: 15 classes, offset 2^24, w0 over s < 7, w1 over s >= 7."""
-    koff = sum(canon(rot(1 << 24, (8 * q) % 61)) for q in range(15)) % P
+def _apply_mfma(C, B, U, V, l0, bl0, m0, n, triple_mistake=False):
+    """bm_apply_mfma_m61's arithmetic for new C: 15 classes combined three at a time,
+    T_k = Q_3k + 2^8 Q_3k+1 + 2^16 Q_3k+2, T_k + 2^41 rotated by 24 k bits, summed."""
+    koff = sum(canon(rot(1 << 41, (24 * k) % 61)) for k in range(5)) % P
     cw = [C[j] if 0 <= j <= l0 else 0 for j in range(n)]
     bw = [B[j] if 0 <= j < bl0 else 0 for j in range(n)]
     Cd, Bd = _planes(cw), _planes(bw)
@@ -225,10 +229,13 @@ def _apply_mfma(C, B, U, V, l0, bl0, m0, n):
         prod = xc @ Ud.T + xb @ Vd.T  # [u][v] = sum_a
         Q = [int(sum(prod[u, q - u] for u in range(8) if 0 <= q - u < 8)) for q in range(15)]
         assert max(abs(x) for x in Q) <= 1 << 24
-        w0 = sum(rot(Q[q] + (1 << 24), (8 * q) % 61) for q in range(7))
-        w1 = sum(rot(Q[q] + (1 << 24), (8 * q) % 61) for q in range(7, 15))
-        assert w0 < 1 << 64 and w1 < 1 << 64
-        out.append((canon((w0 & P) + (w0 >> 61)) + canon((w1 & P) + (w1 >> 61)) - koff) % P)
+        w = 0
+        for k in range(5):
+            t = Q[3 * k] + (Q[3 * k + 1] << (16 if triple_mistake else 8)) + (Q[3 * k + 2] << 16)
+            assert abs(t) < 1 << 41
+            w += rot(t + (1 << 41), (24 * k) % 61)
+        assert w < 1 << 64
+        out.append((canon((w & P) + (w >> 61)) - koff) % P)
     return out
 
 
@@ -241,3 +248,5 @@ def test_application_digit_classes():
         want = [sum(U[a] * (C[j - a] if 0 <= j - a <= l0 else 0) + V[a] * (B[j - m0 - a] if 0 <= j - m0 - a < bl0 else 0)
                     for a in range(64)) % P for j in range(n)]
         assert _apply_mfma(C, B, U, V, l0, bl0, m0, n) == want, k
+        if k == 0:  # a wrong weight inside the triples is caught
+            assert _apply_mfma(C, B, U, V, l0, bl0, m0, n, triple_mistake=True) != want
