@@ -1017,61 +1017,73 @@ __device__ void bm_apply_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
     }
 }
 
-constexpr uint32_t BM_RE = 512;        // coefficients per round of the matrix-core E sums
-constexpr uint32_t BM_SWE = BM_RE + 80;  // reversed s digit plane length of a round
+constexpr uint32_t BM_RE = 512;   // coefficients per round of the matrix-core E sums
+constexpr uint32_t BM_EKC = 9;    // K chunks of 64 per round: s positions k < 576 (511 + 16 lags)
+constexpr uint32_t BM_ESF = 624;  // forward s digit plane length of a round (48 + 576)
+constexpr uint32_t BM_ECR = 608;  // reversed C / D digit plane length (16 + 512 + zero tail)
 
 // The E sums of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.  With
 // D = x^m B (D[j] = B[j - m]) both sums read the same window of s:
 //   Ec(r) = sum_j C[j] s[i0+r-j],   Eb(r) = sum_j D[j] s[i0+r-j],
-// and with signed base-256 digits, Ec(r) == sum_{u,v} 256^(u+v) T_v[u][r] where
-// T_v[u][r] = sum_j C_u[j] s_v[i0+r-j] (rows 8 + u likewise for D).  One MFMA per
-// (digit v of s, K chunk of 64 coefficients): rows 0-7 the digit planes of C, rows
-// 8-15 those of D, columns 16 lags r of one of four lag tiles, K = j.  The A runs
-// are aligned 16-byte runs of the planes; the B operand is the Toeplitz
-// s_v[i0+r-j], per lane a 16-byte run of a reversed s digit plane at a lane-fixed
-// byte offset mod 4, put together from five aligned dwords with byte shifts.  A
-// round takes 512 coefficients, so |T| grows by at most 512 * 2^14 = 2^23 a round;
-// every 8 rounds (|T| <= 2^26) and at the end the T of equal u + v in a lane (four
-// at most, |sum| <= 2^28) go into a residue: plus 2^28, rotated by 8 (u + v) bits
-// mod p, summed, minus the offsets.  Wave w forms lag tile w & 3 over the digits v
-// of s in [4 (w >> 2), 4 (w >> 2) + 4); the two halves meet in LDS.  Rounds holding
-// no coefficient of C (j <= l) nor of D (m <= j < m + bl) are skipped, and the next
-// round's values are fetched from HBM during the current round's products.
+// and with signed base-256 digits Ec(r) == sum_{u,v} 256^(u+v) T_uv(r).  A round
+// takes the 512 coefficients j = j0 + jj; with x0 = i0 - j0 - 511 and the s position
+// k = 511 + c - jj (k < 576, nine K chunks of 64), a lag r = 16 g + c is
+//   T_uv(r) = sum_k S_v[x0 + 16 g + k] C_u[j0 + 511 + c - k].
+// One MFMA per (digit u of C or D, group of four digits v of s, K chunk): rows
+// 4 g + vv hold the Hankel rows S_(4 h + vv)[x0 + 16 g + k] of the forward s digit
+// planes (aligned 16-byte runs), columns c the Toeplitz C_u[j0 + 511 + c - k] of a
+// reversed, zero-padded digit plane of C (or D): per lane a 16-byte run at a
+// lane-fixed byte offset mod 4, put together from five aligned dwords with byte
+// shifts.  Output register i of lane (c, g) is lag 16 g + c, digit v = 4 h + i.
+// Wave w takes C (w < 4) or D, the digits u in [4 ((w >> 1) & 1), + 4), both groups
+// h, and the K chunks [0, 5) (w even) or [5, 9): each A run feeds four MFMAs and each
+// B run two.  |T| grows by at most 2^23 a round; every 8 rounds (|T| <= 2^26) and at
+// the end the T of equal u + v in a lane (four at most, |sum| <= 2^28) go into a
+// residue: plus 2^28, rotated by 8 (u + v) bits mod p, summed, minus the offsets; the
+// four waves of C (of D) meet in LDS.  Rounds holding no coefficient of C (j <= l)
+// nor of D (m <= j < m + bl) are skipped, and the next round's values are fetched
+// from HBM during the current round's products.
 __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uint32_t n, uint32_t i0, uint32_t l,
                                   uint32_t bl, uint32_t m, gptr<const uint64_t> s, gptr<const uint64_t> cp,
                                   gptr<const uint64_t> bp, int8_t *pl, uint64_t *sE) {
     typedef int v4i __attribute__((ext_vector_type(4)));
-    constexpr uint32_t RE = BM_RE, SWE = BM_SWE;
+    constexpr uint32_t RE = BM_RE, KC = BM_EKC, SF = BM_ESF, CR = BM_ECR;
     // the lane constants below are formed anew per call: hoisted out of the caller's
     // loop they would stay live through the application's products
     __asm__ volatile("" : "+v"(lane), "+v"(tid));
-    int8_t *AP = pl, *SP = pl + 16 * RE;  // AP[16][RE]: planes of C, then of D; SP[8][SWE]
-    const uint32_t rho = wid & 3, vh = wid >> 2, col = lane & 15, kq = lane >> 4;
-    const uint32_t zb = 63 - 16 * rho - col + 16 * kq, sh = zb & 3;
+    int8_t *CRp = pl, *SFp = pl + 16 * CR;  // CRp[16][CR]: planes of C, then of D; SFp[8][SF]
+    const uint32_t which = wid >> 2, uh = (wid >> 1) & 1, kh = wid & 1, c = lane & 15, g = lane >> 4;
+    const uint32_t kc0 = kh ? 5 : 0, kc1 = kh ? KC : 5;
+    // A: row lane & 15 = 4 g' + vv of the tile, K bytes 16 (lane >> 4) ..
+    const uint32_t arow = 16 * ((lane & 15) >> 2) + 16 * (lane >> 4), avv = lane & 3;
+    // B: column c, K bytes 16 (lane >> 4) ..: reversed plane index 16 + k - c
+    const uint32_t bz = 16 + 16 * (lane >> 4) - c, sh = bz & 3;
     const v4i zero = {0, 0, 0, 0};
-    // this lane's offsets per fold: 2^28 256^(4 (kq & 1) + 4 vh + c), c < 7
+    // this lane's offsets per fold: 2^28 256^(4 uh + cr) for its 11 classes cr
     uint64_t koff = 0;
-    for (uint32_t c = 0; c < 7; c++)
-        koff = add_mod(koff, canon_m61(rot_m61(1ull << 28, (8 * (4 * (kq & 1) + 4 * vh + c)) % 61)), P61);
+    for (uint32_t cr = 0; cr < 11; cr++) koff = add_mod(koff, canon_m61(rot_m61(1ull << 28, (8 * (4 * uh + cr)) % 61)), P61);
     uint64_t epart = 0;
-    v4i acc[4] = {zero, zero, zero, zero};
+    v4i acc[4][2];  // [digit u - 4 uh][group h of digits of s]
+#pragma unroll
+    for (int ul = 0; ul < 4; ul++) acc[ul][0] = acc[ul][1] = zero;
     uint32_t folds = 0;  // rounds in acc since the last fold (at most 8: |T| <= 2^26)
-    auto fold = [&]() {  // rows 4 kq + i (digit u = 4 (kq & 1) + i of C, or of D); class c = i + v
-        int32_t cl[7];
+    auto fold = [&]() {  // output register i: digit v = 4 h + i; class cr = ul + 4 h + i
+        int32_t cl[11];
 #pragma unroll
-        for (int c = 0; c < 7; c++) cl[c] = 0;
+        for (int cr = 0; cr < 11; cr++) cl[cr] = 0;
 #pragma unroll
-        for (int v = 0; v < 4; v++)
+        for (int ul = 0; ul < 4; ul++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) cl[i + v] += acc[v][i];
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) cl[ul + 4 * h + i] += acc[ul][h][i];
         uint64_t w = 0;
 #pragma unroll
-        for (uint32_t c = 0; c < 7; c++)
-            w = fold_m61_64(
-                w + rot_m61((uint64_t)(uint32_t)(cl[c] + (1 << 28)), (8 * (4 * (kq & 1) + 4 * vh + c)) % 61));
+        for (uint32_t cr = 0; cr < 11; cr++)
+            w = fold_m61_64(w + rot_m61((uint64_t)(uint32_t)(cl[cr] + (1 << 28)), (8 * (4 * uh + cr)) % 61));
         epart = add_mod(epart, sub_mod(canon_m61(w), koff, P61), P61);
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[v] = zero;
+        for (int ul = 0; ul < 4; ul++) acc[ul][0] = acc[ul][1] = zero;
     };
     const uint32_t top_e = max(l, m + bl - 1);
     // rounds holding no coefficient of C (j <= l) nor of D (m <= j < m + bl) are skipped
@@ -1079,11 +1091,12 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         while (j <= top_e && j > l && j + RE <= m) j += RE;
         return j;
     };
-    // this thread's eight values of a round: AP rows of C (tid < 64) or D (tid < 128)
-    // at j0 + k, or SP columns (tid < 202) of s[i0 - j0 + 63 - z]; fetched a round ahead
+    // this thread's eight values of a round: reversed C (tid < 64) or D (tid < 128)
+    // digits at z = 8 (tid & 63) .., value j = j0 + 511 - z; or forward s digits
+    // (tid < 206) at y = 8 (tid - 128) .., value s[x0 + y]; fetched a round ahead
     uint32_t st = tid;
     __asm__ volatile("" : "+v"(st));
-    const uint32_t sw = st < RE / 8 ? 0 : st < 2 * RE / 8 ? 1 : st < 2 * RE / 8 + SWE / 8 ? 2 : 3;
+    const uint32_t sw = st < RE / 8 ? 0 : st < 2 * RE / 8 ? 1 : st < 2 * RE / 8 + SF / 8 ? 2 : 3;
     const uint32_t sk = sw < 2 ? 8 * (st % (RE / 8)) : 8 * (st - 2 * RE / 8);
     auto fetch = [&](uint32_t j0, uint64_t raw[8]) {
 #pragma unroll
@@ -1091,58 +1104,65 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         if (sw == 0) {
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
-                const uint32_t j = j0 + sk + t;
+                const uint32_t j = j0 + RE - 1 - (sk + t);
                 if (j <= l) raw[t] = cp[j];
             }
         } else if (sw == 1) {
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
-                const uint32_t j = j0 + sk + t;
+                const uint32_t j = j0 + RE - 1 - (sk + t);
                 if (j >= m && j - m < bl) raw[t] = bp[j - m];
             }
         } else if (sw == 2) {
-            const int64_t x0 = (int64_t)i0 - (int64_t)j0 + 63 - (int64_t)sk;
+            const int64_t x0 = (int64_t)i0 - (int64_t)j0 - (int64_t)(RE - 1) + (int64_t)sk;
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) {
-                const int64_t x = x0 - (int64_t)t;
+                const int64_t x = x0 + (int64_t)t;
                 if (x >= 0 && x < (int64_t)n) raw[t] = s[(uint32_t)x];
             }
         }
     };
+    // the reversed planes' zero margins: bytes [0, 16) and [16 + RE, CR) of each
+    for (uint32_t k = tid; k < 16 * (CR - RE) / 4; k += BM_BLOCK_THREADS) {
+        const uint32_t pl_ = k / ((CR - RE) / 4), q = k - pl_ * ((CR - RE) / 4);
+        const uint32_t off = q < 4 ? 4 * q : RE + 4 * q;  // 16 front bytes, then the tail from 16 + RE
+        *(uint32_t *)(CRp + pl_ * CR + off) = 0;
+    }
     uint64_t raw[8];
     uint32_t j0 = next_round(0);
     fetch(j0, raw);
     while (j0 <= top_e) {
-        // digit planes: AP[u][k] of C[j0 + k] (zero above l), AP[8 + u][k] of D[j0 + k];
-        // SP[v][z] of s[i0 - j0 + 63 - z] (zero outside [0, n))
         if (sw < 3) {
             uint64_t d[8];
 #pragma unroll
             for (uint32_t t = 0; t < 8; t++) d[t] = digits8_m61(raw[t]);
             if (sw < 2)
-                store_planes8(AP + sw * 8 * RE, RE, sk, d);
+                store_planes8(CRp + sw * 8 * CR, CR, 16 + sk, d);
             else
-                store_planes8(SP, SWE, sk, d);
+                store_planes8(SFp, SF, sk, d);
         }
         __syncthreads();
         const uint32_t jn = next_round(j0 + RE);
         if (jn <= top_e) fetch(jn, raw);  // in flight during the products
+        const int8_t *cplanes = CRp + (which * 8 + 4 * uh) * CR;
 #pragma unroll 1
-        for (uint32_t kc = 0; kc < 8; kc++) {
+        for (uint32_t kc = kc0; kc < kc1; kc++) {
             __asm__ volatile("" ::: "memory");  // keeps the loads of one chunk from being hoisted over the others
-            v4i af;
-            __builtin_memcpy(&af, AP + col * RE + 64 * kc + 16 * kq, 16);
+            v4i a[2];
 #pragma unroll
-            for (uint32_t v = 0; v < 4; v++) {
-                // bytes zb + 64 kc .. + 16 of plane 4 vh + v: five aligned dwords, shifted
-                const uint32_t *q = (const uint32_t *)(SP + (4 * vh + v) * SWE + zb + 64 * kc - sh);
+            for (int h = 0; h < 2; h++) __builtin_memcpy(&a[h], SFp + (4 * h + avv) * SF + arow + 64 * kc, 16);
+#pragma unroll
+            for (uint32_t ul = 0; ul < 4; ul++) {
+                // bytes 16 + 64 kc + 16 (lane >> 4) - c .. + 16 of plane u: five aligned dwords
+                const uint32_t *q = (const uint32_t *)(cplanes + ul * CR + bz + 64 * kc - sh);
                 const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
                 v4i bf;
                 bf[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
                 bf[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
                 bf[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
                 bf[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
-                acc[v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[v], 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < 2; h++) acc[ul][h] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[h], bf, acc[ul][h], 0, 0, 0);
             }
         }
         if (++folds == 8) {
@@ -1153,13 +1173,14 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         j0 = jn;
     }
     fold();
-    // lanes kq and kq ^ 1 hold digits 0-3 and 4-7 of the same sum (Ec for kq < 2, Eb
-    // for kq >= 2); then the two digit halves of s, from the waves w and w ^ 4
-    epart = add_mod(epart, __shfl_xor(epart, 16, 64), P61);
+    // lane (c, g): lag 16 g + c of Ec (waves 0-3) or Eb (waves 4-7); the four waves of
+    // each meet in LDS
     uint64_t *red = (uint64_t *)pl;  // the planes are free after the last round's barrier
-    if ((kq & 1) == 0) red[vh * 2 * BM_BK + (kq >> 1) * BM_BK + 16 * rho + col] = epart;
+    red[(wid & 3) * 2 * BM_BK + which * BM_BK + 16 * g + c] = epart;
     __syncthreads();
-    if (tid < 2 * BM_BK) sE[tid] = add_mod(red[tid], red[2 * BM_BK + tid], P61);
+    if (tid < 2 * BM_BK)
+        sE[tid] = add_mod(add_mod(red[tid], red[2 * BM_BK + tid], P61),
+                          add_mod(red[4 * BM_BK + tid], red[6 * BM_BK + tid], P61), P61);
 }
 
 template <int MODE>

@@ -8,8 +8,9 @@ k_bm_batch_blk (modlin-rs_amd/csrc/bm_kernels.hip) forms both products of a bloc
   * E sums: Ec(r) = sum_j C[j] s[i0+r-j] and Eb(r) = sum_j D[j] s[i0+r-j] with
     D = x^m B, as sum_{u,v} 256^(u+v) T_v[u][r]; rounds of 512 coefficients (rounds
     holding no coefficient of C or D skipped), the int8 products accumulated over at
-    most 8 rounds, then the class sums (c = u+v within a lane's four rows and a wave's
-    four digits of s) offset by 2^28, rotated by 8c bits mod p, summed, minus the offsets;
+    most 8 rounds, then the class sums (c = u+v within a wave's four digits of C or D
+    and all eight digits of s) offset by 2^28, rotated by 8c bits mod p, summed, minus
+    the offsets;
   * new C: sum_{a<64} U[a] C[j-a] + V[a] B[j-m0-a] as sum_s 256^s Q_s[j] over 15
     classes, combined three at a time in int64 (|T| < 2^41), offset 2^41, five
     rotations in one 64-bit sum.
@@ -129,32 +130,28 @@ def _esums_mfma(C, B, s, l, bl, m, i0, shift_mistake=False):
     """bm_esums_mfma_m61's arithmetic: returns (Ec, Eb), each 64 residues."""
     RE, SWE, n = 512, 592, len(s)
     top = max(l, m + bl - 1)
-    koff = {}
-    for h in range(2):
-        for vh in range(2):
-            koff[h, vh] = sum(canon(rot(1 << 28, (8 * (4 * h + 4 * vh + c)) % 61)) for c in range(7)) % P
-    ep = np.zeros((2, 2, 2, 64), dtype=object)  # [which][h][vh][r]
-    for a in np.nditer(ep, flags=['refs_ok'], op_flags=['readwrite']):
-        a[...] = 0
+    # a wave holds the digits u in [4 uh, 4 uh + 4) of C (or D) against all eight digits
+    # of s; its lane's classes are cr = ul + 4 h + i (u = 4 uh + ul, v = 4 h + i)
+    koff = {uh: sum(canon(rot(1 << 28, (8 * (4 * uh + cr)) % 61)) for cr in range(11)) % P for uh in range(2)}
+    ep = [[[0] * 64 for _ in range(2)] for _ in range(2)]  # [which][uh][r]
     acc = np.zeros((8, 16, 64), dtype=np.int64)  # [v][row][r]
     rounds = 0
 
     def fold():
         assert np.abs(acc).max() <= 1 << 26
         for which in range(2):
-            for h in range(2):
-                for vh in range(2):
-                    for r in range(64):
-                        w = 0
-                        for c in range(7):
-                            cl = sum(int(acc[4 * vh + vl, 8 * which + 4 * h + (c - vl), r])
-                                     for vl in range(4) if 0 <= c - vl < 4)
-                            assert abs(cl) <= 1 << 28
-                            sh = (8 * (4 * h + 4 * vh + c) + (8 if shift_mistake and c == 3 else 0)) % 61
-                            w = (w + rot(cl + (1 << 28), sh))
-                            assert w < 1 << 64
-                            w = (w & P) + (w >> 61)
-                        ep[which, h, vh, r] = (ep[which, h, vh, r] + canon(w) - koff[h, vh]) % P
+            for uh in range(2):
+                for r in range(64):
+                    w = 0
+                    for cr in range(11):
+                        cl = sum(int(acc[4 * h + i, 8 * which + 4 * uh + ul, r])
+                                 for ul in range(4) for h in range(2) for i in range(4) if ul + 4 * h + i == cr)
+                        assert abs(cl) <= 1 << 28
+                        sh = (8 * (4 * uh + cr) + (8 if shift_mistake and cr == 3 else 0)) % 61
+                        w = w + rot(cl + (1 << 28), sh)
+                        assert w < 1 << 64
+                        w = (w & P) + (w >> 61)
+                    ep[which][uh][r] = (ep[which][uh][r] + canon(w) - koff[uh]) % P
         acc[...] = 0
 
     j0 = 0
@@ -177,8 +174,8 @@ def _esums_mfma(C, B, s, l, bl, m, i0, shift_mistake=False):
             rounds = 0
         j0 += RE
     fold()
-    return [sum(ep[0, h, vh, r] for h in range(2) for vh in range(2)) % P for r in range(64)], \
-           [sum(ep[1, h, vh, r] for h in range(2) for vh in range(2)) % P for r in range(64)]
+    return [sum(ep[0][uh][r] for uh in range(2)) % P for r in range(64)], \
+           [sum(ep[1][uh][r] for uh in range(2)) % P for r in range(64)]
 
 
 def _esums_direct(C, B, s, l, bl, m, i0):
