@@ -878,6 +878,23 @@ __device__ __forceinline__ void store_planes8(int8_t *dst, uint32_t stride, uint
         *(uint64_t *)(dst + u * stride + k) = (uint64_t)q[u >> 2][0][u & 3] | ((uint64_t)q[u >> 2][1][u & 3] << 32);
 }
 
+// Four values' digit words d[t] -> eight plane words of four bytes (byte t of word u
+// is byte u of d[t]; the 4x4 transposes of store_planes8), stored at
+// dst[u * stride + k .. + 4] (k and stride multiples of 4).
+__device__ __forceinline__ void store_planes4(int8_t *dst, uint32_t stride, uint32_t k, const uint64_t d[4]) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint32_t a = (uint32_t)(d[0] >> (32 * h)), b = (uint32_t)(d[1] >> (32 * h)),
+                       c = (uint32_t)(d[2] >> (32 * h)), e = (uint32_t)(d[3] >> (32 * h));
+        const uint32_t t0 = __byte_perm(a, b, 0x5140), t1 = __byte_perm(a, b, 0x7362);
+        const uint32_t t2 = __byte_perm(c, e, 0x5140), t3 = __byte_perm(c, e, 0x7362);
+        *(uint32_t *)(dst + (4 * h + 0) * stride + k) = __byte_perm(t0, t2, 0x5410);
+        *(uint32_t *)(dst + (4 * h + 1) * stride + k) = __byte_perm(t0, t2, 0x7632);
+        *(uint32_t *)(dst + (4 * h + 2) * stride + k) = __byte_perm(t1, t3, 0x5410);
+        *(uint32_t *)(dst + (4 * h + 3) * stride + k) = __byte_perm(t1, t3, 0x7632);
+    }
+}
+
 // The application of k_bm_batch_blk for p = 2^61 - 1 on the matrix cores.
 // With balanced 8-digit expansions W^ = sum_u 256^u W_u of the window values and
 // P^ = sum_v 256^v P_v of the polynomial coefficients,
@@ -1091,32 +1108,32 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         while (j <= top_e && j > l && j + RE <= m) j += RE;
         return j;
     };
-    // this thread's eight values of a round: reversed C (tid < 64) or D (tid < 128)
-    // digits at z = 8 (tid & 63) .., value j = j0 + 511 - z; or forward s digits
-    // (tid < 206) at y = 8 (tid - 128) .., value s[x0 + y]; fetched a round ahead
+    // this thread's four values of a round: reversed C (tid < 128) or D (tid < 256)
+    // digits at z = 4 (tid & 127) .., value j = j0 + 511 - z; or forward s digits
+    // (tid < 412) at y = 4 (tid - 256) .., value s[x0 + y]; fetched a round ahead
     uint32_t st = tid;
     __asm__ volatile("" : "+v"(st));
-    const uint32_t sw = st < RE / 8 ? 0 : st < 2 * RE / 8 ? 1 : st < 2 * RE / 8 + SF / 8 ? 2 : 3;
-    const uint32_t sk = sw < 2 ? 8 * (st % (RE / 8)) : 8 * (st - 2 * RE / 8);
-    auto fetch = [&](uint32_t j0, uint64_t raw[8]) {
+    const uint32_t sw = st < RE / 4 ? 0 : st < 2 * RE / 4 ? 1 : st < 2 * RE / 4 + SF / 4 ? 2 : 3;
+    const uint32_t sk = sw < 2 ? 4 * (st % (RE / 4)) : 4 * (st - 2 * RE / 4);
+    auto fetch = [&](uint32_t j0, uint64_t raw[4]) {
 #pragma unroll
-        for (uint32_t t = 0; t < 8; t++) raw[t] = 0;
+        for (uint32_t t = 0; t < 4; t++) raw[t] = 0;
         if (sw == 0) {
 #pragma unroll
-            for (uint32_t t = 0; t < 8; t++) {
+            for (uint32_t t = 0; t < 4; t++) {
                 const uint32_t j = j0 + RE - 1 - (sk + t);
                 if (j <= l) raw[t] = cp[j];
             }
         } else if (sw == 1) {
 #pragma unroll
-            for (uint32_t t = 0; t < 8; t++) {
+            for (uint32_t t = 0; t < 4; t++) {
                 const uint32_t j = j0 + RE - 1 - (sk + t);
                 if (j >= m && j - m < bl) raw[t] = bp[j - m];
             }
         } else if (sw == 2) {
             const int64_t x0 = (int64_t)i0 - (int64_t)j0 - (int64_t)(RE - 1) + (int64_t)sk;
 #pragma unroll
-            for (uint32_t t = 0; t < 8; t++) {
+            for (uint32_t t = 0; t < 4; t++) {
                 const int64_t x = x0 + (int64_t)t;
                 if (x >= 0 && x < (int64_t)n) raw[t] = s[(uint32_t)x];
             }
@@ -1128,18 +1145,18 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         const uint32_t off = q < 4 ? 4 * q : RE + 4 * q;  // 16 front bytes, then the tail from 16 + RE
         *(uint32_t *)(CRp + pl_ * CR + off) = 0;
     }
-    uint64_t raw[8];
+    uint64_t raw[4];
     uint32_t j0 = next_round(0);
     fetch(j0, raw);
     while (j0 <= top_e) {
         if (sw < 3) {
-            uint64_t d[8];
+            uint64_t d[4];
 #pragma unroll
-            for (uint32_t t = 0; t < 8; t++) d[t] = digits8_m61(raw[t]);
+            for (uint32_t t = 0; t < 4; t++) d[t] = digits8_m61(raw[t]);
             if (sw < 2)
-                store_planes8(CRp + sw * 8 * CR, CR, 16 + sk, d);
+                store_planes4(CRp + sw * 8 * CR, CR, 16 + sk, d);
             else
-                store_planes8(SFp, SF, sk, d);
+                store_planes4(SFp, SF, sk, d);
         }
         __syncthreads();
         const uint32_t jn = next_round(j0 + RE);

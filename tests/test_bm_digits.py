@@ -4,7 +4,7 @@ k_bm_batch_blk (modlin-rs_amd/csrc/bm_kernels.hip) forms both products of a bloc
 64 steps as int8 MFMAs:
   * digits: the balanced representative of x mod p as eight signed base-256 digits, read
     off as (bal(x) + c) xor c with c = 0x80 in every byte, and moved into digit planes
-    by an 8x8 byte transpose of __byte_perm selectors (store_planes8);
+    by byte transposes of __byte_perm selectors (store_planes8, store_planes4);
   * E sums: Ec(r) = sum_j C[j] s[i0+r-j] and Eb(r) = sum_j D[j] s[i0+r-j] with
     D = x^m B, as sum_{u,v} 256^(u+v) T_v[u][r]; rounds of 512 coefficients (rounds
     holding no coefficient of C or D skipped), the int8 products accumulated over at
@@ -87,6 +87,19 @@ def store_planes8(d, sel=SELECTORS):
     return [q[u >> 2][0][u & 3] | (q[u >> 2][1][u & 3] << 32) for u in range(8)]
 
 
+def store_planes4(d, sel=SELECTORS):
+    """store_planes4: four digit words -> eight 4-byte plane words (byte t of word u is byte u of d[t])."""
+    lo_pair, hi_pair, lo_half, hi_half = sel
+    out = [0] * 8
+    for h in range(2):
+        a, b, c, e = ((d[k] >> (32 * h)) & 0xFFFFFFFF for k in range(4))
+        t0, t1 = byte_perm(a, b, lo_pair), byte_perm(a, b, hi_pair)
+        t2, t3 = byte_perm(c, e, lo_pair), byte_perm(c, e, hi_pair)
+        out[4 * h:4 * h + 4] = [byte_perm(t0, t2, lo_half), byte_perm(t0, t2, hi_half),
+                                byte_perm(t1, t3, lo_half), byte_perm(t1, t3, hi_half)]
+    return out
+
+
 def _is_transpose(words, planes):
     return all((planes[u] >> (8 * t)) & 0xFF == (words[t] >> (8 * u)) & 0xFF for u in range(8) for t in range(8))
 
@@ -118,6 +131,10 @@ def test_digits_and_transpose():
     for _ in range(200):
         words = [rng.getrandbits(64) for _ in range(8)]
         assert _is_transpose(words, store_planes8(words))
+    for _ in range(200):
+        words = [rng.getrandbits(64) for _ in range(4)]
+        planes = store_planes4(words)
+        assert all((planes[u] >> (8 * k)) & 0xFF == (words[k] >> (8 * u)) & 0xFF for u in range(8) for k in range(4))
     # a selector with two nibbles swapped breaks the transpose
     words = [rng.getrandbits(64) for _ in range(8)]
     for k, typo in enumerate((0x1540, 0x7326, 0x4510, 0x7623)):
