@@ -283,8 +283,10 @@ struct MacSum {
 MDL_HD uint64_t lin2_m61_lazy(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
     const uint32_t x0 = (uint32_t)x & 0x7fffffffu, x1 = (uint32_t)(x >> 31);
     const uint32_t y0 = (uint32_t)y & 0x7fffffffu, y1 = (uint32_t)(y >> 31);
-    const uint64_t s = (uint64_t)x0 * b0 + (uint64_t)x1 * (2u * b1) + (uint64_t)y0 * d0 + (uint64_t)y1 * (2u * d1);
-    const uint64_t t = (uint64_t)x0 * b1 + (uint64_t)x1 * b0 + (uint64_t)y0 * d1 + (uint64_t)y1 * d0;
+    // two independent pairs of multiply-adds per sum (not one chain of four): the
+    // dependent depth is two products and an add
+    const uint64_t s = ((uint64_t)x0 * b0 + (uint64_t)x1 * (2u * b1)) + ((uint64_t)y0 * d0 + (uint64_t)y1 * (2u * d1));
+    const uint64_t t = ((uint64_t)x0 * b1 + (uint64_t)x1 * b0) + ((uint64_t)y0 * d1 + (uint64_t)y1 * d0);
     const uint64_t r = s + (t >> 30) + ((t & 0x3fffffffull) << 31);  // < 2^64
     return (r & P61) + (r >> 61);                                      // <= p + 7
 }
