@@ -561,7 +561,7 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
         for (uint32_t t = 0; t < Kp; t++) {
             const uint32_t i = i0 + t;
             uint64_t d = lane_bcast64(e, t);
-            if (MODE == MODE_M61 && d >= P61) d -= P61;
+            if (MODE == MODE_M61 && ((d + 1) >> 61) != 0) d -= P61;  // d >= p for d < 2^62, on the scalar unit
             uint64_t fl = 0;
             if (d == 0) {  // fp.rs:279-282
                 eb = shift_up1(eb, lane);
