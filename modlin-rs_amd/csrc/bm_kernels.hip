@@ -1162,9 +1162,12 @@ __device__ void bm_esums_mfma_m61(uint32_t tid, uint32_t lane, uint32_t wid, uin
         const uint32_t jn = next_round(j0 + RE);
         if (jn <= top_e) fetch(jn, raw);  // in flight during the products
         const int8_t *cplanes = CRp + (which * 8 + 4 * uh) * CR;
-#pragma unroll 1
-        for (uint32_t kc = kc0; kc < kc1; kc++) {
-            __asm__ volatile("" ::: "memory");  // keeps the loads of one chunk from being hoisted over the others
+        // at most five chunks per wave: unrolled, so the loads of later chunks can
+        // be in flight during earlier products
+#pragma unroll
+        for (uint32_t kk = 0; kk < 5; kk++) {
+            const uint32_t kc = kc0 + kk;
+            if (kc >= kc1) break;  // uniform
             v4i a[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) __builtin_memcpy(&a[h], SFp + (4 * h + avv) * SF + arow + 64 * kc, 16);
