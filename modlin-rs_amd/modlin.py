@@ -327,13 +327,44 @@ class DeviceBuffer:
         self.ctx, self.ptr, self.n, self.elem, self.nbytes = ctx, ptr, n, elem, nbytes
 
     def to_host(self) -> np.ndarray:
-        a = np.empty(self.nbytes // (4 if self.elem == ELEM_U32 else 8),
-                     dtype=np.uint32 if self.elem == ELEM_U32 else np.uint64)
-        self.ctx._check(self.ctx._lib.modlin_memcpy_d2h(self.ctx._h, a.ctypes.data, self.ptr, self.nbytes),
-                        "memcpy_d2h")
-        return a
+        return self.read(0, self.nbytes // self.itemsize)
+
+    @property
+    def itemsize(self) -> int:
+        return 4 if self.elem == ELEM_U32 else 8
+
+    def read(self, offset: int, count: int) -> np.ndarray:
+        """Elements offset .. offset+count-1, copied to the host."""
+        if offset < 0 or count < 0 or (offset + count) * self.itemsize > self.nbytes:
+            raise ValueError("read outside the buffer")
+        a = np.empty(max(1, count), dtype=np.uint32 if self.elem == ELEM_U32 else np.uint64)
+        self.ctx._check(self.ctx._lib.modlin_memcpy_d2h(self.ctx._h, a.ctypes.data, self.ptr + offset * self.itemsize,
+                                                        count * self.itemsize), "memcpy_d2h")
+        return a[:count]
+
+    def write(self, offset: int, values: np.ndarray) -> None:
+        """Stores values (this buffer's element type) at element offset."""
+        a = np.ascontiguousarray(values, dtype=np.uint32 if self.elem == ELEM_U32 else np.uint64).ravel()
+        if offset < 0 or (offset + a.size) * self.itemsize > self.nbytes:
+            raise ValueError("write outside the buffer")
+        self.ctx._check(self.ctx._lib.modlin_memcpy_h2d(self.ctx._h, self.ptr + offset * self.itemsize, a.ctypes.data,
+                                                        a.nbytes), "memcpy_h2d")
+
+    def view(self, offset: int, n: int) -> "DeviceBuffer":
+        """A non-owning descriptor of the n x n matrix (or length-n sequence, for
+        square=False callers) stored at element offset."""
+        size = n * n
+        if offset < 0 or (offset + size) * self.itemsize > self.nbytes:
+            raise ValueError("view outside the buffer")
+        v = DeviceBuffer(self.ctx, self.ptr + offset * self.itemsize, n, self.elem, size * self.itemsize)
+        v.owner = False
+        return v
+
+    owner = True
 
     def free(self):
+        if not self.owner:
+            return
         if self.ptr:
             self.ctx._lib.modlin_dev_free(self.ctx._h, self.ptr)
             self.ptr = 0

@@ -78,6 +78,32 @@ def c3(n: int, p: int) -> np.ndarray:
     return m
 
 
+def c3_device(ctx: "M.Context", n: int, p: int) -> "M.DeviceBuffer":
+    """The c3 matrix built in device memory (the same entries as c3(n, p)): the
+    whole n*n prefix of the xoshiro256** seed-11 stream is generated on the
+    device, then the last 64 rows -- whose stream positions hold the
+    coefficients -- are replaced by the combinations of the first 128 rows."""
+    indep = n - 64
+    buf = ctx.prng_fill_device(M.PRNG_XOSHIRO256SS, 11, n * n, field=M.Field(p))
+    coef = buf.read(indep * n, 64 * 128).reshape(64, 128)
+    top = buf.read(0, 128 * n).reshape(128, n)
+    buf.write(indep * n, M.combine_rows(p, top, coef))
+    return buf
+
+
+def mixmax256_matrix_device(ctx: "M.Context", n: int, p: int = M61, seed: int = 1) -> "M.DeviceBuffer":
+    """n x n matrix of consecutive MIXMAX-256 (skip 2) outputs reduced mod p,
+    row-major, built in device memory (main.rs:94-99 fill order).
+
+    Rank bounds.  Every row is a length-n window of one sequence of linear
+    complexity 65280 (README.md:107), and such windows span at most 65280
+    dimensions, so at n = 65536 the corank is at least 256.  Each output block
+    of the generator holds the 255 entries v[1..256) of its 256-word state, so
+    when 255 divides n every row starts a block and rank <= 256 (at 510, 765,
+    1020 the oracle gives exactly 256)."""
+    return ctx.prng_fill_device(M.PRNG_MIXMAX256, seed, n * n, field=M.Field(p))
+
+
 def c4_specs(count: int = 1024) -> list[tuple[int, int, int]]:
     """(kind, n, L) per sequence; kind 0 uniform, 1 recurrence of order L, 2 MIXMAX-17."""
     x = M.prng_fill(M.PRNG_SPLITMIX64, 3, 2 * count)
@@ -118,3 +144,31 @@ def c5_bm(n: int = 200_000) -> np.ndarray:
 def c5_rank(count: int = 256, n: int = 2048, p: int = 65521) -> list[np.ndarray]:
     d = M.prng_fill(M.PRNG_XOSHIRO256SS, 5, count * n * n)
     return [(d[i * n * n:(i + 1) * n * n] % np.uint64(p)).astype(np.uint32).reshape(n, n) for i in range(count)]
+
+
+C5_PLANTED_DEFICIENCY = (1, 2, 5, 17, 64, 200, 511, 1000)
+
+
+def c5_planted(n: int = 2048, p: int = 65521, seed: int = 55) -> list[np.ndarray]:
+    """Rank-deficient members for the small-prime u32 batch: matrix k has its last
+    d_k rows (d_k from C5_PLANTED_DEFICIENCY) replaced by random F_p combinations
+    of its first 32 rows, so its corank is d_k when its first n - d_k rows are
+    independent (w.h.p.; the oracle confirms it for these seeds); entries from
+    xoshiro256** seed + k, reduced mod p, stored as u32."""
+    mats = []
+    for k, d in enumerate(C5_PLANTED_DEFICIENCY):
+        src = 32
+        draws = M.prng_fill(M.PRNG_XOSHIRO256SS, seed + k, n * n + d * src)
+        m = _mod(draws[:n * n], p).reshape(n, n)
+        coef = _mod(draws[n * n:], p).reshape(d, src)
+        m[n - d:] = M.combine_rows(p, m[:src], coef)
+        mats.append(m.astype(np.uint32))
+    return mats
+
+
+def c5_rank_device(ctx: "M.Context", count: int = 256, n: int = 2048, p: int = 65521) -> "tuple[M.DeviceBuffer, list]":
+    """The c5 rank batch generated in device memory as u32: one buffer holding the
+    count matrices back to back (the same entries as c5_rank), and a view of each."""
+    buf = ctx.prng_fill_device(M.PRNG_XOSHIRO256SS, 5, count * n * n, field=M.Field(p), elem=M.ELEM_U32,
+                               square=False)
+    return buf, [buf.view(i * n * n, n) for i in range(count)]

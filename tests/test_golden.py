@@ -66,3 +66,21 @@ def test_c5_fixtures():
     assert rk["p"] == 65521 and rk["n"] == 2048 and len(rk["coranks"]) == 256
     mats = workloads.c5_rank(count=2)
     assert [_digest(m) for m in mats] == rk["sha256_16"][:2]
+
+
+def test_c5_planted_members():
+    """The planted-deficiency u32 members over 65521 have exactly the coranks the
+    GPU batch test asserts (the oracle on every one, 2048 x 2048)."""
+    mats = workloads.c5_planted()
+    assert len(mats) == len(workloads.C5_PLANTED_DEFICIENCY)
+    for d, m in zip(workloads.C5_PLANTED_DEFICIENCY, mats):
+        assert m.dtype == np.uint32 and m.shape == (2048, 2048)
+        assert 2048 - oracle.rank(65521, m.astype(np.uint64), 2048) == d
+
+
+def test_mixmax256_block_rank():
+    """Rows that all start an output block of MIXMAX-256 (255 divides n) have rank
+    at most the state dimension 256; rank exactly 256 once there are 256 rows."""
+    for n in (255, 510):
+        a = (workloads.M.prng_fill(workloads.M.PRNG_MIXMAX256, 1, n * n) % np.uint64(M61)).astype(np.uint64)
+        assert oracle.rank(M61, a, n) == min(n, 256)

@@ -184,3 +184,31 @@ def test_blocked_batch_equals_plain(ctx, p):
     want = [oracle.lc(p, s) for s in seqs]
     assert ctx.lc_batch(f, seqs) == want
     assert ctx.lc_batch(f, seqs, method=modlin.METHOD_PLAIN) == want
+
+
+def test_blocked_batch_high_order_m61(ctx):
+    """The batched blocked kernel on the matrix cores (2^61-1) at large l: dense
+    recurrences of order about 1500 and 4600 (many coefficient rounds of the E sums
+    and of the new C and B, with the E sums' periodic folds), and one recurrence
+    whose coefficients and initial values are all p-1, (p-1)/2 or (p+1)/2 (the
+    balanced-digit edges).  Their linear complexity is exactly the order (a
+    nonzero last coefficient and a full-rank Hankel start, w.h.p.)."""
+    p = M61
+    rng = np.random.default_rng(1500)
+    f = modlin.Field(p)
+    seqs, orders = [], []
+    for L in (1500, 4600):
+        coef = rng.integers(1, p, size=L, dtype=np.uint64)
+        init = rng.integers(0, p, size=L, dtype=np.uint64)
+        seqs.append(modlin.lfsr(p, coef, init, 2 * L + 300))
+        orders.append(L)
+    edge = np.array([p - 1, (p - 1) // 2, (p + 1) // 2], dtype=np.uint64)
+    L = 1200
+    seqs.append(modlin.lfsr(p, edge[rng.integers(0, 3, size=L)], edge[rng.integers(0, 3, size=L)], 2 * L + 300))
+    orders.append(L)
+    print("blocked BM high order", orders, flush=True)
+    want = [oracle.lc(p, s) for s in seqs]
+    assert want == orders
+    assert ctx.lc_batch(f, seqs) == want
+    assert ctx.stats()["bm_coop"] == 0
+    assert ctx.lc_batch(f, seqs, method=modlin.METHOD_PLAIN) == want
