@@ -56,8 +56,14 @@ typedef struct modlin_field {
 
 typedef struct modlin_ctx modlin_ctx_t;
 
-/* Progress callback: done / total units (columns for rank, steps for BM,
- * items for batches).  NULL disables it. */
+/* Progress callback, called on the calling thread while a call runs (NULL
+ * disables it).  modlin_rank: done = matrix columns below the current panel
+ * (the reference's update_with_count per panel, src/fp.rs:251), at every host
+ * check of the iteration loop (every check_every panels), total = n; done never
+ * decreases and the last call has done = total.  modlin_linear_complexity:
+ * done = steps the kernel has passed (the reference's update() per step,
+ * src/fp.rs:273, reported every 64 to 256 steps), total = n, last call
+ * done = total. */
 typedef void (*modlin_progress_fn)(void *user, uint64_t done, uint64_t total);
 
 enum modlin_method {
@@ -129,6 +135,8 @@ typedef struct modlin_tuning {
 } modlin_tuning_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */
+/* modlin_reduce and modlin_reduce_u128 return UINT64_MAX for an invalid field
+ * (NULL, or p outside [2, 2^63)): no residue equals it, since residues are < p. */
 int modlin_field_init(uint64_t p, modlin_field_t *out);
 uint64_t modlin_reduce(const modlin_field_t *f, uint64_t x);
 uint64_t modlin_reduce_u128(const modlin_field_t *f, uint64_t hi, uint64_t lo);

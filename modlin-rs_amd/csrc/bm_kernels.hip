@@ -36,6 +36,8 @@
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 
 #include "engine.hpp"
 #include "rank_common.hpp"
@@ -48,8 +50,17 @@ struct BmSeq {
     uint64_t *u;  // fourth state buffer (the blocked kernels double-buffer c and b)
     uint32_t n;
     uint32_t lc;
+    uint32_t timeout;  // k_bm_batch_blk: a follower's wait for a step record ran out
+    uint32_t pad;
+    uint64_t *prog;    // steps done, for the host's progress callback (pinned host memory), or null
     unsigned long long bytes;  // algorithmic bytes: sum_i 8 + 16 l_i + [d_i != 0] 24 jmax_i + [change] 16 (old_l + 1)
 };
+
+// Steps done so far, for the host's progress callback: a system-scope store to
+// pinned host memory that the host reads while the kernel runs.
+__device__ __forceinline__ void bm_progress(uint64_t *prog, uint64_t steps) {
+    if (prog) __hip_atomic_store(prog, steps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 template <int NT>
 __device__ __forceinline__ uint64_t block_sum_mod(uint64_t v, uint64_t p, uint64_t *red) {
@@ -94,6 +105,7 @@ __global__ __launch_bounds__(NT) void k_bm_batch(BmSeq *seqs, FieldParams f) {
     MacSum<MODE> nxt;    // this thread's part of the next discrepancy, formed by an update
     bool fused = false;  // nxt holds it (the previous step updated c)
     for (uint32_t i = 0; i < n; i++) {
+        if ((i & 255) == 0 && tid == 0) bm_progress(Q.prog, i);
         uint32_t j = tid;
         uint64_t part;
         if (fused) {
@@ -281,6 +293,7 @@ __global__ __launch_bounds__(BM_COOP_THREADS) void k_bm_coop(BmSeq *seq, FieldPa
     bool fresh_b = false;  // b was rewritten at the previous step (no prefetch)
     unsigned long long bytes = 0;
     for (uint32_t i = 0; i < n; i++) {
+        if ((i & 255) == 0 && blockIdx.x == 0 && tid == 0) bm_progress(seq->prog, i);
         const uint32_t jmax = min(bl, n - m);
         const uint32_t a0 = max(lo, m), a1 = min(hi, m + jmax);
         // this thread's indices are j = lo + tid + q NT; the axpy ones (j in [a0, a1))
@@ -487,13 +500,20 @@ __device__ __forceinline__ bool poll_tag(gptr<const uint64_t> a, uint64_t tag, u
 // steps, says how many records are there; followers wait for record t with
 // acquire loads.  A follower holds the block's records one per lane in rv, reloaded
 // (one LDS read for the wave) each time the count moves past what it holds, and
-// takes record t from lane t: no LDS round trip on a step.
+// takes record t from lane t: no LDS round trip on a step.  A wait that runs out
+// sets *fail (LDS); the replay then finishes its bounded loop on records that
+// are not final, and the kernel stops after it and reports the failure.
 __device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *prog, uint32_t t, uint32_t lane,
-                                              uint32_t &avail, uint64_t &rv) {
+                                              uint32_t &avail, uint64_t &rv, uint32_t *fail) {
     if (t >= avail) {
-        for (uint32_t spins = 0; t >= avail && spins < BM_BLOCK_SPIN_LIMIT; spins++) {
+        uint32_t spins = 0;
+        for (; t >= avail && spins < BM_BLOCK_SPIN_LIMIT; spins++) {
             avail = uni32(__hip_atomic_load(prog, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
             if (t >= avail) __builtin_amdgcn_s_sleep(0);
+        }
+        if (t >= avail) {
+            *fail = 1;
+            avail = t + 1;  // no further waits in this block
         }
         rv = rec[lane];  // records below avail are final; the others are not read
     }
@@ -507,11 +527,11 @@ __device__ __forceinline__ uint64_t bm_record(const uint64_t *rec, uint32_t *pro
 template <int MODE>
 __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp, uint32_t lane, uint64_t bb,
                                                uint64_t P, uint64_t Pp, uint32_t *prog, const uint64_t *rec,
-                                               uint64_t *outP, uint64_t *outPb) {
+                                               uint64_t *outP, uint64_t *outPb, uint32_t *fail) {
     uint64_t Pb = 0, rv = 0;
     uint32_t avail = 0;
     for (uint32_t t = 0; t < Kp; t++) {
-        const uint64_t r = bm_record(rec, prog, t, lane, avail, rv);
+        const uint64_t r = bm_record(rec, prog, t, lane, avail, rv, fail);
         const uint32_t fl = (uint32_t)(r >> 62);
         if (fl == 0) {
             Pp = shift_up1(Pp, lane);
@@ -543,12 +563,13 @@ __device__ __forceinline__ void bm_replay_poly(const FieldParams &f, uint32_t Kp
 // with the rest of the reference's scalar state (m, bl) and the byte count.  The
 // block-start state (l, bl, m, bb) is the same in every thread; the new state goes
 // to sState: l, bl, m, bb, whether B changed, the byte count, whether c changed.
+// A follower whose wait for a record ran out sets *sFail.
 template <int MODE>
 __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n, uint32_t i0, uint32_t Kp,
                                                 uint32_t lane, uint32_t wid, uint32_t l, uint32_t bl, uint32_t m,
                                                 uint64_t bb, unsigned long long &bytes, const uint64_t *sE,
                                                 uint64_t (*sPoly)[BM_BK], uint64_t *sRec, uint64_t *sJunk,
-                                                uint32_t *sProg_, uint64_t *sState) {
+                                                uint32_t *sProg_, uint64_t *sState, uint32_t *sFail) {
     constexpr uint32_t K = BM_BK;
     const uint64_t p = f.p;
     uint32_t &sProg = *sProg_;
@@ -591,14 +612,14 @@ __device__ __forceinline__ void bm_replay_block(const FieldParams &f, uint32_t n
         }
         if (lane == 0) sState[3] = bb;
     } else if (wid == 1) {
-        bm_replay_poly<MODE>(f, Kp, lane, bb, lane == 0 ? 1 % p : 0, 0, &sProg, sRec, sPoly[0], sPoly[2]);
+        bm_replay_poly<MODE>(f, Kp, lane, bb, lane == 0 ? 1 % p : 0, 0, &sProg, sRec, sPoly[0], sPoly[2], sFail);
     } else if (wid == 2) {
-        bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3]);
+        bm_replay_poly<MODE>(f, Kp, lane, bb, 0, lane == 0 ? 1 % p : 0, &sProg, sRec, sPoly[1], sPoly[3], sFail);
     } else if (wid == 3) {
         uint32_t avail = 0, upd = 0, bch = 0;
         uint64_t rv = 0;
         for (uint32_t t = 0; t < Kp; t++) {
-            const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, lane, avail, rv) >> 62);
+            const uint32_t fl = (uint32_t)(bm_record(sRec, &sProg, t, lane, avail, rv, sFail) >> 62);
             const uint32_t i = i0 + t;
             bytes += 8ull + 16ull * l;
             if (fl == 0) {
@@ -650,13 +671,13 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
     __shared__ uint64_t sRec[K];
     __shared__ uint64_t sJunk[2 * 64];
     __shared__ uint64_t sState[7];
-    __shared__ uint32_t sAbort, sProg;
+    __shared__ uint32_t sAbort, sProg, sFail;
 
     for (uint32_t k = tid; k < QM * W; k += NT) {
         sCo[k] = 0;
         sBo[k] = 0;
     }
-    if (tid == 0) sAbort = 0;
+    if (tid == 0) sAbort = sFail = 0;
     __syncthreads();
     if (g == 0 && tid == 0) {  // c = 1, b = 1 (fp.rs:267-270)
         sCo[0] = 1 % p;
@@ -669,6 +690,7 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
     unsigned long long bytes = 0;
     for (uint32_t i0 = 0, blk = 0; i0 < n; i0 += K, blk++) {
         const uint32_t Kp = min(K, n - i0);
+        if (g == 0 && tid == 0) bm_progress(Q.prog, i0);
         const uint32_t par = blk & 1;
         const uint64_t tag = (uint64_t)(((blk >> 1) + 1) & 7) << 61;
         // ---- 1. this workgroup's parts of Ec(r) and Eb(r)
@@ -739,8 +761,13 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_block(BmSeq *seq, Field
             break;
         // ---- 3. the K steps (wavefronts 0-3)
         const uint32_t l0 = l, bl0 = bl, m0 = m;
-        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState);
+        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState,
+                              &sFail);
         __syncthreads();
+        if (uni32(sFail) != 0) {
+            if (tid == 0) __hip_atomic_store(&x->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
         l = uni32((uint32_t)sState[0]);
         bl = uni32((uint32_t)sState[1]);
         m = uni32((uint32_t)sState[2]);
@@ -1219,7 +1246,7 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
     __shared__ uint64_t sRec[K];
     __shared__ uint64_t sJunk[2 * 64];
     __shared__ uint64_t sState[7];
-    __shared__ uint32_t sProg;
+    __shared__ uint32_t sProg, sFail;
     // matrix-core application (2^61 - 1): digit planes of the windows and of U|V|Ub|Vb
     __shared__ __attribute__((aligned(16))) int8_t sPl[MODE == MODE_M61 ? 2 : 1][8][MODE == MODE_M61 ? BM_PW : 16];
     __shared__ __attribute__((aligned(16))) int8_t sRPl[MODE == MODE_M61 ? 4 : 1][8][MODE == MODE_M61 ? BM_RP : 16];
@@ -1231,6 +1258,7 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
     if (tid == 0) {  // c = 1, b = 1 (fp.rs:267-270); coefficients above l (of b: at or above bl) are never read
         Cg[0][0] = 1 % p;
         Bg[0][0] = 1 % p;
+        sFail = 0;
     }
     __syncthreads();
     uint32_t l = 0, bl = 1, m = 1, cc = 0, bc = 0;
@@ -1238,6 +1266,7 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
     unsigned long long bytes = 0;
     for (uint32_t i0 = 0; i0 < n; i0 += K) {
         const uint32_t Kp = min(K, n - i0);
+        if (tid == 0) bm_progress(Q.prog, i0);
         // ---- E sums: thread (v, part), v = value (Ec(r), then Eb(r)), part = a
         // quarter of each round's coefficients; the window index of coefficient o
         // of the round for lag r is r - o + R - 1.
@@ -1279,8 +1308,13 @@ __global__ __launch_bounds__(BM_BLOCK_THREADS) void k_bm_batch_blk(BmSeq *seqs, 
         __syncthreads();
         // ---- the K steps (wavefronts 0-3)
         const uint32_t l0 = l, bl0 = bl, m0 = m;
-        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState);
+        bm_replay_block<MODE>(f, n, i0, Kp, lane, wid, l, bl, m, bb, bytes, sE, sPoly, sRec, sJunk, &sProg, sState,
+                              &sFail);
         __syncthreads();
+        if (uni32(sFail) != 0) {  // a record wait ran out: stop, the host reports it
+            if (tid == 0) Q.timeout = 1;
+            return;
+        }
         l = uni32((uint32_t)sState[0]);
         bl = uni32((uint32_t)sState[1]);
         m = uni32((uint32_t)sState[2]);
@@ -1337,10 +1371,43 @@ __global__ void k_widen_seq(const uint32_t *src, uint64_t *dst, size_t count) {
         dst[i] = src[i];
 }
 
+// Waits for the context's stream while reporting the steps a running kernel
+// has stored to the pinned progress word (monotone, at most `total`).
+static int wait_progress(modlin_ctx *ctx, uint64_t total, modlin_progress_fn progress, void *user) {
+    const volatile uint64_t *w = ctx->prog_host;
+    uint64_t last = 0;
+    for (;;) {
+        const hipError_t e = hipStreamQuery(ctx->stream);
+        if (e == hipSuccess) return MODLIN_OK;
+        if (e != hipErrorNotReady) return hip_fail(ctx, e, "hipStreamQuery");
+        const uint64_t v = *w;
+        if (v > last && v <= total) {
+            last = v;
+            progress(user, v, total);
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+}
+
 int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_t count, const modlin_opts_t *opts,
-             uint64_t *lcs) {
+             uint64_t *lcs, modlin_progress_fn progress, void *user) {
     ctx->stats = modlin_stats_t{};
     if (count == 0) return MODLIN_OK;
+    if (count != 1) progress = nullptr;
+    if (progress && !ctx->prog_host) {
+        void *h = nullptr;
+        if (hipHostMalloc(&h, 256, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->err = "pinned progress word allocation failed";
+            return MODLIN_ENOMEM;
+        }
+        ctx->prog_host = (uint64_t *)h;
+    }
+    uint64_t *prog_dev = nullptr;
+    if (progress) {
+        *(volatile uint64_t *)ctx->prog_host = 0;
+        MDL_HIP(hipHostGetDevicePointer((void **)&prog_dev, ctx->prog_host, 0));
+    }
     const FieldParams f = make_params(p);
     const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
     std::vector<size_t> offS(count), offC(count);
@@ -1399,6 +1466,9 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         hs[k].u = cd + 3 * n;
         hs[k].n = (uint32_t)n;
         hs[k].lc = 0;
+        hs[k].timeout = 0;
+        hs[k].pad = 0;
+        hs[k].prog = prog_dev;
         hs[k].bytes = 0;
     }
     BmSeq *ds = (BmSeq *)base;
@@ -1426,6 +1496,7 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
                 void *bargs[] = {(void *)&ds, (void *)&f, (void *)&bx};
                 if (hipLaunchCooperativeKernel(kb, dim3(Gb), dim3(BM_BLOCK_THREADS), bargs, 0, st) == hipSuccess) {
                     if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+                    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
                     uint32_t timeout = 0;
                     rc = copy_d2h(ctx, &timeout, &bx->timeout, sizeof(timeout));
                     if (rc) return rc;
@@ -1468,6 +1539,7 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
             goto single;
         }
         if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+        if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
         uint32_t timeout = 0;
         rc = copy_d2h(ctx, &timeout, &cs->timeout, sizeof(timeout));
         if (rc) return rc;
@@ -1524,8 +1596,14 @@ single:
     }
     if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
     MDL_HIP(hipGetLastError());
+    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
     rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));
     if (rc) return rc;
+    for (uint64_t k = 0; k < count; k++)
+        if (hs[k].timeout) {
+            ctx->err = "batched Berlekamp-Massey: a wait for a step record ran out";
+            return MODLIN_EHIP;
+        }
     double bytes = 0;
     for (uint64_t k = 0; k < count; k++) {
         lcs[ord[k]] = hs[k].lc;

@@ -38,14 +38,15 @@ extern "C" {
 
 int modlin_field_init(uint64_t p, modlin_field_t *out) { return fill_field(p, out); }
 
+// An invalid field gives UINT64_MAX, which no residue (< p < 2^63) equals.
 uint64_t modlin_reduce(const modlin_field_t *f, uint64_t x) {
-    if (!f || !valid_p(f->p)) return 0;
+    if (!f || !valid_p(f->p)) return UINT64_MAX;
     const FieldParams g = mdl::make_params(f->p);
     return mdl::reduce128(g, 0, x);
 }
 
 uint64_t modlin_reduce_u128(const modlin_field_t *f, uint64_t hi, uint64_t lo) {
-    if (!f || !valid_p(f->p)) return 0;
+    if (!f || !valid_p(f->p)) return UINT64_MAX;
     const FieldParams g = mdl::make_params(f->p);
     return mdl::reduce128(g, hi, lo);
 }
@@ -113,6 +114,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (ctx->arena) hipFree(ctx->arena);
     mdl::release_pinned(ctx);
     mdl::release_generators(ctx);
+    if (ctx->prog_host) hipHostFree(ctx->prog_host);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->evA) hipEventDestroy(ctx->evA);
@@ -194,7 +196,7 @@ int modlin_linear_complexity(modlin_ctx_t *ctx, const modlin_field_t *f, const u
     hipSetDevice(ctx->device);
     modlin_seq_desc_t d{s, n, MODLIN_ELEM_U64, MODLIN_MEM_HOST};
     try {
-        int rc = mdl::lc_batch(ctx, f->p, &d, 1, nullptr, lc);
+        int rc = mdl::lc_batch(ctx, f->p, &d, 1, nullptr, lc, progress, user);
         if (rc == MODLIN_OK && progress) progress(user, n, n);
         return rc;
     } catch (const std::bad_alloc &) {

@@ -327,6 +327,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         }
     }
     ctx->stats.graph = use_graph ? 1.0 : 0.0;
+    // Progress in matrix columns, as the reference's update_with_count(jend - col)
+    // per panel (fp.rs:251): at every host check, the columns below each matrix's
+    // current panel (all n once it is done), never decreasing (a missed pivot moves
+    // a panel back).
+    uint64_t total_cols = 0, shown_cols = 0;
+    for (uint64_t i = 0; i < count; i++) total_cols += mats[i].n;
     MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
     while (h_active > 0) {
         if (use_graph)
@@ -345,8 +351,18 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
                 upd_launches += 2;
             }
         }
-        if (progress) progress(user, count - h_active, count);
+        if (progress && h_active > 0) {
+            rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
+            if (rc) return rc;
+            uint64_t cols = 0;
+            for (uint64_t i = 0; i < count; i++) cols += hm[i].done ? hm[i].n : std::min(hm[i].col, hm[i].n);
+            if (cols > shown_cols) {
+                shown_cols = cols;
+                progress(user, cols, total_cols);
+            }
+        }
     }
+    if (progress) progress(user, total_cols, total_cols);
     MDL_HIP(hipStreamSynchronize(aux));
     for (auto &e : evs) hipEventDestroy(e);
     rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));

@@ -34,6 +34,9 @@ struct modlin_ctx {
     void *pinned[2] = {nullptr, nullptr};
     size_t pinned_bytes = 0;
     hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+    // one pinned, coherent host word that a running BM kernel updates with its
+    // step count (the progress callback of modlin_linear_complexity)
+    uint64_t *prog_host = nullptr;
     // device generators: tabulated jump powers per generator kind, and a workspace
     void *gen_tab[8] = {};
     void *gen_ws = nullptr;
@@ -69,7 +72,9 @@ int prng_fill_device(modlin_ctx *ctx, uint32_t kind, uint64_t seed, uint64_t ski
                      const modlin_field_t *field, uint32_t elem, void *out, uint64_t count);
 void release_generators(modlin_ctx *ctx);
 
+// progress (may be null) is called with the steps done of a single sequence
+// (count == 1) while its kernel runs.
 int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_t count, const modlin_opts_t *opts,
-             uint64_t *lcs);
+             uint64_t *lcs, modlin_progress_fn progress = nullptr, void *user = nullptr);
 
 }  // namespace mdl

@@ -223,7 +223,15 @@ class Context:
         return {name: getattr(s, name) for name, _ in _Stats._fields_}
 
     # --- single-item entry points (src/fp.rs:125, 260) -------------------
-    def rank(self, field: Field, m: np.ndarray, n: int | None = None) -> int:
+    @staticmethod
+    def _progress_fn(progress):
+        """C callback for progress(done, total), or a null one."""
+        if progress is None:
+            return _PROGRESS()
+        return _PROGRESS(lambda _user, done, total: progress(int(done), int(total)))
+
+    def rank(self, field: Field, m: np.ndarray, n: int | None = None, progress=None) -> int:
+        """Rank of the n x n matrix m; progress(done_columns, n) is called while it runs."""
         a = np.ascontiguousarray(m, dtype=np.uint64)
         if n is None:
             n = int(round(a.size ** 0.5))
@@ -231,15 +239,18 @@ class Context:
             raise ValueError("matrix must hold n*n entries")
         out = ctypes.c_uint64()
         ptr = _u64p(a) if a.size else None
-        self._check(self._lib.modlin_rank(self._h, ctypes.byref(field._f), ptr, n, _PROGRESS(), None,
+        cb = self._progress_fn(progress)
+        self._check(self._lib.modlin_rank(self._h, ctypes.byref(field._f), ptr, n, cb, None,
                                           ctypes.byref(out)), "rank")
         return out.value
 
-    def linear_complexity(self, field: Field, s: np.ndarray) -> int:
+    def linear_complexity(self, field: Field, s: np.ndarray, progress=None) -> int:
+        """Linear complexity of s; progress(done_steps, len(s)) is called while it runs."""
         a = np.ascontiguousarray(s, dtype=np.uint64)
         out = ctypes.c_uint64()
         ptr = _u64p(a) if a.size else None
-        self._check(self._lib.modlin_linear_complexity(self._h, ctypes.byref(field._f), ptr, a.size, _PROGRESS(),
+        cb = self._progress_fn(progress)
+        self._check(self._lib.modlin_linear_complexity(self._h, ctypes.byref(field._f), ptr, a.size, cb,
                                                        None, ctypes.byref(out)), "linear_complexity")
         return out.value
 

@@ -78,3 +78,17 @@ def test_context_without_gpu_fails_loudly():
         pytest.skip("a GPU driver is present")
     with pytest.raises(RuntimeError):
         modlin.Context(0)
+
+
+def test_reduce_invalid_field_sentinel():
+    """modlin_reduce / modlin_reduce_u128 answer an invalid field with UINT64_MAX,
+    a value no residue (< p < 2^63) can take; valid fields reduce as fp.rs:63-79."""
+    lib = modlin.load()
+    bad = modlin._FieldT()
+    bad.p = 1
+    assert lib.modlin_reduce(ctypes.byref(bad), 5) == (1 << 64) - 1
+    assert lib.modlin_reduce_u128(ctypes.byref(bad), 1, 5) == (1 << 64) - 1
+    assert lib.modlin_reduce(None, 5) == (1 << 64) - 1
+    f = modlin.Field(97)
+    assert f.reduce(1000) == 1000 % 97
+    assert f.reduce_u128((1 << 127) + 3) == ((1 << 127) + 3) % 97

@@ -212,3 +212,20 @@ def test_blocked_batch_high_order_m61(ctx):
     assert ctx.lc_batch(f, seqs) == want
     assert ctx.stats()["bm_coop"] == 0
     assert ctx.lc_batch(f, seqs, method=modlin.METHOD_PLAIN) == want
+
+
+@pytest.mark.parametrize("n, coop", [(5000, False), (200_000, True)])
+def test_lc_progress_in_steps(ctx, n, coop):
+    """modlin_linear_complexity reports progress in steps (fp.rs:273): monotone,
+    total n, the last report n -- on the one-workgroup blocked kernel and on the
+    cooperative one (the README's MIXMAX-256 run)."""
+    s = workloads.c5_bm(n) if coop else (modlin.prng_fill(modlin.PRNG_XOROSHIRO128PP, 2, n) % np.uint64(M61)).astype(
+        np.uint64)
+    seen = []
+    lc = ctx.linear_complexity(modlin.Field(M61), s, progress=lambda done, total: seen.append((done, total)))
+    assert lc == (65280 if coop else oracle.lc(M61, s))
+    assert seen and all(t == n for _, t in seen)
+    done = [d for d, _ in seen]
+    assert done == sorted(done) and done[-1] == n
+    if coop:
+        assert len(done) >= 3   # reports while the kernel runs

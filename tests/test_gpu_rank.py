@@ -210,3 +210,21 @@ def test_tuning_table_loaded_and_cleared(ctx, tmp_path):
             ctx.load_tuning("")
     with pytest.raises(ValueError):
         ctx.load_tuning(str(tmp_path / "missing.txt"))
+
+
+@pytest.mark.parametrize("n", [1000, 3000])
+def test_rank_progress_in_columns(ctx, n):
+    """modlin_rank reports progress in columns (fp.rs:251): monotone, total n, the
+    last report n; a deficient MIXMAX-17 matrix still reports every column."""
+    rng = np.random.default_rng(n)
+    f = modlin.Field(M61)
+    for a in (rng.integers(0, M61, size=(n, n), dtype=np.uint64),
+              (modlin.prng_fill(modlin.PRNG_MIXMAX17, 9, n * n) % np.uint64(M61)).astype(np.uint64).reshape(n, n)):
+        seen = []
+        r = ctx.rank(f, a, progress=lambda done, total: seen.append((done, total)))
+        assert r == oracle.rank(M61, a, n)
+        assert seen and all(t == n for _, t in seen)
+        done = [d for d, _ in seen]
+        assert done == sorted(done) and done[-1] == n and len(set(done)) == len(done)
+        if n >= 3000:
+            assert len(done) >= 2   # intermediate reports (one host check per 8 panels)
