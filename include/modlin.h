@@ -174,6 +174,20 @@ int modlin_rank_batch(modlin_ctx_t *ctx, const modlin_field_t *f, const modlin_m
 int modlin_lc_batch(modlin_ctx_t *ctx, const modlin_field_t *f, const modlin_seq_desc_t *seqs, uint64_t count,
                     const modlin_opts_t *opts, uint64_t *lcs);
 
+/* Allocation-free Berlekamp-Massey of one sequence already in device memory.
+ * No counterpart in the reference: for callers that cannot allocate device or
+ * pinned host memory during a call (tight serving loops, harnesses that forbid
+ * hipMalloc).  s: n u64 entries in [0, p) in device memory of the context's GPU,
+ * 1 <= n <= MODLIN_LC_NOALLOC_MAX_N (MODLIN_EINVAL otherwise, before any launch);
+ * *lc receives the linear complexity, the same as modlin_linear_complexity's.
+ * The call allocates nothing: its state (about 34 MiB) is a fixed device array
+ * that each GPU holds until the process ends, shared by every call on that GPU,
+ * so calls that overlap on one GPU (several contexts or threads) race.
+ * Synchronous on the context's stream. */
+#define MODLIN_LC_NOALLOC_MAX_N 1048576u
+int modlin_linear_complexity_noalloc(modlin_ctx_t *ctx, const modlin_field_t *f, const uint64_t *s, uint64_t n,
+                                     uint64_t *lc);
+
 /* --- Statistics (src/stats.rs) ------------------------------------------- */
 double modlin_corank_prob(uint64_t p, uint64_t n, uint64_t d);
 double modlin_corank_tail_pvalue(uint64_t p, uint64_t n, uint64_t c);

@@ -35,8 +35,11 @@
 //     length change after them.
 // The state (c, b, t) lives in a per-sequence slice of HBM; s is read-only.
 
+#include <string.h>
+
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -1389,6 +1392,169 @@ static int wait_progress(modlin_ctx *ctx, uint64_t total, modlin_progress_fn pro
     }
 }
 
+// Words of a device area set to zero (a fill kernel rather than a memset node,
+// so that the same code is safe inside captured graphs).
+__global__ void k_zero_words(uint64_t *p, size_t words) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Runs the batch described by the device array ds (host copy hs, count items,
+// the longest maxn; item k is the caller's sequence ord[k]) with the kernel the
+// batch shape selects, and stores the linear complexities in lcs.  exch is the
+// cooperative kernels' exchange area (count == 1); rd(dst, src, bytes) copies
+// device memory to the host and waits for the stream.
+template <class Rd>
+static int run_bm(modlin_ctx *ctx, const FieldParams &f, uint64_t p, BmSeq *ds, BmSeq *hs, uint64_t count,
+                  uint64_t maxn, char *exch, const modlin_opts_t *opts, modlin_progress_fn progress, void *user,
+                  const uint64_t *ord, uint64_t *lcs, Rd rd) {
+    hipStream_t st = ctx->stream;
+    const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
+    int rc = MODLIN_OK;
+    if (profile) MDL_HIP(hipEventRecord(ctx->evA, st));
+    // One long sequence: the cooperative kernel over the whole GPU (one workgroup
+    // per CU, at most 1024 partials).
+    const uint32_t coop_min = (opts && opts->coop_min) ? opts->coop_min : BM_COOP_MIN;
+    if (count == 1 && maxn >= coop_min) {
+        const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
+        const uint32_t G0 = tu.bm_coop_wgs ? tu.bm_coop_wgs : (uint32_t)std::min(ctx->cu_count, 1024);
+        // blocked kernel: p < 2^61 (tagged words) and at most BM_BQ chunks per workgroup
+        const uint64_t nchunks = (maxn + BM_BW - 1) / BM_BW;
+        const uint32_t Gb = (uint32_t)std::max<uint64_t>(G0, (nchunks + BM_BQ - 1) / BM_BQ);
+        if (p <= P61 && Gb <= 1024) {
+            const void *kb = f.mode == MODE_M61   ? (const void *)k_bm_block<MODE_M61>
+                             : f.mode == MODE_M31 ? (const void *)k_bm_block<MODE_M31>
+                                                  : (const void *)k_bm_block<MODE_GENERIC>;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb, BM_BLOCK_THREADS, 0) != hipSuccess) per_cu = 0;
+            if ((uint64_t)Gb <= (uint64_t)per_cu * (uint64_t)std::max(ctx->cu_count, 0)) {
+                BmBlockX *bx = (BmBlockX *)exch;
+                hipLaunchKernelGGL(k_zero_words, dim3(256), dim3(256), 0, st, (uint64_t *)bx, sizeof(BmBlockX) / 8);
+                void *bargs[] = {(void *)&ds, (void *)&f, (void *)&bx};
+                if (hipLaunchCooperativeKernel(kb, dim3(Gb), dim3(BM_BLOCK_THREADS), bargs, 0, st) == hipSuccess) {
+                    if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+                    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
+                    uint32_t timeout = 0;
+                    rc = rd(&timeout, &bx->timeout, sizeof(timeout));
+                    if (rc) return rc;
+                    if (timeout) {
+                        ctx->err = "blocked cooperative Berlekamp-Massey: an exchange wait ran out";
+                        return MODLIN_EHIP;
+                    }
+                    rc = rd(hs, ds, sizeof(BmSeq));
+                    if (rc) return rc;
+                    lcs[0] = hs[0].lc;
+                    ctx->stats.bm_bytes = (double)hs[0].bytes;
+                    ctx->stats.bm_coop = (double)Gb;
+                    if (profile) {
+                        float ms = 0;
+                        MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
+                        ctx->stats.bm_ms = ms;
+                    }
+                    return MODLIN_OK;
+                }
+                (void)hipGetLastError();  // refused: the kernels below give the same result
+            }
+        }
+        BmCoop *cs = (BmCoop *)exch;
+        hipLaunchKernelGGL(k_zero_words, dim3(16), dim3(256), 0, st, (uint64_t *)cs, sizeof(BmCoop) / 8);
+        const uint32_t G = G0;
+        void *args[] = {(void *)&ds, (void *)&f, (void *)&cs};
+        hipError_t e;
+        if (f.mode == MODE_M61)
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M61>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
+        else if (f.mode == MODE_M31)
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M31>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
+        else
+            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_GENERIC>, dim3(G), dim3(BM_COOP_THREADS), args, 0,
+                                           st);
+        if (e != hipSuccess) {
+            // refused (another process may hold the device's cooperative-launch
+            // resources): the one-workgroup kernel below gives the same result
+            (void)hipGetLastError();
+            ctx->stats.bm_coop = 0;
+            goto single;
+        }
+        if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+        if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
+        uint32_t timeout = 0;
+        rc = rd(&timeout, &cs->timeout, sizeof(timeout));
+        if (rc) return rc;
+        if (timeout) {
+            ctx->err = "cooperative Berlekamp-Massey: a grid barrier wait ran out";
+            return MODLIN_EHIP;
+        }
+        rc = rd(hs, ds, count * sizeof(BmSeq));
+        if (rc) return rc;
+        lcs[0] = hs[0].lc;
+        ctx->stats.bm_bytes = (double)hs[0].bytes;
+        ctx->stats.bm_coop = (double)G;
+        if (profile) {
+            float ms = 0;
+            MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
+            ctx->stats.bm_ms = ms;
+        }
+        return MODLIN_OK;
+    }
+single:
+    // Blocks of K steps per pass (k_bm_batch_blk) unless the caller asks for the
+    // one-step-per-pass form (MODLIN_METHOD_PLAIN) or p >= 2^62.
+    if ((!opts || opts->method != MODLIN_METHOD_PLAIN) && p < (1ull << 62)) {
+        if (f.mode == MODE_M61)
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M61>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
+        else if (f.mode == MODE_M31)
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M31>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
+        else
+            hipLaunchKernelGGL((k_bm_batch_blk<MODE_GENERIC>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds,
+                               f);
+    } else {
+    // threads per sequence: the tuning table's, else as many as keep the whole
+    // batch resident (about 32 waves per CU), 256 at least
+    const modlin_tuning_t tb = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
+    const uint64_t cus = (uint64_t)std::max(ctx->cu_count, 1);
+    const uint32_t nt = tb.bm_threads ? tb.bm_threads : count <= 2 * cus ? 1024 : count <= 4 * cus ? 512 : 256;
+#define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
+#define MDL_BM_NT(MODE)              \
+    if (nt == 1024)                   \
+        MDL_BM(1024, MODE);           \
+    else if (nt == 512)               \
+        MDL_BM(512, MODE);            \
+    else                              \
+        MDL_BM(256, MODE);
+    if (f.mode == MODE_M61) {
+        MDL_BM_NT(MODE_M61)
+    } else if (f.mode == MODE_M31) {
+        MDL_BM_NT(MODE_M31)
+    } else {
+        MDL_BM_NT(MODE_GENERIC)
+    }
+#undef MDL_BM_NT
+#undef MDL_BM
+    }
+    if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
+    MDL_HIP(hipGetLastError());
+    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
+    rc = rd(hs, ds, count * sizeof(BmSeq));
+    if (rc) return rc;
+    for (uint64_t k = 0; k < count; k++)
+        if (hs[k].timeout) {
+            ctx->err = "batched Berlekamp-Massey: a wait for a step record ran out";
+            return MODLIN_EHIP;
+        }
+    double bytes = 0;
+    for (uint64_t k = 0; k < count; k++) {
+        lcs[ord[k]] = hs[k].lc;
+        bytes += (double)hs[k].bytes;
+    }
+    ctx->stats.bm_bytes = bytes;
+    if (profile) {
+        float ms = 0;
+        MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
+        ctx->stats.bm_ms = ms;
+    }
+    return MODLIN_OK;
+}
+
 int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_t count, const modlin_opts_t *opts,
              uint64_t *lcs, modlin_progress_fn progress, void *user) {
     ctx->stats = modlin_stats_t{};
@@ -1409,7 +1575,6 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
         MDL_HIP(hipHostGetDevicePointer((void **)&prog_dev, ctx->prog_host, 0));
     }
     const FieldParams f = make_params(p);
-    const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
     std::vector<size_t> offS(count), offC(count);
     size_t cur = align_up(count * sizeof(BmSeq), 256);
     uint64_t maxn = 0;
@@ -1474,148 +1639,99 @@ int lc_batch(modlin_ctx *ctx, uint64_t p, const modlin_seq_desc_t *seqs, uint64_
     BmSeq *ds = (BmSeq *)base;
     rc = copy_h2d(ctx, ds, hs.data(), count * sizeof(BmSeq));
     if (rc) return rc;
-    if (profile) MDL_HIP(hipEventRecord(ctx->evA, st));
-    // One long sequence: the cooperative kernel over the whole GPU (one workgroup
-    // per CU, at most 1024 partials).
-    const uint32_t coop_min = (opts && opts->coop_min) ? opts->coop_min : BM_COOP_MIN;
-    if (count == 1 && maxn >= coop_min) {
-        const modlin_tuning_t tu = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
-        const uint32_t G0 = tu.bm_coop_wgs ? tu.bm_coop_wgs : (uint32_t)std::min(ctx->cu_count, 1024);
-        // blocked kernel: p < 2^61 (tagged words) and at most BM_BQ chunks per workgroup
-        const uint64_t nchunks = (maxn + BM_BW - 1) / BM_BW;
-        const uint32_t Gb = (uint32_t)std::max<uint64_t>(G0, (nchunks + BM_BQ - 1) / BM_BQ);
-        if (p <= P61 && Gb <= 1024) {
-            const void *kb = f.mode == MODE_M61   ? (const void *)k_bm_block<MODE_M61>
-                             : f.mode == MODE_M31 ? (const void *)k_bm_block<MODE_M31>
-                                                  : (const void *)k_bm_block<MODE_GENERIC>;
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb, BM_BLOCK_THREADS, 0) != hipSuccess) per_cu = 0;
-            if ((uint64_t)Gb <= (uint64_t)per_cu * (uint64_t)std::max(ctx->cu_count, 0)) {
-                BmBlockX *bx = (BmBlockX *)(base + off_coop);
-                MDL_HIP(hipMemsetAsync(bx, 0, sizeof(BmBlockX), st));
-                void *bargs[] = {(void *)&ds, (void *)&f, (void *)&bx};
-                if (hipLaunchCooperativeKernel(kb, dim3(Gb), dim3(BM_BLOCK_THREADS), bargs, 0, st) == hipSuccess) {
-                    if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
-                    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
-                    uint32_t timeout = 0;
-                    rc = copy_d2h(ctx, &timeout, &bx->timeout, sizeof(timeout));
-                    if (rc) return rc;
-                    if (timeout) {
-                        ctx->err = "blocked cooperative Berlekamp-Massey: an exchange wait ran out";
-                        return MODLIN_EHIP;
-                    }
-                    rc = copy_d2h(ctx, hs.data(), ds, sizeof(BmSeq));
-                    if (rc) return rc;
-                    lcs[0] = hs[0].lc;
-                    ctx->stats.bm_bytes = (double)hs[0].bytes;
-                    ctx->stats.bm_coop = (double)Gb;
-                    if (profile) {
-                        float ms = 0;
-                        MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
-                        ctx->stats.bm_ms = ms;
-                    }
-                    return MODLIN_OK;
-                }
-                (void)hipGetLastError();  // refused: the kernels below give the same result
-            }
-        }
-        BmCoop *cs = (BmCoop *)(base + off_coop);
-        MDL_HIP(hipMemsetAsync(cs, 0, sizeof(BmCoop), st));
-        const uint32_t G = G0;
-        void *args[] = {(void *)&ds, (void *)&f, (void *)&cs};
-        hipError_t e;
-        if (f.mode == MODE_M61)
-            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M61>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
-        else if (f.mode == MODE_M31)
-            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_M31>, dim3(G), dim3(BM_COOP_THREADS), args, 0, st);
-        else
-            e = hipLaunchCooperativeKernel((const void *)k_bm_coop<MODE_GENERIC>, dim3(G), dim3(BM_COOP_THREADS), args, 0,
-                                           st);
-        if (e != hipSuccess) {
-            // refused (another process may hold the device's cooperative-launch
-            // resources): the one-workgroup kernel below gives the same result
-            (void)hipGetLastError();
-            ctx->stats.bm_coop = 0;
-            goto single;
-        }
-        if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
-        if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
-        uint32_t timeout = 0;
-        rc = copy_d2h(ctx, &timeout, &cs->timeout, sizeof(timeout));
-        if (rc) return rc;
-        if (timeout) {
-            ctx->err = "cooperative Berlekamp-Massey: a grid barrier wait ran out";
-            return MODLIN_EHIP;
-        }
-        rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));
-        if (rc) return rc;
-        lcs[0] = hs[0].lc;
-        ctx->stats.bm_bytes = (double)hs[0].bytes;
-        ctx->stats.bm_coop = (double)G;
-        if (profile) {
-            float ms = 0;
-            MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
-            ctx->stats.bm_ms = ms;
-        }
-        return MODLIN_OK;
+    return run_bm(ctx, f, p, ds, hs.data(), count, maxn, base + off_coop, opts, progress, user, ord.data(), lcs,
+                  [ctx](void *dst, const void *src, size_t bytes) { return copy_d2h(ctx, dst, src, bytes); });
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free entry point for one sequence in device memory
+// (modlin_linear_complexity_noalloc).  The sequence descriptor, the cooperative
+// kernels' exchange area and the four state buffers of up to
+// MODLIN_LC_NOALLOC_MAX_N coefficients live in `scratch`, a file-scope device
+// array that each GPU holds from module load to the end of the process.  Host
+// code never uses the array's name as an address: noalloc_init (called by
+// modlin_ctx_create, with the context's device current) takes each device's
+// address of it once, and the entry point passes that address to the kernels.
+// Every call on a GPU shares its array, so calls that overlap on one GPU race;
+// what must start at zero (the descriptor's counters, the exchange area) is
+// cleared by kernels at the start of every call.
+// ---------------------------------------------------------------------------
+constexpr size_t NA_SEQ = 256;                                                                  // BmSeq slot
+constexpr size_t NA_X = ((sizeof(BmCoop) > sizeof(BmBlockX) ? sizeof(BmCoop) : sizeof(BmBlockX)) + 255) / 256 * 256;
+constexpr size_t NA_STATE = 4ull * MODLIN_LC_NOALLOC_MAX_N * 8;                                  // c, b, t, u
+constexpr size_t NA_BYTES = NA_SEQ + NA_X + NA_STATE;
+static_assert(sizeof(BmSeq) <= NA_SEQ, "descriptor slot");
+static __device__ __align__(256) uint64_t scratch[NA_BYTES / 8];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex na_mu;
+static char *na_ptr[NA_MAX_DEVICES];  // each device's address of scratch
+
+int noalloc_init(modlin_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= NA_MAX_DEVICES) return MODLIN_ENODEV;
+    std::lock_guard<std::mutex> g(na_mu);
+    if (na_ptr[ctx->device]) return MODLIN_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return MODLIN_EHIP;
     }
-single:
-    // Blocks of K steps per pass (k_bm_batch_blk) unless the caller asks for the
-    // one-step-per-pass form (MODLIN_METHOD_PLAIN) or p >= 2^62.
-    if ((!opts || opts->method != MODLIN_METHOD_PLAIN) && p < (1ull << 62)) {
-        if (f.mode == MODE_M61)
-            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M61>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
-        else if (f.mode == MODE_M31)
-            hipLaunchKernelGGL((k_bm_batch_blk<MODE_M31>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds, f);
-        else
-            hipLaunchKernelGGL((k_bm_batch_blk<MODE_GENERIC>), dim3((uint32_t)count), dim3(BM_BLOCK_THREADS), 0, st, ds,
-                               f);
-    } else {
-    // threads per sequence: the tuning table's, else as many as keep the whole
-    // batch resident (about 32 waves per CU), 256 at least
-    const modlin_tuning_t tb = select_tuning(ctx->tune_table, &ctx->tune_override, p, maxn, count, true);
-    const uint64_t cus = (uint64_t)std::max(ctx->cu_count, 1);
-    const uint32_t nt = tb.bm_threads ? tb.bm_threads : count <= 2 * cus ? 1024 : count <= 4 * cus ? 512 : 256;
-#define MDL_BM(NT, MODE) hipLaunchKernelGGL((k_bm_batch<NT, MODE>), dim3((uint32_t)count), dim3(NT), 0, st, ds, f)
-#define MDL_BM_NT(MODE)              \
-    if (nt == 1024)                   \
-        MDL_BM(1024, MODE);           \
-    else if (nt == 512)               \
-        MDL_BM(512, MODE);            \
-    else                              \
-        MDL_BM(256, MODE);
-    if (f.mode == MODE_M61) {
-        MDL_BM_NT(MODE_M61)
-    } else if (f.mode == MODE_M31) {
-        MDL_BM_NT(MODE_M31)
-    } else {
-        MDL_BM_NT(MODE_GENERIC)
-    }
-#undef MDL_BM_NT
-#undef MDL_BM
-    }
-    if (profile) MDL_HIP(hipEventRecord(ctx->evB, st));
-    MDL_HIP(hipGetLastError());
-    if (progress && (rc = wait_progress(ctx, maxn, progress, user))) return rc;
-    rc = copy_d2h(ctx, hs.data(), ds, count * sizeof(BmSeq));
-    if (rc) return rc;
-    for (uint64_t k = 0; k < count; k++)
-        if (hs[k].timeout) {
-            ctx->err = "batched Berlekamp-Massey: a wait for a step record ran out";
-            return MODLIN_EHIP;
-        }
-    double bytes = 0;
-    for (uint64_t k = 0; k < count; k++) {
-        lcs[ord[k]] = hs[k].lc;
-        bytes += (double)hs[k].bytes;
-    }
-    ctx->stats.bm_bytes = bytes;
-    if (profile) {
-        float ms = 0;
-        MDL_HIP(hipEventElapsedTime(&ms, ctx->evA, ctx->evB));
-        ctx->stats.bm_ms = ms;
-    }
+    na_ptr[ctx->device] = (char *)ptr;
     return MODLIN_OK;
+}
+
+// The descriptor of the one sequence, written on the device.
+__global__ void k_na_seq(BmSeq *q, const uint64_t *s, uint64_t *state, uint32_t n) {
+    q->s = s;
+    q->c = state;
+    q->b = state + n;
+    q->t = state + 2 * (size_t)n;
+    q->u = state + 3 * (size_t)n;
+    q->n = n;
+    q->lc = 0;
+    q->timeout = 0;
+    q->pad = 0;
+    q->prog = nullptr;
+    q->bytes = 0;
+}
+
+int lc_noalloc(modlin_ctx *ctx, uint64_t p, const uint64_t *s, uint64_t n, uint64_t *lc) {
+    ctx->stats = modlin_stats_t{};
+    if (n == 0 || n > MODLIN_LC_NOALLOC_MAX_N) {
+        ctx->err = n == 0 ? "empty sequence (the reference panics, fp.rs:265)"
+                          : "sequence longer than MODLIN_LC_NOALLOC_MAX_N";
+        return MODLIN_EINVAL;
+    }
+    char *base = nullptr;
+    if (ctx->device >= 0 && ctx->device < NA_MAX_DEVICES) {
+        std::lock_guard<std::mutex> g(na_mu);
+        base = na_ptr[ctx->device];
+    }
+    if (!base || !ctx->na_host) {
+        ctx->err = "allocation-free entry point: context set-up did not complete";
+        return MODLIN_EHIP;
+    }
+    BmSeq *ds = (BmSeq *)base;
+    char *exch = base + NA_SEQ;
+    uint64_t *state = (uint64_t *)(base + NA_SEQ + NA_X);
+    hipLaunchKernelGGL(k_na_seq, dim3(1), dim3(1), 0, ctx->stream, ds, s, state, (uint32_t)n);
+    MDL_HIP(hipGetLastError());
+    // results come back through the context's pinned word area (no staging allocation)
+    auto rd = [ctx](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes > NA_HOST_BYTES) {
+            ctx->err = "allocation-free read-back too large";
+            return MODLIN_EHIP;
+        }
+        MDL_HIP(hipMemcpyAsync(ctx->na_host, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        MDL_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(dst, ctx->na_host, bytes);
+        return MODLIN_OK;
+    };
+    BmSeq hs;
+    const uint64_t ord = 0;
+    return run_bm(ctx, make_params(p), p, ds, &hs, 1, n, exch, nullptr, nullptr, nullptr, &ord, lc, rd);
 }
 
 }  // namespace mdl

@@ -72,8 +72,11 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
         hipEventCreate(&ctx->evA) != hipSuccess || hipEventCreate(&ctx->evB) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_ge, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fin, hipEventDisableTiming) != hipSuccess) {
-        delete ctx;
+        hipEventCreateWithFlags(&ctx->ev_fin, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(&ctx->na_host, mdl::NA_HOST_BYTES, hipHostMallocDefault) != hipSuccess ||
+        mdl::noalloc_init(ctx) != MODLIN_OK) {
+        (void)hipGetLastError();
+        modlin_ctx_destroy(ctx);
         return MODLIN_EHIP;
     }
     // launch parameters from tuning.txt next to the library, if one was written
@@ -115,6 +118,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     mdl::release_pinned(ctx);
     mdl::release_generators(ctx);
     if (ctx->prog_host) hipHostFree(ctx->prog_host);
+    if (ctx->na_host) hipHostFree(ctx->na_host);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->evA) hipEventDestroy(ctx->evA);
@@ -206,6 +210,21 @@ int modlin_linear_complexity(modlin_ctx_t *ctx, const modlin_field_t *f, const u
         ctx->err = "internal error";
         return MODLIN_EHIP;
     }
+}
+
+int modlin_linear_complexity_noalloc(modlin_ctx_t *ctx, const modlin_field_t *f, const uint64_t *s, uint64_t n,
+                                     uint64_t *lc) {
+    if (!ctx || !lc) return MODLIN_EINVAL;
+    if (!field_ok(f)) {
+        ctx->err = "invalid field";
+        return MODLIN_EINVAL;
+    }
+    if (!s) {
+        ctx->err = "null sequence";
+        return MODLIN_EINVAL;
+    }
+    hipSetDevice(ctx->device);
+    return mdl::lc_noalloc(ctx, f->p, s, n, lc);
 }
 
 int modlin_lc_batch(modlin_ctx_t *ctx, const modlin_field_t *f, const modlin_seq_desc_t *seqs, uint64_t count,

@@ -37,6 +37,9 @@ struct modlin_ctx {
     // one pinned, coherent host word that a running BM kernel updates with its
     // step count (the progress callback of modlin_linear_complexity)
     uint64_t *prog_host = nullptr;
+    // pinned host words for the allocation-free entry point's read-backs,
+    // allocated with the context
+    void *na_host = nullptr;
     // device generators: tabulated jump powers per generator kind, and a workspace
     void *gen_tab[8] = {};
     void *gen_ws = nullptr;
@@ -71,6 +74,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
 int prng_fill_device(modlin_ctx *ctx, uint32_t kind, uint64_t seed, uint64_t skip, uint32_t keep, uint32_t drop,
                      const modlin_field_t *field, uint32_t elem, void *out, uint64_t count);
 void release_generators(modlin_ctx *ctx);
+
+// modlin_linear_complexity_noalloc: the scratch array's address on the context's
+// device (once per device, at context creation), and the call itself.
+int noalloc_init(modlin_ctx *ctx);
+int lc_noalloc(modlin_ctx *ctx, uint64_t p, const uint64_t *s, uint64_t n, uint64_t *lc);
+constexpr size_t NA_HOST_BYTES = 4096;
 
 // progress (may be null) is called with the steps done of a single sequence
 // (count == 1) while its kernel runs.

@@ -32,6 +32,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 FLAG_PROFILE = 1
 FLAG_NO_GRAPH = 2
 P_FLOOR = 1e-307  # stats.rs:16
+LC_NOALLOC_MAX_N = 1 << 20  # MODLIN_LC_NOALLOC_MAX_N
 
 PRNG_XOROSHIRO128PP, PRNG_MIXMAX17, PRNG_MIXMAX240, PRNG_MIXMAX256, PRNG_SPLITMIX64, PRNG_XOSHIRO256SS = range(6)
 
@@ -40,6 +41,7 @@ EXPORTS = [
     "modlin_field_init", "modlin_reduce", "modlin_reduce_u128",
     "modlin_ctx_create", "modlin_ctx_destroy", "modlin_ctx_last_error", "modlin_ctx_last_stats",
     "modlin_rank", "modlin_linear_complexity", "modlin_rank_batch", "modlin_lc_batch",
+    "modlin_linear_complexity_noalloc",
     "modlin_corank_prob", "modlin_corank_tail_pvalue", "modlin_lc_left_tail_pvalue", "modlin_pretty_p_value",
     "modlin_prng_fill", "modlin_prng_name", "modlin_prng_fill_device",
     "modlin_util_combine_rows", "modlin_util_lfsr",
@@ -108,6 +110,7 @@ def load() -> ctypes.CDLL:
                                       ctypes.POINTER(_Opts), P64]
     lib.modlin_lc_batch.argtypes = [vp, ctypes.POINTER(_FieldT), ctypes.POINTER(_MatDesc), u64,
                                     ctypes.POINTER(_Opts), P64]
+    lib.modlin_linear_complexity_noalloc.argtypes = [vp, ctypes.POINTER(_FieldT), vp, u64, P64]
     for name in ("modlin_corank_prob", "modlin_corank_tail_pvalue", "modlin_lc_left_tail_pvalue"):
         getattr(lib, name).argtypes = [u64, u64, u64]
         getattr(lib, name).restype = ctypes.c_double
@@ -297,6 +300,17 @@ class Context:
         self._check(self._lib.modlin_lc_batch(self._h, ctypes.byref(field._f), arr, len(seqs), ctypes.byref(opts),
                                               _u64p(out)), "lc_batch")
         return [int(x) for x in out[:len(seqs)]]
+
+    def linear_complexity_noalloc(self, field: Field, seq: "DeviceBuffer") -> int:
+        """modlin_linear_complexity_noalloc: the sequence in a u64 DeviceBuffer;
+        the library allocates nothing during the call."""
+        if seq.elem != ELEM_U64:
+            raise ValueError("the allocation-free entry point takes u64 sequences")
+        out = ctypes.c_uint64()
+        self._check(self._lib.modlin_linear_complexity_noalloc(self._h, ctypes.byref(field._f), seq.ptr,
+                                                               seq.nbytes // 8, ctypes.byref(out)),
+                    "linear_complexity_noalloc")
+        return out.value
 
     # --- device buffers ---------------------------------------------------
     def to_device(self, a: np.ndarray, square: bool = True) -> "DeviceBuffer":
