@@ -139,20 +139,74 @@ def fixture(name):
     return json.load(open(path)) if os.path.exists(path) else None
 
 
-def single_large(ctx, n, p, steps):
-    """One n x n planted-corank matrix: MFMA and plain paths, columns/s."""
-    a = W.c3(n, p)
+def single_large(ctx, n, p, steps, plain=True, warmup=1):
+    """One n x n planted-corank matrix (built on the device): MFMA and plain
+    paths, columns/s; each result must be corank 64."""
+    buf = W.c3_device(ctx, n, p)
     field = M.Field(p)
-    dev = [ctx.to_device(a)]
+    dev = [buf]
     out = {"n": n, "p": p}
-    for name, method in (("mfma", M.METHOD_MFMA), ("plain", M.METHOD_PLAIN)):
-        ranks, t, st, _ = run(ctx, field, dev, method, steps, 1)
+    L = limb_count(p)
+    methods = (("mfma", M.METHOD_MFMA), ("plain", M.METHOD_PLAIN)) if plain else (("mfma", M.METHOD_MFMA),)
+    for name, method in methods:
+        ranks, t, st, _ = run(ctx, field, dev, method, steps, warmup)
         out[f"{name}_columns_per_s"] = n / t
         out[f"{name}_ms"] = t * 1e3
         out[f"{name}_corank"] = n - ranks[0]
         out[f"{name}_update_ms"] = st["update_ms"]
-    out["speedup_mfma_vs_plain"] = out["mfma_columns_per_s"] / out["plain_columns_per_s"]
+        if name == "mfma" and st["update_ms"]:
+            out["mfma_update_frac"] = 2.0 * L * L * st["macs"] / (st["update_ms"] / 1e3) / 1e12 / PEAK_INT8_TOPS
+    if plain:
+        out["speedup_mfma_vs_plain"] = out["mfma_columns_per_s"] / out["plain_columns_per_s"]
+    out["corank_ok"] = all(out[f"{name}_corank"] == 64 for name, _ in methods)
+    buf.free()
+    return out
+
+
+def mixmax256_rank(ctx, n=65536):
+    """The README's MIXMAX-256-detecting rank test (a couple of hours on the
+    reference's CPU, README.md:110-113): an n x n matrix of consecutive
+    MIXMAX-256 (skip 2) seed-1 outputs over 2^61-1, built on the device; the
+    corank must be at least n - 65280 (every row is a window of a sequence of
+    linear complexity 65280).  One timed run."""
+    buf = W.mixmax256_matrix_device(ctx, n)
+    field = M.Field(M61)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    r = ctx.rank_batch(field, [buf], method=M.METHOD_MFMA)[0]
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    buf.free()
+    return {"n": n, "bytes": 8 * n * n, "rank": r, "corank": n - r, "s": dt, "columns_per_s": n / dt,
+            "corank_bound_ok": n - r >= n - 65280,
+            "p_value": M.pretty_p_value(M.corank_tail_pvalue(M61, n, n - r))}
+
+
+def c1_line(ctx, steps=20):
+    """The README's 500 x 500 MIXMAX-17 rank (README.md:61-71: 55,700.74 columns/s
+    on the reference's CPU, corank 432) on both GPU paths, with the oracle's port
+    on the host beside it (one thread and all threads)."""
+    mm, ctl = W.c1()
+    field = M.Field(M61)
+    dev = [ctx.to_device(mm)]
+    out = {"n": 500, "readme_cpu_columns_per_s": 55700.74}
+    for name, method in (("mfma", M.METHOD_MFMA), ("plain", M.METHOD_PLAIN)):
+        ranks, t, _, _ = run(ctx, field, dev, method, steps, 2)
+        out[f"{name}_columns_per_s"] = 500 / t
+        out[f"{name}_corank"] = 500 - ranks[0]
     dev[0].free()
+    O = _oracle()
+    cores = O.num_threads()
+    for label, th in (("cpu_port_1t", 1), ("cpu_port_all", cores)):
+        O.set_num_threads(th)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 1.0:
+            O.rank(M61, mm, 500)
+            reps += 1
+        out[f"{label}_columns_per_s"] = 500 * reps / (time.perf_counter() - t0)
+    O.set_num_threads(cores)
+    out["cpu_threads"] = cores
     return out
 
 
@@ -173,7 +227,8 @@ def bm_batch(ctx, steps):
     st = ctx.stats()
     fx = fixture("c4_lc.json")
     out = {"sequences": len(seqs), "steps": total, "steps_per_s": total / dt, "ms": dt * 1e3,
-           "kernel_ms": st["bm_ms"], "hbm_GBps": st["bm_bytes"] / (st["bm_ms"] / 1e3) / 1e9 if st["bm_ms"] else None,
+           "kernel_ms": st["bm_ms"],
+           "reference_algorithmic_GBps": st["bm_bytes"] / (st["bm_ms"] / 1e3) / 1e9 if st["bm_ms"] else None,
            "lc_match_fixture": (fx["lc"] == lcs) if fx else None}
     for d in dev:
         d.free()
@@ -200,17 +255,63 @@ def bm_long(ctx, steps):
 
 
 def rank_batch_u32(ctx, steps):
-    """256 matrices 2048 x 2048 over 65521, u32 storage (the small-prime batch)."""
-    mats = W.c5_rank(count=256)
+    """256 matrices 2048 x 2048 over 65521, u32 storage (the small-prime batch),
+    generated on the device."""
+    buf, views = W.c5_rank_device(ctx)
     field = M.Field(65521)
-    dev = [ctx.to_device(m) for m in mats]
-    ranks, t, st, _ = run(ctx, field, dev, M.METHOD_AUTO, steps, 1)
+    ranks, t, st, _ = run(ctx, field, views, M.METHOD_AUTO, steps, 1)
     fx = fixture("c5_rank_coranks.json")
-    for d in dev:
-        d.free()
+    buf.free()
+    L = st["limb_count"]
     return {"matrices": 256, "n": 2048, "p": 65521, "columns_per_s": 256 * 2048 / t, "ms": t * 1e3,
-            "update_ms": st["update_ms"], "limbs": st["limb_count"],
+            "update_ms": st["update_ms"], "limbs": L,
+            "update_frac": (2.0 * L * L * st["macs"] / (st["update_ms"] / 1e3) / 1e12 / PEAK_INT8_TOPS)
+            if st["update_ms"] else None,
             "coranks_match_fixture": ([2048 - r for r in ranks] == fx["coranks"]) if fx else None}
+
+
+def c5_mixed(steps=2):
+    """The mixed configuration: the 200,000-step MIXMAX-256 BM and the 256 x 2048^2
+    u32 rank batch over 65521 running at the same time, each on a context (stream)
+    of its own, driven from two host threads; wall time of the pair, against the
+    two run one after the other."""
+    import threading
+    cb, cr = M.Context(0), M.Context(0)
+    seq = cb.to_device(W.c5_bm(), square=False)
+    buf, views = W.c5_rank_device(cr)
+    fb, fr = M.Field(M61), M.Field(65521)
+    res = {}
+
+    def bm():
+        res["lc"] = cb.lc_batch(fb, [seq])[0]
+
+    def rk():
+        res["ranks"] = cr.rank_batch(fr, views)
+
+    bm(); rk()  # warm-up (allocations, graph capture)
+    cb.synchronize(); cr.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bm(); rk()
+    t_seq = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        th = [threading.Thread(target=bm), threading.Thread(target=rk)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+    t_par = (time.perf_counter() - t0) / steps
+    fx_r, fx_b = fixture("c5_rank_coranks.json"), fixture("c5_bm.json")
+    out = {"ms_concurrent": t_par * 1e3, "ms_one_after_other": t_seq * 1e3,
+           "bm_steps_per_s": 200_000 / t_par, "rank_columns_per_s": 256 * 2048 / t_par,
+           "lc": res["lc"], "lc_match_fixture": (fx_b["lc"] == res["lc"]) if fx_b else None,
+           "coranks_match_fixture": ([2048 - r for r in res["ranks"]] == fx_r["coranks"]) if fx_r else None}
+    seq.free()
+    buf.free()
+    cb.close()
+    cr.close()
+    return out
 
 
 def run(ctx, field, dev, method, steps, warmup):
@@ -239,7 +340,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the 8192^2 and Berlekamp-Massey lines")
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 256 MiB copy kernel")
     ap.add_argument("--full", action="store_true",
-                    help="also the 8192^2 matrix over 2^31-1, the 16384^2 matrices and the 256 x 2048^2 u32 batch")
+                    help="also the 32768^2 planted matrix and the 65536^2 MIXMAX-256 rank")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU only; --gpus above 1 is not supported "
@@ -303,14 +404,18 @@ def main():
     for d in dev:
         d.free()
     if not args.no_extras:
+        out["c1_500_mixmax17"] = c1_line(ctx)
         out["single_8192_p61"] = single_large(ctx, 8192, M61, 2)
-        out["bm_batch_1024"] = bm_batch(ctx, 2)
-        out["bm_long_200k"] = bm_long(ctx, 1)
-    if args.full:
         out["single_8192_p31"] = single_large(ctx, 8192, (1 << 31) - 1, 2)
         out["single_16384_p61"] = single_large(ctx, 16384, M61, 1)
         out["single_16384_p31"] = single_large(ctx, 16384, (1 << 31) - 1, 1)
+        out["bm_batch_1024"] = bm_batch(ctx, 2)
+        out["bm_long_200k"] = bm_long(ctx, 1)
         out["rank_batch_256x2048_u32_p65521"] = rank_batch_u32(ctx, 2)
+        out["c5_mixed_bm200k_rank256x2048"] = c5_mixed()
+    if args.full:
+        out["single_32768_p61"] = single_large(ctx, 32768, M61, 1, plain=False, warmup=0)
+        out["mixmax256_rank_65536"] = mixmax256_rank(ctx)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(ctx)
     if not args.no_cpu:
