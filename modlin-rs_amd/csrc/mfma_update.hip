@@ -347,35 +347,44 @@ __device__ __forceinline__ void limb_tile_mma_ahold(const v4i *af, const int8_t 
 }
 
 // Far part of the update, by row band: work item (row band, XCD group x)
-// updates the column tiles tc = x, x + 8, ... of that row band.  The row band's
-// A digit tile is staged in LDS once and stays there while the group's B digit
-// tiles stream through (the next one prefetched into registers), so each tile
-// stages 32 KiB instead of 64.  Items are numbered rows-major with the group
-// fastest; with a grid that is a multiple of 8, item t runs on XCD t mod 8 = x,
-// so an XCD's L2 holds its own column tiles.
+// updates the column tiles tc = x, x + 8, ... of that row band.  Items are
+// numbered rows-major with the group fastest; with a grid that is a multiple of
+// 8, item t runs on XCD t mod 8 = x, so an XCD's L2 holds its own column tiles.
 //
-// A workgroup is 4 waves; wave w owns rows 16 w .. 16 w + 15 of the 64 x 64 tile
-// and computes its four 16 x 16 column quarters one after the other.  Two
+// A workgroup is 4 waves; wave w owns rows 16 w .. 16 w + 15 of the band and
+// keeps its A digit fragments (L of them, read once per item straight from
+// HBM) in registers.  The columns go by in steps of 32: the B digit planes of a
+// step are staged in LDS (double-buffered, one barrier per step; the step after
+// next is fetched into registers meanwhile), and each wave computes the step's
+// two 16 x 16 column quarters together, so every digit plane lb feeds 2 L
+// independent accumulator chains (2 quarters x L classes) -- the 16x16x64
+// instruction needs about 16 chains in flight to run near its peak rate (one
+// quarter at a time for generic moduli of 7 or 8 digits, whose epilogue needs
+// the registers).  Two
 // workgroups share a CU and synchronise independently, so the matrix-core work
-// of one overlaps the epilogue (VALU) of the other.
+// of one overlaps the epilogue (VALU) of the other.  C values of a quarter are
+// fetched one step ahead, right after the quarter's own epilogue.
 constexpr uint32_t FAR_THREADS = 256;
+constexpr uint32_t FSTEP = 32;  // columns per step
 
 template <int L, int MODE>
 __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
                                                                const RankPlan *plan, FieldParams f, LimbParams lp,
                                                                uint32_t par, uint64_t *trash) {
     constexpr uint32_t KA = L * KSTEP;
-    constexpr uint32_t KAP = KA + 16;
+    constexpr uint32_t KAP = KA + 16;           // LDS row stride: consecutive columns start 4 banks apart
     constexpr int NC = 2 * L - 1;
-    constexpr uint32_t CH = KA / 16;
-    constexpr uint32_t NCH = UT * CH;
+    constexpr uint32_t CH = KA / 16;            // 16-byte chunks per column
+    constexpr uint32_t NCH = FSTEP * CH;        // chunks per step
     constexpr uint32_t NV = (NCH + FAR_THREADS - 1) / FAR_THREADS;
-    __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
+    constexpr int QW = (MODE == MODE_GENERIC && L >= 7) ? 1 : 2;  // quarters computed together
+    constexpr int H = (int)HOFF;
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][FSTEP * KAP];
     const uint32_t total = plan->far_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint32_t wr = wave * 16;
     const uint32_t koff = 16 * (lane >> 4);
+    const gptr<uint64_t> tr = to_global(trash) + tid;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
         // Everything the item needs, read once into registers: the stores below go
@@ -394,21 +403,20 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         const uint32_t C = cdiv(width, UT);
         if (pb == 0 || x >= C) continue;  // uniform
         const uint32_t r0 = band * UT, nr = min(UT, O - r0);
-        v4i rb[NV];
-        __syncthreads();  // the previous item's last tile is done with As
+        // Steps: tile x + 8 (j / 2), half j % 2; the last tile's second half only if
+        // it holds a column.
+        const uint32_t ntile = (C - x + 7) / 8;
+        const uint32_t last_cb = cbase + (x + 8 * (ntile - 1)) * UT;
+        const uint32_t nstep = 2 * ntile - (last_cb + FSTEP < cbase + width ? 0u : 1u);
+        auto step_cb = [&](uint32_t j) { return cbase + (x + 8 * (j >> 1)) * UT + FSTEP * (j & 1); };
+        // This wave's A fragments: row wr + (lane & 15), K bytes koff.. of each plane.
+        v4i af[L];
         {
-            const gptr<const v4i> ga = Alg + (size_t)r0 * (KA / 16);
+            const gptr<const v4i> ga = Alg + ((size_t)(r0 + wr + (lane & 15)) * KA + koff) / 16;
 #pragma unroll
-            for (uint32_t j = 0; j < NV; j++)
-                if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = ga[tid + j * FAR_THREADS];
-#pragma unroll
-            for (uint32_t j = 0; j < NV; j++) {
-                const uint32_t v = tid + j * FAR_THREADS, row = v / CH, ch = v - row * CH;
-                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(As + row * KAP + ch * 16) = rb[j];
-            }
+            for (int la = 0; la < L; la++) af[la] = ga[la * (KSTEP / 16)];
         }
         // This lane's rows; a row outside the band loads and stores through `tr`.
-        const gptr<uint64_t> tr = to_global(trash) + tid;
         gptr<uint64_t> mrow[4];
         bool rok[4];
 #pragma unroll
@@ -417,68 +425,87 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
             rok[r] = i < nr;
             mrow[r] = Mg + (size_t)permg[rank + r0 + min(i, nr - 1)] * ld + c0;
         }
-        {
-            const gptr<const v4i> gb = Xlg + (size_t)(cbase + x * UT) * (KA / 16);
+        v4i rb[NV];
+        auto fetch_b = [&](uint32_t j) {
+            const gptr<const v4i> gb = Xlg + (size_t)step_cb(j) * (KA / 16);
 #pragma unroll
-            for (uint32_t j = 0; j < NV; j++)
-                if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = gb[tid + j * FAR_THREADS];
-        }
-        __syncthreads();  // As is written
-        v4i af[L];  // this wave's A fragments: rows wr + (lane & 15), K bytes koff.. of each plane
+            for (uint32_t k = 0; k < NV; k++)
+                if (NCH % FAR_THREADS == 0 || tid + k * FAR_THREADS < NCH) rb[k] = gb[tid + k * FAR_THREADS];
+        };
+        auto stage_b = [&](uint32_t buf) {
 #pragma unroll
-        for (int la = 0; la < L; la++) af[la] = *(const v4i *)(As + (wr + (lane & 15)) * KAP + la * KSTEP + koff);
-        // C values of this lane's elements, one tile ahead: cvb[q][r] = row r, column
-        // 16 q + (lane & 15) of the tile; quarter q of the next tile is loaded as soon
-        // as quarter q of this one is done, so the loads have a whole tile to land.
-        uint64_t cvb[4][4];
-        {
-            const uint32_t cb = cbase + x * UT, nc = min(UT, cbase + width - cb);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t c = 16 * q + (lane & 15);
-#pragma unroll
-                for (int r = 0; r < 4; r++) cvb[q][r] = *((rok[r] && c < nc) ? mrow[r] + cb + c : tr);
+            for (uint32_t k = 0; k < NV; k++) {
+                const uint32_t v = tid + k * FAR_THREADS, row = v / CH, ch = v - row * CH;
+                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(&Bs[buf][row * KAP + ch * 16]) = rb[k];
             }
-        }
-        for (uint32_t tc = x; tc < C; tc += 8) {
-            const uint32_t cb = cbase + tc * UT, nc = min(UT, cbase + width - cb);
-            const uint32_t cbn = cb + 8 * UT;
-            const uint32_t ncn = tc + 8 < C ? min(UT, cbase + width - cbn) : 0;
-            __syncthreads();  // the previous tile is done with Bs
+        };
+        // C values of this lane's elements of step j: cv[q][r] = row r, column
+        // 16 q + (lane & 15) of the step.
+        uint64_t cv[2][4];
+        auto fetch_c = [&](uint32_t j, int q) {
+            const uint32_t cb = step_cb(j), c = 16 * q + (lane & 15);
+            const bool cok = cb + c < cbase + width;
 #pragma unroll
-            for (uint32_t j = 0; j < NV; j++) {
-                const uint32_t v = tid + j * FAR_THREADS, row = v / CH, ch = v - row * CH;
-                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
-            }
-            __syncthreads();
-            if (tc + 8 < C) {  // prefetch the next B digit tile
-                const gptr<const v4i> gb = Xlg + (size_t)(cbase + (tc + 8) * UT) * (KA / 16);
+            for (int r = 0; r < 4; r++) cv[q][r] = *((rok[r] && cok) ? mrow[r] + cb + c : tr);
+        };
+        fetch_b(0);
+        __syncthreads();  // the previous item is done with Bs
+        stage_b(0);
+        fetch_c(0, 0);
+        fetch_c(0, 1);
+        if (nstep > 1) fetch_b(1);
+        __syncthreads();
+        for (uint32_t j = 0; j < nstep; j++) {
+            const uint32_t buf = j & 1;
+            if (j + 1 < nstep) stage_b(buf ^ 1);   // its last reader, step j - 1, is past the barrier
+            if (j + 2 < nstep) fetch_b(j + 2);
+            const uint32_t cb = step_cb(j);
+            const int8_t *B0 = &Bs[buf][(lane & 15) * KAP + koff];
+            // acc[q][s] = init_s + sum_{la+lb=s} A_la B_lb over this wave's rows and
+            // quarter q's columns; init_s = HOFF for even classes, 0 for odd ones
+            // (limb.hpp), the C operand of each class's first product.  QW quarters
+            // at a time (QW = 1 where the generic epilogue needs the registers).
 #pragma unroll
-                for (uint32_t j = 0; j < NV; j++)
-                    if (NCH % FAR_THREADS == 0 || tid + j * FAR_THREADS < NCH) rb[j] = gb[tid + j * FAR_THREADS];
-            }
+            for (int q0 = 0; q0 < 2; q0 += QW) {
+                v4i acc[QW][NC];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t c = 16 * q + (lane & 15);
-                v4i acc[NC];
-                limb_tile_mma_ahold<L, KAP>(af, Bs, lane, 16 * q, acc);
-                // epilogue: C - sum_s 256^s P_s (mod p)
+                for (int lb = 0; lb < L; lb++) {
+                    v4i bq[QW];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    int32_t P[NC];
+                    for (int q = 0; q < QW; q++) bq[q] = *(const v4i *)(B0 + (q0 + q) * 16 * KAP + lb * KSTEP);
 #pragma unroll
-                    for (int k = 0; k < NC; k++) P[k] = acc[k][r];
-                    uint64_t v;
-                    if constexpr (MODE == MODE_M61 && NC == 15)
-                        v = limb_sub_m61(cvb[q][r], lp.kc, P);
-                    else
-                        v = sub_mod(add_mod(cvb[q][r], lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
-                    *((rok[r] && c < nc) ? mrow[r] + cb + c : tr) = v;
+                    for (int la = 0; la < L; la++) {
+                        const int s = la + lb;
+                        const bool first = (lb == 0) || (la == L - 1);  // loop order: lb outer, la inner
+                        const v4i i0 = (s & 1) ? (v4i){0, 0, 0, 0} : (v4i){H, H, H, H};
+#pragma unroll
+                        for (int q = 0; q < QW; q++)
+                            acc[q][s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[la], bq[q], first ? i0 : acc[q][s], 0, 0,
+                                                                              0);
+                    }
                 }
-                // quarter q of the next tile
+                // epilogue: C - sum_s 256^s P_s (mod p), then quarter q's C values of the next step
 #pragma unroll
-                for (int r = 0; r < 4; r++) cvb[q][r] = *((rok[r] && c < ncn) ? mrow[r] + cbn + c : tr);
+                for (int qq = 0; qq < QW; qq++) {
+                    const int q = q0 + qq;
+                    const uint32_t c = 16 * q + (lane & 15);
+                    const bool cok = cb + c < cbase + width;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        int32_t P[NC];
+#pragma unroll
+                        for (int k = 0; k < NC; k++) P[k] = acc[qq][k][r];
+                        uint64_t v;
+                        if constexpr (MODE == MODE_M61 && NC == 15)
+                            v = limb_sub_m61(cv[q][r], lp.kc, P);
+                        else
+                            v = sub_mod(add_mod(cv[q][r], lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
+                        *((rok[r] && cok) ? mrow[r] + cb + c : tr) = v;
+                    }
+                    if (j + 1 < nstep) fetch_c(j + 1, q);
+                }
             }
+            __syncthreads();  // Bs[buf ^ 1] is staged; Bs[buf] is free
         }
     }
 }
