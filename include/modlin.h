@@ -93,6 +93,8 @@ typedef struct modlin_seq_desc {
 #define MODLIN_FLAG_PROFILE 1u  /* time the dominant kernels with HIP events (direct launches) */
 #define MODLIN_FLAG_NO_GRAPH 2u /* launch each iteration block directly instead of replaying the
                                    context's captured hipGraph of it */
+#define MODLIN_FLAG_NO_DEFER 4u /* rank, MFMA method: update the far columns once per panel instead
+                                   of once per two panels (the same ranks) */
 
 typedef struct modlin_opts {
     uint32_t method; /* modlin_method.  BM: PLAIN runs one step per pass over c; AUTO and MFMA

@@ -19,7 +19,7 @@ namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L);
+                  uint32_t L, uint32_t defer);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
 void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
@@ -139,12 +139,15 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     const bool profile = opts && (opts->flags & MODLIN_FLAG_PROFILE);
     const uint32_t method = opts ? opts->method : (uint32_t)MODLIN_METHOD_AUTO;
     const bool use_mfma = method != MODLIN_METHOD_PLAIN;
+    // far updates of two panels at once (MFMA method), unless the caller asks for
+    // one per panel
+    const bool defer = use_mfma && !(opts && (opts->flags & MODLIN_FLAG_NO_DEFER));
     const LimbParams lp = make_limb_params(p);
     const size_t KA = (size_t)lp.L * KSTEP;
 
     // Workspace layout (256-byte aligned regions).
     std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offE(count), offS(count),
-        offAl(count), offXl(count), offZl(count);
+        offAl(count), offXl(count), offZl(count), offAlP(count), offXlP(count);
     size_t cur = 0;
     const size_t A = 256;
     cur = align_up(cur + count * sizeof(RankMat), A);
@@ -182,12 +185,18 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offE[i] = cur; cur = align_up(cur + ETA_WORDS * 8, A);
         offS[i] = 0;
         if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
-        offAl[i] = offXl[i] = offZl[i] = 0;
+        offAl[i] = offXl[i] = offZl[i] = offAlP[i] = offXlP[i] = 0;
         if (use_mfma) {
+            // digit columns are read in whole 64-column tiles: each X buffer is
+            // followed by 64 columns' worth of slack
             const size_t npad = (n + UT - 1) / UT * UT;
             offAl[i] = cur; cur = align_up(cur + npad * KA, A);
             offXl[i] = cur; cur = align_up(cur + npad * KA, A);
             offZl[i] = cur; cur = align_up(cur + BMAX * KA, A);
+            if (defer) {
+                offAlP[i] = cur; cur = align_up(cur + npad * KA, A);
+                offXlP[i] = cur; cur = align_up(cur + (npad + UT) * KA, A);
+            }
         }
     }
     int rc = arena_reserve(ctx, cur);
@@ -210,6 +219,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.Al = use_mfma ? (int8_t *)(base + offAl[i]) : nullptr;
         S.Xl = use_mfma ? (int8_t *)(base + offXl[i]) : nullptr;
         S.Zl = use_mfma ? (int8_t *)(base + offZl[i]) : nullptr;
+        S.AlP = defer ? (int8_t *)(base + offAlP[i]) : nullptr;
+        S.XlP = defer ? (int8_t *)(base + offXlP[i]) : nullptr;
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
         S.done = (n == 0) ? 1u : 0u;
@@ -270,7 +281,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             const uint32_t par = (uint32_t)(s & 1);
             hipError_t e;
             if ((e = hipStreamWaitEvent(aux, ctx->ev_fin, 0)) != hipSuccess) return e;
-            launch_block(aux, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u);
+            launch_block(aux, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
             if ((e = hipEventRecord(ctx->ev_ge, aux)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(st, ctx->ev_ge, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);
@@ -299,8 +310,9 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     bool use_graph = !profile && !(opts && (opts->flags & MODLIN_FLAG_NO_GRAPH));
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof key, "%p/%u/%u/%d/%llu/%u/%u/%u/%u/%d", (void *)base, nmat, b, (int)use_mfma,
-                 (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L, check_every);
+        snprintf(key, sizeof key, "%p/%u/%u/%d/%d/%llu/%u/%u/%u/%u/%d", (void *)base, nmat, b, (int)use_mfma,
+                 (int)defer, (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L,
+                 check_every);
         if (!ctx->rank_graph || ctx->rank_graph_key != key) {
             if (ctx->rank_graph) {
                 hipGraphExecDestroy(ctx->rank_graph);

@@ -31,11 +31,20 @@ namespace mdl {
 
 constexpr uint32_t HOFF = 2139029504u;  // 8 pairs * 64 terms * 16256 * 257
 
+// Two panels accumulated together (K <= 128, the far update of a deferred
+// pair of panels): a class sum then lies in [-2^24, 2^24] (8 pairs x 128 terms
+// x 2^14), too wide for the pair words.  Every class accumulator instead starts
+// at KOFF2 = 2^25 and stays in [2^24, 3 2^24], a non-negative 26-bit value, so
+// S' = sum_s 256^s P'_s = sum_s 256^s P_s + k2 with k2 = KOFF2 sum_s 256^s mod p.
+constexpr int32_t KOFF2 = 1 << 25;
+
 struct LimbParams {
     uint64_t pow256[16];  // 256^l mod p
     uint64_t pow32[5];    // 2^(32 i) mod p: weights of the recombined 32-bit words
     uint64_t k0;          // sum_{t < L} HOFF 2^(16 t) mod p (the pair words' offsets)
     uint64_t kc;          // k0 + 2p (limb_sub_m61)
+    uint64_t k2;          // sum_{s < 2L-1} KOFF2 256^s mod p (the offsets of limb_residue_k2)
+    uint64_t kc2;         // k2 + 2p (limb_sub_m61_k2)
     uint32_t L;           // digits per element
     uint32_t pad;
 };
@@ -73,6 +82,10 @@ inline LimbParams make_limb_params(uint64_t p) {
     }
     lp.k0 = (uint64_t)k0;
     lp.kc = (uint64_t)(k0 + 2 * (u128)p);
+    u128 k2 = 0;
+    for (uint32_t c = 0; c + 1 < 2 * lp.L; c++) k2 = (k2 + (u128)KOFF2 % p * lp.pow256[c]) % p;
+    lp.k2 = (uint64_t)k2;
+    lp.kc2 = (uint64_t)(k2 + 2 * (u128)p);
     return lp;
 }
 
@@ -180,6 +193,59 @@ MDL_HD uint64_t limb_sub_m61(uint64_t cv, uint64_t kc, const int32_t *acc) {
     const uint32_t yh = (xh & 0x1fffffffu) + c;
     // r = cv + k0 + 2p - y in [p - 3, 4p), congruent to cv - (S - k0)
     uint64_t r = cv + kc - (((uint64_t)yh << 32) | yl);
+    r = (r & P61) + (r >> 61);  // <= p + 3
+    return r >= P61 ? r - P61 : r;
+}
+
+// Residue mod p of S' = sum_s 256^s P'_s from the class accumulators of two
+// accumulated panels (every class started at KOFF2, so each P'_s lies in
+// [0, 2^26)); the caller subtracts k2.  Returns the canonical residue.
+template <int MODE, int NC>
+MDL_HD uint64_t limb_residue_k2(const FieldParams &f, const LimbParams &lp, const int32_t *acc) {
+    if constexpr (MODE == MODE_M31 && NC == 7) {
+        // 256^s == 2^(8 s mod 31): exponents 0, 8, 16, 24, 1, 9, 17; the sum is below 2^53
+        uint64_t x = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) x += (uint64_t)(uint32_t)acc[c] << ((8 * c) % 31);
+        return fold_m31(0, x);
+    }
+    if constexpr (NC <= 5) {
+        // S' < 2^26 (1 + 2^8 + ... + 2^32) < 2^59 fits one word
+        uint64_t x = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) x += (uint64_t)(uint32_t)acc[c] << (8 * c);
+        return reduce128(f, 0, x);
+    }
+    // weights 256^s mod p: each product is below 2^89, the sum below 2^93
+    uint64_t th = 0, tl = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        uint64_t ph, pl;
+        mul_wide((uint64_t)(uint32_t)acc[c], lp.pow256[c], ph, pl);
+        add128(th, tl, ph, pl);
+    }
+    return reduce128(f, th, tl);
+}
+
+// (cv - sum_s 256^s P_s) mod 2^61 - 1 for cv in [0, p), from the 15 class
+// accumulators of two accumulated panels of L = 8 digits (every class started
+// at KOFF2); kc2 = k2 + 2p.  Class s has weight 2^(8 s mod 61): classes 0-4 and
+// 8-12 (exponents up to 35) stay below 2^61, classes 5-7 and 13-14 are rotated
+// (the bits from 61 up come back at bit 0).  The 15 terms add up below 2^64.
+MDL_HD uint64_t limb_sub_m61_k2(uint64_t cv, uint64_t kc2, const int32_t *acc) {
+    uint64_t x = 0;
+#pragma unroll
+    for (int c = 0; c < 15; c++) {
+        const uint64_t v = (uint32_t)acc[c];
+        const int e = (8 * c) % 61;
+        if (e + 26 <= 61)
+            x += v << e;
+        else
+            x += ((v << e) & P61) + (v >> (61 - e));
+    }
+    const uint64_t y = (x & P61) + (x >> 61);  // <= p + 7, congruent to S'
+    // r = cv + k2 + 2p - y in [p - 7, 4p), congruent to cv - (S' - k2)
+    uint64_t r = cv + kc2 - y;
     r = (r & P61) + (r >> 61);  // <= p + 3
     return r >= P61 ? r - P61 : r;
 }

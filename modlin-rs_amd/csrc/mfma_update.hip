@@ -127,10 +127,18 @@ __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs
 }
 
 // Digit planes of the two update operands.
-//  gather task: Al[i][la*64 + k] = digit la of M[perm[rank + i]][J[k]] (zero for k >= pb)
+//  gather task: Al[i][la*64 + k] = digit la of M[perm[rank + i]][J[k]] (zero for k >= pb);
+//               for a PAIR iteration also AlP[i][la*64 + k] = digit la of
+//               M[perm[rank + i]][J'[k]], J' the pending panel's pivot columns.
 //  X task:      Xl[c][lb*64 + k] = digit lb of X[k][c], X = Z M[R, c0:] mod p (the
 //               reference's U-block solve, fp.rs:180-200), computed as a limb product
 //               on the matrix cores: Z and the pivot rows are split into digits in LDS.
+//               A DEFER iteration writes XlP instead.  In a PAIR iteration the
+//               pivot rows still lack the pending panel's update in the far
+//               columns c >= dcol, so there X = Z Y with
+//                 Y[s][c] = M[R[s]][c] - sum_a M[R[s]][J'[a]] X'[a][c]  (mod p),
+//               X' the pending panel's X (digits in XlP): a first limb product
+//               forms Y, whose digits then take the pivot rows' place.
 template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
                                                       const RankPlan *plan, FieldParams f, LimbParams lp,
@@ -154,44 +162,106 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
         if (lt < ng) {
             const uint32_t r0 = lt * GATHER_ROWS;
             const uint32_t rows = min(GATHER_ROWS, O - r0);
-            for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
-                const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
-                const uint64_t *mrow = S.M + (size_t)S.perm[I.rank + r0 + i] * ld;
-                uint64_t x[4];
+            const bool pair = I.kind == IT_PAIR;
+            // own panel's columns (when it has pivots), then the pending panel's (PAIR)
+            for (uint32_t pass = pb > 0 ? 0u : 1u; pass < (pair ? 2u : 1u); pass++) {
+                const uint32_t *Jc = pass ? I.dJ : I.J;
+                const uint32_t kb = pass ? I.dpb : pb;
+                int8_t *out = pass ? S.AlP : S.Al;
+                for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
+                    const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
+                    const uint64_t *mrow = S.M + (size_t)S.perm[I.rank + r0 + i] * ld;
+                    uint64_t x[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) x[j] = (k4 + j < pb) ? mrow[I.J[k4 + j]] : 0;
-                uint32_t w[L];
-                digits4<L>(x, p, w);
-                uint32_t *dst = (uint32_t *)(S.Al + (size_t)(r0 + i) * KA + k4);
+                    for (int j = 0; j < 4; j++) x[j] = (k4 + j < kb) ? mrow[Jc[k4 + j]] : 0;
+                    uint32_t w[L];
+                    digits4<L>(x, p, w);
+                    uint32_t *dst = (uint32_t *)(out + (size_t)(r0 + i) * KA + k4);
 #pragma unroll
-                for (int la = 0; la < L; la++) dst[la * (KSTEP / 4)] = w[la];
+                    for (int la = 0; la < L; la++) dst[la * (KSTEP / 4)] = w[la];
+                }
             }
             continue;
         }
         const uint32_t cb = (lt - ng) * XCOLS;
         const uint32_t cols = min(XCOLS, I.cend - I.c0 - cb);
+        // uniform: the task has columns the pending panel has not updated
+        const bool corr = I.kind == IT_PAIR && I.c0 + cb + XCOLS > I.dcol;
         __syncthreads();
-        // As[k][la*64 + s] = digit la of Z[k][s] (split once by k_block_z);
-        // Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
-        for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
-            const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
-            *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
-        }
-        for (uint32_t idx = tid; idx < XCOLS * (KSTEP / 4); idx += blockDim.x) {
-            const uint32_t c = idx % XCOLS, s4 = (idx / XCOLS) * 4;
-            uint64_t x[4];
+        if (corr) {
+            // As[s][la*64 + a] = digit la of M[R[s]][J'[a]]; Bs[c][lb*64 + a] = digit lb of
+            // X'[a][c] for the columns c0 + cb + c >= dcol (zero for the others)
+            const uint32_t dpb = I.dpb;
+            for (uint32_t idx = tid; idx < UT * (KSTEP / 4); idx += blockDim.x) {
+                const uint32_t sl = idx / (KSTEP / 4), a4 = (idx - sl * (KSTEP / 4)) * 4;
+                uint64_t x[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)I.R[s4 + j] * ld + I.c0 + cb + c] : 0;
-            uint32_t w[L];
-            digits4<L>(x, p, w);
+                for (int j = 0; j < 4; j++)
+                    x[j] = (sl < pb && a4 + j < dpb) ? S.M[(size_t)I.R[sl] * ld + I.dJ[a4 + j]] : 0;
+                uint32_t w[L];
+                digits4<L>(x, p, w);
 #pragma unroll
-            for (int lb = 0; lb < L; lb++) *(uint32_t *)(Bs + c * KAP + lb * KSTEP + s4) = w[lb];
+                for (int la = 0; la < L; la++) *(uint32_t *)(As + sl * KAP + la * KSTEP + a4) = w[la];
+            }
+            for (uint32_t idx = tid; idx < XCOLS * (KA / 16); idx += blockDim.x) {
+                const uint32_t c = idx / (KA / 16), ch = idx - c * (KA / 16);
+                const bool live = c < cols && I.c0 + cb + c >= I.dcol;
+                *(v4i *)(Bs + c * KAP + ch * 16) =
+                    live ? ((const v4i *)(S.XlP + (size_t)(cb + c + I.dxoff) * KA))[ch] : (v4i){0, 0, 0, 0};
+            }
+            __syncthreads();
+            v4i acc[NC][2];
+            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+            // Y = M[R[s]][c] - product (mod p) for this lane's 4 slots and 2 columns
+            uint64_t y[2][4];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const uint32_t c = wc + 16 * b + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t sl = wr + 4 * (lane >> 4) + r;
+                    const uint64_t pr = sub_mod(acc_residue<MODE, NC>(f, lp, acc, b, r), lp.k0, p);
+                    y[b][r] = (c < cols && sl < pb) ? sub_mod(S.M[(size_t)I.R[sl] * ld + I.c0 + cb + c], pr, p) : 0;
+                }
+            }
+            __syncthreads();  // every wave is done with As and Bs
+            // Bs[c][lb*64 + s] = digit lb of Y[s][c]; As = digits of Z
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const uint32_t c = wc + 16 * b + (lane & 15), s4 = wr + 4 * (lane >> 4);
+                uint32_t w[L];
+                digits4<L>(y[b], p, w);
+#pragma unroll
+                for (int lb = 0; lb < L; lb++) *(uint32_t *)(Bs + c * KAP + lb * KSTEP + s4) = w[lb];
+            }
+            for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
+                const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
+                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
+            }
+        } else {
+            // As[k][la*64 + s] = digit la of Z[k][s] (split once by k_block_z);
+            // Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
+            for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
+                const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
+                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
+            }
+            for (uint32_t idx = tid; idx < XCOLS * (KSTEP / 4); idx += blockDim.x) {
+                const uint32_t c = idx % XCOLS, s4 = (idx / XCOLS) * 4;
+                uint64_t x[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)I.R[s4 + j] * ld + I.c0 + cb + c] : 0;
+                uint32_t w[L];
+                digits4<L>(x, p, w);
+#pragma unroll
+                for (int lb = 0; lb < L; lb++) *(uint32_t *)(Bs + c * KAP + lb * KSTEP + s4) = w[lb];
+            }
         }
         __syncthreads();
         v4i acc[NC][2];
         limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
         // X = sum_s 256^s P_s mod p, then its digits (4 consecutive k per lane and column)
+        int8_t *xl = own_xl(S, I);
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const uint32_t c = wc + 16 * b + (lane & 15);
@@ -203,7 +273,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 digits4<L>(x, p, dw);
                 const uint32_t k4 = wr + 4 * (lane >> 4);
 #pragma unroll
-                for (int lb = 0; lb < L; lb++) *(uint32_t *)(S.Xl + (size_t)(cb + c) * KA + lb * KSTEP + k4) = dw[lb];
+                for (int lb = 0; lb < L; lb++) *(uint32_t *)(xl + (size_t)(cb + c) * KA + lb * KSTEP + k4) = dw[lb];
             }
         }
     }
@@ -234,7 +304,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
         if (cur.pb) {
             const RankMat &S = mats[cur.mi];
             const v4i *ga = (const v4i *)(S.Al + (size_t)cur.r0 * KA);
-            const v4i *gb = (const v4i *)(S.Xl + (size_t)cur.cb * KA);
+            const v4i *gb = (const v4i *)(own_xl(S, S.it[par]) + (size_t)cur.cb * KA);
 #pragma unroll
             for (uint32_t j = 0; j < NV; j++) {
                 if (NCH % 512 == 0 || tid + j * 512 < NCH) {
@@ -278,7 +348,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
             if (nxt.pb) {
                 const RankMat &Sn = mats[nxt.mi];
                 const v4i *ga = (const v4i *)(Sn.Al + (size_t)nxt.r0 * KA);
-                const v4i *gb = (const v4i *)(Sn.Xl + (size_t)nxt.cb * KA);
+                const v4i *gb = (const v4i *)(own_xl(Sn, Sn.it[par]) + (size_t)nxt.cb * KA);
 #pragma unroll
                 for (uint32_t j = 0; j < NV; j++) {
                     if (NCH % 512 == 0 || tid + j * 512 < NCH) {
@@ -351,162 +421,215 @@ __device__ __forceinline__ void limb_tile_mma_ahold(const v4i *af, const int8_t 
 // numbered rows-major with the group fastest; with a grid that is a multiple of
 // 8, item t runs on XCD t mod 8 = x, so an XCD's L2 holds its own column tiles.
 //
+// A PAIR iteration's far part carries two panels: the pending one (digits AlP
+// of M[O, J'], XlP of its X, offset by dxoff columns) and its own (Al, Xl), up
+// to 128 pivots accumulated together -- each far element is read and written
+// once per two panels.  Every class accumulator starts at KOFF2 and the
+// recombination is limb_residue_k2 / limb_sub_m61_k2 (exact for K <= 128), for
+// single panels too.
+//
 // A workgroup is 4 waves; wave w owns rows 16 w .. 16 w + 15 of the band and
-// keeps its A digit fragments (L of them, read once per item straight from
+// keeps its A digit fragments (L per panel, read once per item straight from
 // HBM) in registers.  The columns go by in steps of 32: the B digit planes of a
-// step are staged in LDS (double-buffered, one barrier per step; the step after
-// next is fetched into registers meanwhile), and each wave computes the step's
-// two 16 x 16 column quarters together, so every digit plane lb feeds 2 L
-// independent accumulator chains (2 quarters x L classes) -- the 16x16x64
-// instruction needs about 16 chains in flight to run near its peak rate (one
-// quarter at a time for generic moduli of 7 or 8 digits, whose epilogue needs
-// the registers).  Two
-// workgroups share a CU and synchronise independently, so the matrix-core work
-// of one overlaps the epilogue (VALU) of the other.  C values of a quarter are
-// fetched one step ahead, right after the quarter's own epilogue.
+// step (one tile per panel) are staged in LDS, double-buffered with one barrier
+// per step: the next step's tiles are fetched into registers at the start of a
+// step and stored once its first products are issued.  Each wave
+// computes QW 16 x 16 column quarters at a time, digit plane lb outer and la
+// inner, so consecutive products go to different classes.  Two workgroups share
+// a CU and synchronise independently, so the matrix-core work of one overlaps
+// the epilogue (VALU) of the other.  C values of a quarter are fetched one step
+// ahead, right after the quarter's own epilogue.
 constexpr uint32_t FAR_THREADS = 256;
 constexpr uint32_t FSTEP = 32;  // columns per step
+
+// Everything an item of the far update needs, read once into registers: the
+// stores go through global pointers that the compiler cannot tell apart from
+// the RankMat array.
+struct FarItem {
+    gptr<uint64_t> Mg;
+    gptr<const uint32_t> permg;
+    gptr<const v4i> Ag[2], Xg[2];  // panel h: 0 the pending one, 1 this iteration's own
+    uint32_t x, r0, nr, rank, ld, cbase, width, C;
+};
+
+// One item with the panels H0 .. H1 (compile-time, so every product's
+// accumulator operand is known).
+template <int L, int MODE, int H0, int H1>
+__device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const FieldParams &f, const LimbParams &lp,
+                                         gptr<uint64_t> tr, uint32_t tid, uint32_t lane, uint32_t wr) {
+    constexpr uint32_t KA = L * KSTEP;
+    constexpr uint32_t KAP = KA + 16;           // LDS row stride: consecutive columns start 4 banks apart
+    constexpr int NC = 2 * L - 1;
+    constexpr uint32_t CH = KA / 16;            // 16-byte chunks per column
+    constexpr uint32_t NCH = FSTEP * CH;        // chunks per panel tile of a step
+    constexpr uint32_t NV = (NCH + FAR_THREADS - 1) / FAR_THREADS;
+    constexpr uint32_t TILE = FSTEP * KAP;      // LDS bytes of one panel tile
+    constexpr uint32_t BUF = 2 * TILE;
+    constexpr int QW = L >= 5 ? 1 : 2;          // quarters computed together
+    const uint32_t koff = 16 * (lane >> 4);
+    const uint32_t cbase = it.cbase, width = it.width, x = it.x;
+    // Steps: tile x + 8 (j / 2), half j % 2; the last tile's second half only if
+    // it holds a column.
+    const uint32_t ntile = (it.C - x + 7) / 8;
+    const uint32_t last_cb = cbase + (x + 8 * (ntile - 1)) * UT;
+    const uint32_t nstep = 2 * ntile - (last_cb + FSTEP < cbase + width ? 0u : 1u);
+    auto step_cb = [&](uint32_t j) { return cbase + (x + 8 * (j >> 1)) * UT + FSTEP * (j & 1); };
+    // This wave's A fragments: row wr + (lane & 15), K bytes koff.. of each plane.
+    v4i af[H1 - H0 + 1][L];
+#pragma unroll
+    for (int h = H0; h <= H1; h++) {
+        const gptr<const v4i> ga = it.Ag[h] + ((size_t)(it.r0 + wr + (lane & 15)) * KA + koff) / 16;
+#pragma unroll
+        for (int la = 0; la < L; la++) af[h - H0][la] = ga[la * (KSTEP / 16)];
+    }
+    // This lane's rows; a row outside the band loads and stores through `tr`.
+    gptr<uint64_t> mrow[4];
+    bool rok[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t i = wr + 4 * (lane >> 4) + r;
+        rok[r] = i < it.nr;
+        mrow[r] = it.Mg + (size_t)it.permg[it.rank + it.r0 + min(i, it.nr - 1)] * it.ld;
+    }
+    v4i rb[H1 - H0 + 1][NV];
+    auto fetch_b = [&](uint32_t j) {
+#pragma unroll
+        for (int h = H0; h <= H1; h++) {
+            const gptr<const v4i> gb = it.Xg[h] + (size_t)step_cb(j) * (KA / 16);
+#pragma unroll
+            for (uint32_t k = 0; k < NV; k++)
+                if (NCH % FAR_THREADS == 0 || tid + k * FAR_THREADS < NCH) rb[h - H0][k] = gb[tid + k * FAR_THREADS];
+        }
+    };
+    auto stage_b = [&](uint32_t buf) {
+#pragma unroll
+        for (int h = H0; h <= H1; h++) {
+#pragma unroll
+            for (uint32_t k = 0; k < NV; k++) {
+                const uint32_t v = tid + k * FAR_THREADS, row = v / CH, ch = v - row * CH;
+                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(Bs + buf * BUF + h * TILE + row * KAP + ch * 16) = rb[h - H0][k];
+            }
+        }
+    };
+    // C values of this lane's elements of step j: cv[q][r] = row r, column
+    // 16 q + (lane & 15) of the step.
+    uint64_t cv[2][4];
+    auto fetch_c = [&](uint32_t j, int q) {
+        const uint32_t cb = step_cb(j), c = 16 * q + (lane & 15);
+        const bool cok = cb + c < cbase + width;
+#pragma unroll
+        for (int r = 0; r < 4; r++) cv[q][r] = *((rok[r] && cok) ? mrow[r] + cb + c : tr);
+    };
+    fetch_b(0);
+    __syncthreads();  // the previous item is done with Bs
+    stage_b(0);
+    fetch_c(0, 0);
+    fetch_c(0, 1);
+    __syncthreads();
+    for (uint32_t j = 0; j < nstep; j++) {
+        const uint32_t buf = j & 1;
+        // the next step's tiles: fetched now, staged once the first quarters'
+        // products are issued (their buffer's last reader, step j - 1, is past the
+        // barrier), so that the registers are free again for the epilogue.  The
+        // last step fetches and stages its own tiles again into the idle buffer
+        // (no branch, so the compiler keeps the stores after the products).
+        const uint32_t jn = j + 1 < nstep ? j + 1 : j;
+        fetch_b(jn);
+        __builtin_amdgcn_sched_barrier(0);  // the loads go out before the products
+        const uint32_t cb = step_cb(j);
+        const int8_t *B0 = Bs + buf * BUF + (lane & 15) * KAP + koff;
+#pragma unroll
+        for (int q0 = 0; q0 < 2; q0 += QW) {
+            // acc[q][s] = KOFF2 + sum over the panels of sum_{la+lb=s} A_la B_lb for
+            // this wave's rows and quarter q0 + q's columns
+            v4i acc[QW][NC];
+#pragma unroll
+            for (int h = H0; h <= H1; h++) {
+#pragma unroll
+                for (int lb = 0; lb < L; lb++) {
+                    v4i bq[QW];
+#pragma unroll
+                    for (int q = 0; q < QW; q++) bq[q] = *(const v4i *)(B0 + h * TILE + (q0 + q) * 16 * KAP + lb * KSTEP);
+#pragma unroll
+                    for (int la = 0; la < L; la++) {
+                        const int s = la + lb;
+                        // first product of class s: lb outer, la inner, first panel
+                        const bool first = h == H0 && ((lb == 0) || (la == L - 1));
+#pragma unroll
+                        for (int q = 0; q < QW; q++)
+                            acc[q][s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                af[h - H0][la], bq[q], first ? (v4i){KOFF2, KOFF2, KOFF2, KOFF2} : acc[q][s], 0, 0, 0);
+                    }
+                }
+            }
+            if (q0 == 0) {
+                __builtin_amdgcn_sched_barrier(0);
+                stage_b(buf ^ 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // epilogue: C - sum_s 256^s P_s (mod p), then quarter q's C values of the next step
+#pragma unroll
+            for (int qq = 0; qq < QW; qq++) {
+                const int q = q0 + qq;
+                const uint32_t c = 16 * q + (lane & 15);
+                const bool cok = cb + c < cbase + width;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int32_t P[NC];
+#pragma unroll
+                    for (int k = 0; k < NC; k++) P[k] = acc[qq][k][r];
+                    uint64_t v;
+                    if constexpr (MODE == MODE_M61 && NC == 15)
+                        v = limb_sub_m61_k2(cv[q][r], lp.kc2, P);
+                    else
+                        v = sub_mod(add_mod(cv[q][r], lp.k2, f.p), limb_residue_k2<MODE, NC>(f, lp, P), f.p);
+                    *((rok[r] && cok) ? mrow[r] + cb + c : tr) = v;
+                }
+                fetch_c(jn, q);
+            }
+        }
+        __syncthreads();  // the next buffer is staged; this one is free
+    }
+}
 
 template <int L, int MODE>
 __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
                                                                const RankPlan *plan, FieldParams f, LimbParams lp,
                                                                uint32_t par, uint64_t *trash) {
     constexpr uint32_t KA = L * KSTEP;
-    constexpr uint32_t KAP = KA + 16;           // LDS row stride: consecutive columns start 4 banks apart
-    constexpr int NC = 2 * L - 1;
-    constexpr uint32_t CH = KA / 16;            // 16-byte chunks per column
-    constexpr uint32_t NCH = FSTEP * CH;        // chunks per step
-    constexpr uint32_t NV = (NCH + FAR_THREADS - 1) / FAR_THREADS;
-    constexpr int QW = (MODE == MODE_GENERIC && L >= 7) ? 1 : 2;  // quarters computed together
-    constexpr int H = (int)HOFF;
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][FSTEP * KAP];
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2 * 2 * FSTEP * (KA + 16)];  // 2 buffers x 2 panel tiles
     const uint32_t total = plan->far_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
-    const uint32_t wr = wave * 16;
-    const uint32_t koff = 16 * (lane >> 4);
     const gptr<uint64_t> tr = to_global(trash) + tid;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
-        // Everything the item needs, read once into registers: the stores below go
-        // through global pointers that the compiler cannot tell apart from *mats.
         const RankMat &S = mats[mi];
         const IterState &I = S.it[par];
         const uint32_t lt = item - item_off[mi];
-        const uint32_t x = lt & 7, band = lt >> 3;
-        const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld, rank = I.rank, c0 = I.c0;
-        const gptr<uint64_t> Mg = to_global(S.M);
-        const gptr<const uint32_t> permg = to_global((const uint32_t *)S.perm);
-        const gptr<const v4i> Alg = to_global((const v4i *)S.Al);
-        const gptr<const v4i> Xlg = to_global((const v4i *)S.Xl);
-        uint32_t cbase, width;
-        update_range(I, 1, cbase, width);
-        const uint32_t C = cdiv(width, UT);
-        if (pb == 0 || x >= C) continue;  // uniform
-        const uint32_t r0 = band * UT, nr = min(UT, O - r0);
-        // Steps: tile x + 8 (j / 2), half j % 2; the last tile's second half only if
-        // it holds a column.
-        const uint32_t ntile = (C - x + 7) / 8;
-        const uint32_t last_cb = cbase + (x + 8 * (ntile - 1)) * UT;
-        const uint32_t nstep = 2 * ntile - (last_cb + FSTEP < cbase + width ? 0u : 1u);
-        auto step_cb = [&](uint32_t j) { return cbase + (x + 8 * (j >> 1)) * UT + FSTEP * (j & 1); };
-        // This wave's A fragments: row wr + (lane & 15), K bytes koff.. of each plane.
-        v4i af[L];
-        {
-            const gptr<const v4i> ga = Alg + ((size_t)(r0 + wr + (lane & 15)) * KA + koff) / 16;
-#pragma unroll
-            for (int la = 0; la < L; la++) af[la] = ga[la * (KSTEP / 16)];
-        }
-        // This lane's rows; a row outside the band loads and stores through `tr`.
-        gptr<uint64_t> mrow[4];
-        bool rok[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const uint32_t i = wr + 4 * (lane >> 4) + r;
-            rok[r] = i < nr;
-            mrow[r] = Mg + (size_t)permg[rank + r0 + min(i, nr - 1)] * ld + c0;
-        }
-        v4i rb[NV];
-        auto fetch_b = [&](uint32_t j) {
-            const gptr<const v4i> gb = Xlg + (size_t)step_cb(j) * (KA / 16);
-#pragma unroll
-            for (uint32_t k = 0; k < NV; k++)
-                if (NCH % FAR_THREADS == 0 || tid + k * FAR_THREADS < NCH) rb[k] = gb[tid + k * FAR_THREADS];
-        };
-        auto stage_b = [&](uint32_t buf) {
-#pragma unroll
-            for (uint32_t k = 0; k < NV; k++) {
-                const uint32_t v = tid + k * FAR_THREADS, row = v / CH, ch = v - row * CH;
-                if (NCH % FAR_THREADS == 0 || v < NCH) *(v4i *)(&Bs[buf][row * KAP + ch * 16]) = rb[k];
-            }
-        };
-        // C values of this lane's elements of step j: cv[q][r] = row r, column
-        // 16 q + (lane & 15) of the step.
-        uint64_t cv[2][4];
-        auto fetch_c = [&](uint32_t j, int q) {
-            const uint32_t cb = step_cb(j), c = 16 * q + (lane & 15);
-            const bool cok = cb + c < cbase + width;
-#pragma unroll
-            for (int r = 0; r < 4; r++) cv[q][r] = *((rok[r] && cok) ? mrow[r] + cb + c : tr);
-        };
-        fetch_b(0);
-        __syncthreads();  // the previous item is done with Bs
-        stage_b(0);
-        fetch_c(0, 0);
-        fetch_c(0, 1);
-        if (nstep > 1) fetch_b(1);
-        __syncthreads();
-        for (uint32_t j = 0; j < nstep; j++) {
-            const uint32_t buf = j & 1;
-            if (j + 1 < nstep) stage_b(buf ^ 1);   // its last reader, step j - 1, is past the barrier
-            if (j + 2 < nstep) fetch_b(j + 2);
-            const uint32_t cb = step_cb(j);
-            const int8_t *B0 = &Bs[buf][(lane & 15) * KAP + koff];
-            // acc[q][s] = init_s + sum_{la+lb=s} A_la B_lb over this wave's rows and
-            // quarter q's columns; init_s = HOFF for even classes, 0 for odd ones
-            // (limb.hpp), the C operand of each class's first product.  QW quarters
-            // at a time (QW = 1 where the generic epilogue needs the registers).
-#pragma unroll
-            for (int q0 = 0; q0 < 2; q0 += QW) {
-                v4i acc[QW][NC];
-#pragma unroll
-                for (int lb = 0; lb < L; lb++) {
-                    v4i bq[QW];
-#pragma unroll
-                    for (int q = 0; q < QW; q++) bq[q] = *(const v4i *)(B0 + (q0 + q) * 16 * KAP + lb * KSTEP);
-#pragma unroll
-                    for (int la = 0; la < L; la++) {
-                        const int s = la + lb;
-                        const bool first = (lb == 0) || (la == L - 1);  // loop order: lb outer, la inner
-                        const v4i i0 = (s & 1) ? (v4i){0, 0, 0, 0} : (v4i){H, H, H, H};
-#pragma unroll
-                        for (int q = 0; q < QW; q++)
-                            acc[q][s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[la], bq[q], first ? i0 : acc[q][s], 0, 0,
-                                                                              0);
-                    }
-                }
-                // epilogue: C - sum_s 256^s P_s (mod p), then quarter q's C values of the next step
-#pragma unroll
-                for (int qq = 0; qq < QW; qq++) {
-                    const int q = q0 + qq;
-                    const uint32_t c = 16 * q + (lane & 15);
-                    const bool cok = cb + c < cbase + width;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        int32_t P[NC];
-#pragma unroll
-                        for (int k = 0; k < NC; k++) P[k] = acc[qq][k][r];
-                        uint64_t v;
-                        if constexpr (MODE == MODE_M61 && NC == 15)
-                            v = limb_sub_m61(cv[q][r], lp.kc, P);
-                        else
-                            v = sub_mod(add_mod(cv[q][r], lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
-                        *((rok[r] && cok) ? mrow[r] + cb + c : tr) = v;
-                    }
-                    if (j + 1 < nstep) fetch_c(j + 1, q);
-                }
-            }
-            __syncthreads();  // Bs[buf ^ 1] is staged; Bs[buf] is free
-        }
+        FarItem it;
+        it.x = lt & 7;
+        const uint32_t band = lt >> 3;
+        const uint32_t O = S.n - I.rank, pb = I.pb;
+        const bool pair = uni32(I.kind) == IT_PAIR;
+        update_range(I, 1, it.cbase, it.width);
+        it.C = cdiv(it.width, UT);
+        if ((pb == 0 && !pair) || it.x >= it.C) continue;  // uniform: no pivots, or no tile
+        it.Mg = to_global(S.M) + I.c0;
+        it.permg = to_global((const uint32_t *)S.perm);
+        it.Ag[0] = to_global((const v4i *)S.AlP);
+        it.Ag[1] = to_global((const v4i *)S.Al);
+        it.Xg[0] = to_global((const v4i *)S.XlP) + (size_t)(pair ? I.dxoff : 0u) * (KA / 16);
+        it.Xg[1] = to_global((const v4i *)S.Xl);
+        it.r0 = band * UT;
+        it.nr = min(UT, O - it.r0);
+        it.rank = I.rank;
+        it.ld = S.ld;
+        if (!pair)
+            far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+        else if (pb > 0)
+            far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+        else
+            far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
     }
 }
 

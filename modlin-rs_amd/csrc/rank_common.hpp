@@ -12,6 +12,15 @@ namespace mdl {
 // rank-pb update with pb <= BMAX.
 constexpr int BMAX = 64;
 
+// Kinds of panel iteration (MFMA method).  A DEFER iteration updates only the
+// near columns [c0, jend + 2 BMAX) -- what the next two panels need -- and leaves
+// its contribution to the far columns pending; the next iteration (PAIR) applies
+// it together with its own, as one update with up to 2 BMAX pivots, so each far
+// element is read and written once per two panels.  Its own pivot rows, which
+// the pending update has not reached, get it inside the X product:
+// X' = Z (M[R, c] - M[R, J'] X'_pending[c]) for the far columns c.
+enum : uint32_t { IT_NORMAL = 0, IT_DEFER = 1, IT_PAIR = 2 };
+
 // Parameters of one panel iteration, written by k_block_ge and read by the
 // kernels of the same iteration.  RankMat keeps two, selected by iteration
 // parity, so that the next panel's elimination can run while the far columns
@@ -24,10 +33,15 @@ struct IterState {
     uint32_t c0, cend;       // trailing-update column range
     uint32_t split;          // first column of the far part of the update (jend + BMAX)
     uint32_t resid;          // the panel needs a residual (missed-pivot) check
+    uint32_t kind;           // IT_NORMAL, IT_DEFER or IT_PAIR
+    uint32_t dpb;            // IT_PAIR: pivots of the pending panel
+    uint32_t dcol;           // IT_PAIR: first column the pending panel has not updated (= split)
+    uint32_t dxoff;          // IT_PAIR: c0 minus the pending panel's c0 (its X digit columns)
     uint32_t pad;
     uint32_t J[BMAX];        // pivot columns of the panel (absolute)
     uint32_t R[BMAX];        // physical pivot rows, in pivot order
     uint32_t top[BMAX];      // physical rows of the top block in their new order
+    uint32_t dJ[BMAX];       // IT_PAIR: pivot columns of the pending panel (absolute)
 };
 
 // One matrix of a rank batch, resident in device memory.  The matrix is the
@@ -44,6 +58,8 @@ struct RankMat {
     int8_t *Al;        // int8 limb planes of Ab (MFMA path)
     int8_t *Xl;        // int8 limb planes of the shifted X (MFMA path)
     int8_t *Zl;        // int8 limb planes of Z, rows K-contiguous (MFMA path)
+    int8_t *AlP;       // IT_PAIR: int8 limb planes of M[O, J'] for the pending panel's columns J'
+    int8_t *XlP;       // IT_DEFER: int8 limb planes of its X (read by its near update and the next PAIR)
     uint32_t n, ld;
     uint32_t rank, col;      // pivots found; first column not yet eliminated
     uint32_t first_nz_col;   // smallest panel column with a nonzero residual
@@ -51,6 +67,9 @@ struct RankMat {
     uint32_t iters;
     uint32_t pend;           // missed pivot: position whose row moves to `rank`, applied by
                              // the next k_plan (0xffffffff: none); see k_finish
+    uint32_t dpend;          // a DEFER iteration's far contribution is pending
+    uint32_t dpb, dcol, dc0; // its pivots, first column not updated, and base column c0
+    uint32_t dJ[BMAX];       // its pivot columns
     IterState it[2];
     // Counters, on a cache line of their own, kept by k_plan (no atomics in the
     // update kernels).
@@ -154,6 +173,11 @@ __device__ __forceinline__ bool tile_geom(uint32_t O, uint32_t cbase, uint32_t w
     nr = min(UT, O - r0);
     nc = min(UT, cbase + width - cb);
     return true;
+}
+
+// The X digit planes an iteration writes and its near update reads.
+__device__ __forceinline__ int8_t *own_xl(const RankMat &S, const IterState &I) {
+    return I.kind == IT_DEFER ? S.XlP : S.Xl;
 }
 
 // Column range of the near (far = 0) or far (far = 1) part of the update,

@@ -43,7 +43,8 @@ __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
 // k_block_ge: pivot search and fraction-free LU of the panel part of the top block
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par) {
+__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
+                                                   uint32_t defer) {
     RankMat &S = mats[blockIdx.x];
     if (S.done) return;
     // One LDS row per block row: [0, BMAX) panel entries T, then own (the
@@ -56,6 +57,7 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
+    const uint32_t dpend = S.dpend;  // read before the barriers below; thread 0 clears it at the end
     const uint32_t jend = min(col + b, n);
     const uint32_t w = jend - col;
     const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
@@ -179,22 +181,50 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         I.J[tid] = col + Jl[tid];
         I.R[tid] = rowid[tid];
     }
+    // Iteration kind (MFMA method): a pending far contribution makes this a PAIR
+    // iteration; otherwise a clean panel (pivots, no missed-pivot check) with far
+    // columns beyond the next two panels and rows left below DEFERs its own.
+    const bool below = (n - rank0) > Bt;
+    const uint32_t resid = (pb < w && below) ? 1u : 0u;
+    const uint32_t kind = dpend ? IT_PAIR
+                          : (defer && pb > 0 && !resid && jend + 2 * BMAX < n && rank0 + pb < n) ? IT_DEFER
+                                                                                                : IT_NORMAL;
+    if (kind == IT_PAIR && tid < BMAX) I.dJ[tid] = S.dJ[tid];
+    if (kind == IT_DEFER && tid < pb) S.dJ[tid] = col + Jl[tid];
     if (tid == 0) {
         I.rank0 = rank0;
         I.rank = rank0 + pb;
         I.Bt = Bt;
         I.jend = jend;
         I.pb = pb;
-        I.split = jend + BMAX;
-        const bool below = (n - rank0) > Bt;
-        const uint32_t resid = (pb < w && below) ? 1u : 0u;
         I.resid = resid;
-        if (pb > 0) {
-            I.c0 = resid ? col : jend;
+        I.kind = kind;
+        if (kind == IT_PAIR) {
+            // near part [c0, dcol): this panel only; far part [dcol, n): both panels
+            I.split = S.dcol;
+            I.c0 = pb > 0 ? (resid ? col : jend) : (resid ? col : S.dcol);
             I.cend = n;
+            I.dpb = S.dpb;
+            I.dcol = S.dcol;
+            I.dxoff = I.c0 - S.dc0;
+            S.dpend = 0;
+        } else if (kind == IT_DEFER) {
+            I.split = jend + 2 * BMAX;  // near part only; the far part is pending
+            I.c0 = jend;
+            I.cend = n;
+            S.dpend = 1;
+            S.dpb = pb;
+            S.dcol = jend + 2 * BMAX;
+            S.dc0 = jend;
         } else {
-            I.c0 = col;
-            I.cend = resid ? jend : col;  // detection only, or nothing at all
+            I.split = jend + BMAX;
+            if (pb > 0) {
+                I.c0 = resid ? col : jend;
+                I.cend = n;
+            } else {
+                I.c0 = col;
+                I.cend = resid ? jend : col;  // detection only, or nothing at all
+            }
         }
         S.rank = rank0 + pb;
         S.first_nz_col = 0xffffffffu;
@@ -371,21 +401,27 @@ __global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint
                 const uint32_t O = S.n - I.rank;
                 const uint32_t cols = I.cend - I.c0;
                 if (O > 0 && cols > 0) {
+                    const bool pair = I.kind == IT_PAIR;
+                    // gather rows (own and, for a PAIR, the pending panel's columns) and X columns
                     if (I.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
+                    else if (pair) g = cdiv(O, GATHER_ROWS);
                     uint32_t base, wn, wf;
                     update_range(I, 0, base, wn);
                     un = part_tiles(O, wn, 0);
                     update_range(I, 1, base, wf);
+                    if (I.kind == IT_DEFER) wf = 0;  // pending: applied by the next iteration
                     // far work: XCD-numbered tiles (VALU), or (row band, XCD group) items (MFMA)
                     uf = far_items ? (wf > 0 ? cdiv(O, UT) * 8 : 0) : part_tiles(O, wf, 1);
-                    if (I.pb > 0) {
-                        // The updates' work: O x cols x pb products; per tile 16 B per
-                        // element and 8 B per operand element, summed over the tiles of
-                        // both parts (R row tiles, C column tiles).
+                    const uint32_t kfar = I.pb + (pair ? I.dpb : 0u);
+                    if (I.pb > 0 || pair) {
+                        // The update kernels' work: O x wn x pb products in the near part
+                        // and O x wf x kfar in the far part; per tile 16 B per element
+                        // and 8 B per operand element, summed over the tiles of both
+                        // parts (R row tiles, C column tiles).
                         RankMat &Sw = mats[i];
                         const unsigned long long R = cdiv(O, UT), C = cdiv(wn, UT) + cdiv(wf, UT);
-                        Sw.macs += (unsigned long long)O * cols * I.pb;
-                        Sw.upd_bytes += 16ull * O * cols + 8ull * I.pb * (O * C + (unsigned long long)cols * R);
+                        Sw.macs += (unsigned long long)O * wn * I.pb + (unsigned long long)O * wf * kfar;
+                        Sw.upd_bytes += 16ull * O * (wn + wf) + 8ull * kfar * (O * C + (unsigned long long)(wn + wf) * R);
                     }
                 }
             }
@@ -612,15 +648,15 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 // Host launchers
 // ---------------------------------------------------------------------------
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L) {
+                  uint32_t L, uint32_t defer) {
     if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par);
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     }
 }

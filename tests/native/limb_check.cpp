@@ -94,9 +94,65 @@ static void run(uint64_t p, int trials, int kind) {
     }
 }
 
+// Two accumulated panels (K up to 128): classes started at KOFF2, recombined with
+// limb_residue_k2 (and limb_sub_m61_k2 for 2^61 - 1).
+template <int MODE, int L>
+static void run_k2(uint64_t p, int trials, int kind) {
+    constexpr int NC = 2 * L - 1;
+    const FieldParams f = make_params(p);
+    const LimbParams lp = make_limb_params(p);
+    for (int tr = 0; tr < trials; tr++) {
+        const int K = kind == 0 ? 1 + (int)(rng() % 128) : 128;
+        static int8_t a[128][8], x[128][8];
+        for (int k = 0; k < K; k++)
+            for (int l = 0; l < L; l++) {
+                int da, dx;
+                digits(kind, l, da, dx);
+                a[k][l] = (int8_t)da;
+                x[k][l] = (int8_t)dx;
+            }
+        int32_t acc[NC];
+        for (int s = 0; s < NC; s++) acc[s] = KOFF2;
+        for (int k = 0; k < K; k++)
+            for (int la = 0; la < L; la++)
+                for (int lb = 0; lb < L; lb++) {
+                    const int64_t v = (int64_t)acc[la + lb] + (int64_t)a[k][la] * x[k][lb];
+                    if (v >= (1 << 26) || v < 0) {
+                        printf("class outside [0, 2^26) p=%llu L=%d kind=%d\n", (unsigned long long)p, L, kind);
+                        bad++;
+                        return;
+                    }
+                    acc[la + lb] = (int32_t)v;
+                }
+        i128 want = 0;
+        for (int k = 0; k < K; k++)
+            want = (want + (i128)(((u128)value_mod(a[k], L, p) * (u128)value_mod(x[k], L, p)) % p)) % (i128)p;
+        uint64_t got = limb_residue_k2<MODE, NC>(f, lp, acc);
+        got = got >= lp.k2 ? got - lp.k2 : got + p - lp.k2;
+        if ((uint64_t)want != got) {
+            if (bad < 8)
+                printf("limb_residue_k2 mismatch p=%llu L=%d kind=%d K=%d\n", (unsigned long long)p, L, kind, K);
+            bad++;
+        }
+        if (MODE == MODE_M61 && L == 8) {
+            for (uint64_t cv : {(uint64_t)(rng() % p), (uint64_t)0, p - 1}) {
+                const uint64_t sub = limb_sub_m61_k2(cv, lp.kc2, acc);
+                const uint64_t exp = cv >= (uint64_t)want ? cv - (uint64_t)want : cv + p - (uint64_t)want;
+                if (sub != exp) {
+                    if (bad < 8) printf("limb_sub_m61_k2 mismatch kind=%d K=%d\n", kind, K);
+                    bad++;
+                }
+            }
+        }
+    }
+}
+
 template <int L>
 static void generic(uint64_t p, int trials) {
-    for (int kind = 0; kind < 5; kind++) run<MODE_GENERIC, L>(p, kind ? 20 : trials, kind);
+    for (int kind = 0; kind < 5; kind++) {
+        run<MODE_GENERIC, L>(p, kind ? 20 : trials, kind);
+        run_k2<MODE_GENERIC, L>(p, kind ? 20 : trials / 4, kind);
+    }
 }
 
 int main() {
@@ -104,6 +160,8 @@ int main() {
     for (int kind = 0; kind < 5; kind++) {
         run<MODE_M61, 8>(P61, kind ? 50 : 10 * n, kind);
         run<MODE_M31, 4>(P31, kind ? 50 : 10 * n, kind);
+        run_k2<MODE_M61, 8>(P61, kind ? 50 : 2 * n, kind);
+        run_k2<MODE_M31, 4>(P31, kind ? 50 : 2 * n, kind);
     }
     const uint64_t primes[] = {2ull, 3ull, 5ull, 7ull, 127ull, 131ull, 251ull, 257ull, 65521ull, 65537ull,
                                16777213ull, 4294967291ull, 1099511627689ull, 281474976710597ull,

@@ -228,3 +228,28 @@ def test_rank_progress_in_columns(ctx, n):
         assert done == sorted(done) and done[-1] == n and len(set(done)) == len(done)
         if n >= 3000:
             assert len(done) >= 2   # intermediate reports (one host check per 8 panels)
+
+
+@pytest.mark.parametrize("p", [M61, (1 << 31) - 1, 65521, 251, (1 << 62) + 135])
+def test_deferred_far_update(ctx, p):
+    """The MFMA method's paired panels (far columns updated once per two panels,
+    up to 128 pivots per pass, the second panel's pivot rows corrected inside its X
+    product) give the oracle's ranks, as does one far update per panel: mixed
+    sizes, deficient and planted matrices, hidden pivots (missed-pivot restarts
+    between pairs), and panel widths below 64."""
+    rng = np.random.default_rng(p % 997)
+    mats = [rng.integers(0, p, size=(n, n), dtype=np.uint64) for n in (130, 200, 257, 390, 700)]
+    mats += [workloads.c3(520, p), workloads.c3(777, p)]
+    r = 90
+    mats.append(modlin.combine_rows(p, rng.integers(0, p, size=(r, 600), dtype=np.uint64),
+                                    rng.integers(0, p, size=(600, r), dtype=np.uint64)))
+    mats += _hidden_pivot_cases(p, rng)[5:]
+    z = rng.integers(0, p, size=(400, 400), dtype=np.uint64)
+    z[:, 150:260] = 0                                   # a run of zero columns in mid-matrix
+    mats.append(z)
+    want = [oracle.rank(p, a, a.shape[0]) for a in mats]
+    for panel in (0, 64, 48, 17):
+        print(f"deferred p={p} panel={panel}", flush=True)
+        assert _rank_all(ctx, p, mats, modlin.METHOD_MFMA, panel=panel) == want, panel
+    got = ctx.rank_batch(modlin.Field(p), mats, method=modlin.METHOD_MFMA, defer=False)
+    assert got == want
