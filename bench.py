@@ -19,8 +19,8 @@ Berlekamp-Massey workload (1024 sequences, steps/s) and the per-item checks
 against the CPU-computed fixtures in tests/golden/.
 
 --copy-calibration additionally ranks one all-zero 4096 x 4096 u32 matrix held
-in device memory: its staging copy (64 MiB read, 64 MiB written) and widening
-kernel k_widen_u32 (64 MiB read, 128 MiB written) are copy kernels of known
+in device memory: its set-up kernel k_setup widens it into the workspace (64
+MiB read, 128 MiB written, plus 16 KiB of row order), a copy kernel of known
 size that a counter run uses to scale FETCH_SIZE / WRITE_SIZE to bytes.
 """
 import argparse
@@ -130,7 +130,7 @@ def copy_calibration(ctx, n=4096):
     r = ctx.rank_batch(M.Field(M61), [dev])
     ctx.synchronize()
     dev.free()
-    return {"copy_kernel": "k_widen_u32", "elements": n * n, "bytes_read": 4 * n * n, "bytes_written": 8 * n * n,
+    return {"copy_kernel": "k_setup", "elements": n * n, "bytes_read": 4 * n * n, "bytes_written": 8 * n * n + 4 * n,
             "rank": r[0]}
 
 

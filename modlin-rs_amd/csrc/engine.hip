@@ -114,13 +114,38 @@ int copy_d2h(modlin_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return MODLIN_OK;
 }
 
-__global__ void k_widen_u32(const uint32_t *src, uint64_t *dst, size_t count) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
-}
+// Set-up of one matrix of a batch: its working copy (from the caller's device
+// matrix or the staged u32 copy; none when the host copy already landed in M)
+// and the identity row order.
+struct SetupJob {
+    const void *src;
+    uint64_t *M;
+    uint32_t *perm;
+    uint64_t elems;  // n * n
+    uint32_t n;
+    uint32_t form;   // 0: u64 in 16-byte pieces, 1: u64 one by one, 2: u32 widened
+};
 
-__global__ void k_iota(uint32_t *perm, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) perm[i] = i;
+// All matrices of a batch in one launch: workgroup (x, y) covers part x of job y.
+__global__ void k_setup(const SetupJob *jobs) {
+    const SetupJob J = jobs[blockIdx.y];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J.src) {
+        if (J.form == 2) {
+            const uint32_t *src = (const uint32_t *)J.src;
+            for (uint64_t i = t0; i < J.elems; i += stride) J.M[i] = src[i];
+        } else if (J.form == 1) {
+            const uint64_t *src = (const uint64_t *)J.src;
+            for (uint64_t i = t0; i < J.elems; i += stride) J.M[i] = src[i];
+        } else {
+            const ulonglong2 *src = (const ulonglong2 *)J.src;
+            ulonglong2 *dst = (ulonglong2 *)J.M;
+            for (uint64_t i = t0; i < J.elems / 2; i += stride) dst[i] = src[i];
+            if (t0 == 0 && (J.elems & 1)) J.M[J.elems - 1] = ((const uint64_t *)J.src)[J.elems - 1];
+        }
+    }
+    for (uint64_t i = t0; i < J.n; i += stride) J.perm[i] = (uint32_t)i;
 }
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -150,7 +175,13 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offAl(count), offXl(count), offZl(count), offAlP(count), offXlP(count);
     size_t cur = 0;
     const size_t A = 256;
-    cur = align_up(cur + count * sizeof(RankMat), A);
+    // the matrices' state, their set-up jobs and the active count, copied to the
+    // device together
+    cur = count * sizeof(RankMat);
+    const size_t off_jobs = cur;
+    cur += count * sizeof(SetupJob);
+    const size_t off_active = cur;
+    cur = align_up(cur + sizeof(uint32_t), A);
     const size_t off_gx = cur;
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_upd = cur;
@@ -159,8 +190,6 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_plan = cur;
     cur = align_up(cur + sizeof(RankPlan), A);
-    const size_t off_active = cur;
-    cur = align_up(cur + sizeof(uint32_t), A);
     const size_t off_trash = cur;
     cur = align_up(cur + TRASH_WORDS * sizeof(uint64_t), A);
     for (uint64_t i = 0; i < count; i++) {
@@ -204,7 +233,10 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     char *base = (char *)ctx->arena;
     hipStream_t st = ctx->stream;
 
-    std::vector<RankMat> hm(count);
+    // host image of [RankMat x count | SetupJob x count | active]
+    std::vector<char> himg(off_active + sizeof(uint32_t));
+    RankMat *hm = (RankMat *)himg.data();
+    SetupJob *jobs = (SetupJob *)(himg.data() + off_jobs);
     uint32_t active = 0;
     for (uint64_t i = 0; i < count; i++) {
         const uint64_t n = mats[i].n;
@@ -229,18 +261,21 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         active++;
         const size_t elems = n * n;
         const bool dev = mats[i].mem == MODLIN_MEM_DEVICE;
-        const size_t esz = mats[i].elem == MODLIN_ELEM_U64 ? 8 : 4;
-        void *dst = mats[i].elem == MODLIN_ELEM_U64 ? (void *)S.M : (void *)(base + offS[i]);
-        if (dev) {
-            MDL_HIP(hipMemcpyAsync(dst, mats[i].data, elems * esz, hipMemcpyDeviceToDevice, st));
-        } else {
-            rc = copy_h2d(ctx, dst, mats[i].data, elems * esz);
+        const bool u32 = mats[i].elem == MODLIN_ELEM_U32;
+        SetupJob &J = jobs[i];
+        J.M = S.M;
+        J.perm = S.perm;
+        J.elems = elems;
+        J.n = (uint32_t)n;
+        J.form = u32 ? 2u : (((uintptr_t)mats[i].data & 15) ? 1u : 0u);
+        J.src = mats[i].data;
+        if (!dev) {
+            // host entries: staged straight into M (u64) or into the stage (u32)
+            void *dst = u32 ? (void *)(base + offS[i]) : (void *)S.M;
+            rc = copy_h2d(ctx, dst, mats[i].data, elems * (u32 ? 4 : 8));
             if (rc) return rc;
+            J.src = u32 ? dst : nullptr;
         }
-        if (mats[i].elem == MODLIN_ELEM_U32)
-            hipLaunchKernelGGL(k_widen_u32, dim3(1024), dim3(256), 0, st, (const uint32_t *)dst, S.M, elems);
-        hipLaunchKernelGGL(k_iota, dim3(std::max<uint32_t>(1, (uint32_t)((n + 255) / 256))), dim3(256), 0, st,
-                           S.perm, (uint32_t)n);
     }
     RankMat *dmats = (RankMat *)(base + 0);
     RankLaunch rl;
@@ -250,10 +285,13 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     rl.plan = (RankPlan *)(base + off_plan);
     rl.trash = (uint64_t *)(base + off_trash);
     uint32_t *dactive = (uint32_t *)(base + off_active);
-    rc = copy_h2d(ctx, dmats, hm.data(), count * sizeof(RankMat));
+    *(uint32_t *)(himg.data() + off_active) = active;
+    rc = copy_h2d(ctx, dmats, himg.data(), himg.size());
     if (rc) return rc;
-    rc = copy_h2d(ctx, dactive, &active, sizeof(uint32_t));
-    if (rc) return rc;
+    {
+        const uint32_t gx = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(1024, 4096 / count));
+        hipLaunchKernelGGL(k_setup, dim3(gx, (uint32_t)count), dim3(256), 0, st, (const SetupJob *)(base + off_jobs));
+    }
     MDL_HIP(hipGetLastError());
 
     const uint32_t nmat = (uint32_t)count;
@@ -364,7 +402,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             }
         }
         if (progress && h_active > 0) {
-            rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
+            rc = copy_d2h(ctx, hm, dmats, count * sizeof(RankMat));
             if (rc) return rc;
             uint64_t cols = 0;
             for (uint64_t i = 0; i < count; i++) cols += hm[i].done ? hm[i].n : std::min(hm[i].col, hm[i].n);
@@ -377,7 +415,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     if (progress) progress(user, total_cols, total_cols);
     MDL_HIP(hipStreamSynchronize(aux));
     for (auto &e : evs) hipEventDestroy(e);
-    rc = copy_d2h(ctx, hm.data(), dmats, count * sizeof(RankMat));
+    rc = copy_d2h(ctx, hm, dmats, count * sizeof(RankMat));
     if (rc) return rc;
     double macs = 0, ubytes = 0;
     for (uint64_t i = 0; i < count; i++) {

@@ -363,9 +363,76 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
 // k_plan: tile counts and prefix sums (single workgroup)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *near_off,
-                                              uint32_t *far_off, RankPlan *plan, uint32_t par, uint32_t far_items) {
-    __shared__ uint32_t sg[256], sn[256], sf[256];
+// Tile counts of matrix S in this iteration: gather tasks g, near tiles un, far
+// tiles or items uf; adds the update kernels' work to its counters.
+__device__ __forceinline__ void plan_counts(RankMat &S, uint32_t par, uint32_t far_items, uint32_t &g, uint32_t &un,
+                                            uint32_t &uf) {
+    g = un = uf = 0;
+    if (S.done) return;
+    const IterState &I = S.it[par];
+    const uint32_t O = S.n - I.rank;
+    const uint32_t cols = I.cend - I.c0;
+    if (O == 0 || cols == 0) return;
+    const bool pair = I.kind == IT_PAIR;
+    // gather rows (own and, for a PAIR, the pending panel's columns) and X columns
+    if (I.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
+    else if (pair) g = cdiv(O, GATHER_ROWS);
+    uint32_t base, wn, wf;
+    update_range(I, 0, base, wn);
+    un = part_tiles(O, wn, 0);
+    update_range(I, 1, base, wf);
+    if (I.kind == IT_DEFER) wf = 0;  // pending: applied by the next iteration
+    // far work: XCD-numbered tiles (VALU), or (row band, XCD group) items (MFMA)
+    uf = far_items ? (wf > 0 ? cdiv(O, UT) * 8 : 0) : part_tiles(O, wf, 1);
+    const uint32_t kfar = I.pb + (pair ? I.dpb : 0u);
+    if (I.pb > 0 || pair) {
+        // The update kernels' work: O x wn x pb products in the near part and
+        // O x wf x kfar in the far part; per tile 16 B per element and 8 B per
+        // operand element, summed over the tiles of both parts (R row tiles, C
+        // column tiles).
+        const unsigned long long R = cdiv(O, UT), C = cdiv(wn, UT) + cdiv(wf, UT);
+        S.macs += (unsigned long long)O * wn * I.pb + (unsigned long long)O * wf * kfar;
+        S.upd_bytes += 16ull * O * (wn + wf) + 8ull * kfar * (O * C + (unsigned long long)(wn + wf) * R);
+    }
+}
+
+// Inclusive prefix sums of x over the workgroup (PT threads, at most 16 waves).
+template <uint32_t PT>
+__device__ __forceinline__ void block_incl_scan3(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t (*wsum)[3],
+                                                 uint32_t tid) {
+    const uint32_t lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t ta = __shfl_up(a, off, 64), tb = __shfl_up(b, off, 64), tc = __shfl_up(c, off, 64);
+        if (lane >= off) {
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+    }
+    if (lane == 63) {
+        wsum[w][0] = a;
+        wsum[w][1] = b;
+        wsum[w][2] = c;
+    }
+    __syncthreads();
+    uint32_t pa = 0, pb = 0, pc = 0;
+    for (uint32_t k = 0; k < w; k++) {
+        pa += wsum[k][0];
+        pb += wsum[k][1];
+        pc += wsum[k][2];
+    }
+    a += pa;
+    b += pb;
+    c += pc;
+}
+
+constexpr uint32_t PLAN_THREADS = 1024;
+
+__global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off,
+                                                       uint32_t *near_off, uint32_t *far_off, RankPlan *plan,
+                                                       uint32_t par, uint32_t far_items) {
+    __shared__ uint32_t wsum[PLAN_THREADS / 64][3];
     __shared__ uint32_t base_g, base_n, base_f;
     const uint32_t tid = threadIdx.x;
     // Commit the last iteration's missed-pivot swap, then the top blocks' new row
@@ -391,65 +458,22 @@ __global__ __launch_bounds__(256) void k_plan(RankMat *mats, uint32_t nmat, uint
     }
     if (tid == 0) base_g = base_n = base_f = 0;
     __syncthreads();
-    for (uint32_t i0 = 0; i0 < nmat; i0 += 256) {
+    for (uint32_t i0 = 0; i0 < nmat; i0 += PLAN_THREADS) {
         const uint32_t i = i0 + tid;
         uint32_t g = 0, un = 0, uf = 0;
+        if (i < nmat) plan_counts(mats[i], par, far_items, g, un, uf);
+        uint32_t sg = g, sn = un, sf = uf;
+        block_incl_scan3<PLAN_THREADS>(sg, sn, sf, wsum, tid);
         if (i < nmat) {
-            const RankMat &S = mats[i];
-            if (!S.done) {
-                const IterState &I = S.it[par];
-                const uint32_t O = S.n - I.rank;
-                const uint32_t cols = I.cend - I.c0;
-                if (O > 0 && cols > 0) {
-                    const bool pair = I.kind == IT_PAIR;
-                    // gather rows (own and, for a PAIR, the pending panel's columns) and X columns
-                    if (I.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
-                    else if (pair) g = cdiv(O, GATHER_ROWS);
-                    uint32_t base, wn, wf;
-                    update_range(I, 0, base, wn);
-                    un = part_tiles(O, wn, 0);
-                    update_range(I, 1, base, wf);
-                    if (I.kind == IT_DEFER) wf = 0;  // pending: applied by the next iteration
-                    // far work: XCD-numbered tiles (VALU), or (row band, XCD group) items (MFMA)
-                    uf = far_items ? (wf > 0 ? cdiv(O, UT) * 8 : 0) : part_tiles(O, wf, 1);
-                    const uint32_t kfar = I.pb + (pair ? I.dpb : 0u);
-                    if (I.pb > 0 || pair) {
-                        // The update kernels' work: O x wn x pb products in the near part
-                        // and O x wf x kfar in the far part; per tile 16 B per element
-                        // and 8 B per operand element, summed over the tiles of both
-                        // parts (R row tiles, C column tiles).
-                        RankMat &Sw = mats[i];
-                        const unsigned long long R = cdiv(O, UT), C = cdiv(wn, UT) + cdiv(wf, UT);
-                        Sw.macs += (unsigned long long)O * wn * I.pb + (unsigned long long)O * wf * kfar;
-                        Sw.upd_bytes += 16ull * O * (wn + wf) + 8ull * kfar * (O * C + (unsigned long long)(wn + wf) * R);
-                    }
-                }
-            }
-        }
-        sg[tid] = g;
-        sn[tid] = un;
-        sf[tid] = uf;
-        __syncthreads();
-        // inclusive Hillis-Steele scans
-        for (uint32_t off = 1; off < 256; off <<= 1) {
-            const uint32_t ag = tid >= off ? sg[tid - off] : 0, an = tid >= off ? sn[tid - off] : 0,
-                           af = tid >= off ? sf[tid - off] : 0;
-            __syncthreads();
-            sg[tid] += ag;
-            sn[tid] += an;
-            sf[tid] += af;
-            __syncthreads();
-        }
-        if (i < nmat) {
-            gx_off[i] = base_g + sg[tid] - g;
-            near_off[i] = base_n + sn[tid] - un;
-            far_off[i] = base_f + sf[tid] - uf;
+            gx_off[i] = base_g + sg - g;
+            near_off[i] = base_n + sn - un;
+            far_off[i] = base_f + sf - uf;
         }
         __syncthreads();
-        if (tid == 255) {
-            base_g += sg[255];
-            base_n += sn[255];
-            base_f += sf[255];
+        if (tid == PLAN_THREADS - 1) {
+            base_g += sg;
+            base_n += sn;
+            base_f += sf;
         }
         __syncthreads();
     }
@@ -662,7 +686,7 @@ void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldPara
 }
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.plan,
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_THREADS), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.plan,
                        par, rl.far_items);
 }
 
