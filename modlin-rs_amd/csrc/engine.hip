@@ -188,6 +188,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_far = cur;
     cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+    const size_t off_xf = cur;
+    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
     const size_t off_plan = cur;
     cur = align_up(cur + sizeof(RankPlan), A);
     const size_t off_trash = cur;
@@ -221,7 +223,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             const size_t npad = (n + UT - 1) / UT * UT;
             offAl[i] = cur; cur = align_up(cur + npad * KA, A);
             offXl[i] = cur; cur = align_up(cur + npad * KA, A);
-            offZl[i] = cur; cur = align_up(cur + BMAX * KA, A);
+            offZl[i] = cur; cur = align_up(cur + 2 * BMAX * KA, A);  // one per iteration parity
             if (defer) {
                 offAlP[i] = cur; cur = align_up(cur + npad * KA, A);
                 offXlP[i] = cur; cur = align_up(cur + (npad + UT) * KA, A);
@@ -282,6 +284,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     rl.gx_off = (uint32_t *)(base + off_gx);
     rl.near_off = (uint32_t *)(base + off_upd);
     rl.far_off = (uint32_t *)(base + off_far);
+    rl.xf_off = (uint32_t *)(base + off_xf);
     rl.plan = (RankPlan *)(base + off_plan);
     rl.trash = (uint64_t *)(base + off_trash);
     uint32_t *dactive = (uint32_t *)(base + off_active);
@@ -335,6 +338,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             if (prof) hipEventRecord(evs[4 * s + 1], st);
             launch_finish(st, dmats, nmat, dactive, par);
             if ((e = hipEventRecord(ctx->ev_fin, st)) != hipSuccess) return e;
+            if (use_mfma) launch_mfma(st, dmats, nmat, f, lp, rl, par, -2);  // far X, off the panel chain
             if (prof) hipEventRecord(evs[4 * s + 2], st);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, 1);

@@ -141,14 +141,14 @@ __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs
 //               forms Y, whose digits then take the pivot rows' place.
 template <int L, int MODE>
 __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nmat, const uint32_t *gx_off,
-                                                      const RankPlan *plan, FieldParams f, LimbParams lp,
-                                                      uint32_t par) {
+                                                      const RankPlan *plan, FieldParams f, LimbParams lp, uint32_t par,
+                                                      uint32_t part) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;
     constexpr int NC = 2 * L - 1;
     __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
     __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
-    const uint32_t total = plan->gx_total;
+    const uint32_t total = part ? plan->xf_total : plan->gx_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
     const uint64_t p = f.p;
@@ -156,9 +156,10 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
         const uint32_t mi = find_mat(gx_off, nmat, t);
         RankMat &S = mats[mi];
         const IterState &I = S.it[par];
-        const uint32_t lt = t - gx_off[mi];
         const uint32_t O = S.n - I.rank, pb = I.pb, ld = S.ld;
         const uint32_t ng = cdiv(O, GATHER_ROWS);
+        // part 0: row tasks, then the near X tasks; part 1: the far X tasks
+        const uint32_t lt = t - gx_off[mi] + (part ? ng + x_near_tasks(I) : 0u);
         if (lt < ng) {
             const uint32_t r0 = lt * GATHER_ROWS;
             const uint32_t rows = min(GATHER_ROWS, O - r0);
@@ -236,14 +237,14 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             }
             for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
                 const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
-                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
+                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + ((size_t)par * BMAX + k) * KA))[ch];
             }
         } else {
             // As[k][la*64 + s] = digit la of Z[k][s] (split once by k_block_z);
             // Bs[c][lb*64 + s] = digit lb of M[R[s]][c0 + cb + c]
             for (uint32_t idx = tid; idx < UT * (KA / 16); idx += blockDim.x) {
                 const uint32_t k = idx / (KA / 16), ch = idx - k * (KA / 16);
-                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + (size_t)k * KA))[ch];
+                *(v4i *)(As + k * KAP + ch * 16) = ((const v4i *)(S.Zl + ((size_t)par * BMAX + k) * KA))[ch];
             }
             for (uint32_t idx = tid; idx < XCOLS * (KSTEP / 4); idx += blockDim.x) {
                 const uint32_t c = idx % XCOLS, s4 = (idx / XCOLS) * 4;
@@ -637,9 +638,9 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
 template <int L, int MODE>
 static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                           const RankLaunch &rl, uint32_t par, int what) {
-    if (what < 0)
-        hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(rl.grid_gx), dim3(512), 0, st, dmats, nmat, rl.gx_off,
-                           rl.plan, f, lp, par);
+    if (what < 0)  // -1: rows and near X; -2: far X
+        hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(rl.grid_gx), dim3(512), 0, st, dmats, nmat,
+                           what == -1 ? rl.gx_off : rl.xf_off, rl.plan, f, lp, par, what == -1 ? 0u : 1u);
     else if (what == 0)
         hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(rl.grid_near), dim3(512), 0, st, dmats, nmat, rl.near_off,
                            rl.plan, f, lp, par, 0u);
@@ -648,7 +649,8 @@ static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const F
                            rl.plan, f, lp, par, rl.trash);
 }
 
-// what = -1: digit gather and X product; 0: near update; 1: far update.
+// what = -1: digit gather and the near X product; -2: the far X product; 0: near
+// update; 1: far update.
 void launch_mfma(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const LimbParams &lp,
                  const RankLaunch &rl, uint32_t par, int what) {
 #define MDL_L(LL) launch_mfma_L<LL, MODE_GENERIC>(st, dmats, nmat, f, lp, rl, par, what)

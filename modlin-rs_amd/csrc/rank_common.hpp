@@ -57,7 +57,8 @@ struct RankMat {
     uint64_t *Eta;     // row operations of the last block elimination (ETA_* layout)
     int8_t *Al;        // int8 limb planes of Ab (MFMA path)
     int8_t *Xl;        // int8 limb planes of the shifted X (MFMA path)
-    int8_t *Zl;        // int8 limb planes of Z, rows K-contiguous (MFMA path)
+    int8_t *Zl;        // int8 limb planes of Z, rows K-contiguous (MFMA path), one block per iteration
+                       // parity: the far X product of an iteration reads it while the next k_block_z writes
     int8_t *AlP;       // IT_PAIR: int8 limb planes of M[O, J'] for the pending panel's columns J'
     int8_t *XlP;       // IT_DEFER: int8 limb planes of its X (read by its near update and the next PAIR)
     uint32_t n, ld;
@@ -81,7 +82,7 @@ struct RankMat {
 // gather tasks and of the near (columns [c0, split)) and far ([split, cend))
 // update tiles.
 struct RankPlan {
-    uint32_t gx_total, near_total, far_total, pad;
+    uint32_t gx_total, near_total, far_total, xf_total;
 };
 
 // Record of one block elimination, written by k_block_ge and replayed by
@@ -102,6 +103,7 @@ constexpr uint32_t ZSLOTS = BMAX / ZG;                 // slot waves per workgro
 // Device buffers and grid sizes of the per-iteration launches.
 struct RankLaunch {
     uint32_t *gx_off, *near_off, *far_off;  // nmat + 1 offsets each
+    uint32_t *xf_off;                       // MFMA: X tasks of the far columns, launched after k_finish
     RankPlan *plan;
     uint32_t grid_gx, grid_near, grid_far;
     uint32_t far_items;                     // far work counted as (row band, XCD group) items (MFMA)
@@ -187,6 +189,14 @@ __device__ __forceinline__ void update_range(const IterState &I, uint32_t far, u
     const uint32_t nw = min(cols, I.split - I.c0);
     base = far ? nw : 0;
     width = far ? cols - nw : nw;
+}
+
+// MFMA method: the X column tasks (XCOLS wide, from c0) that the near update
+// needs -- those that start below split; the rest are computed after k_finish,
+// off the chain that the next panel waits for.
+__device__ __forceinline__ uint32_t x_near_tasks(const IterState &I) {
+    const uint32_t cols = I.cend - I.c0, nw = min(cols, I.split - I.c0);
+    return I.pb > 0 ? cdiv(nw, XCOLS) : 0u;
 }
 
 __device__ __forceinline__ uint32_t find_mat(const uint32_t *off, uint32_t nmat, uint32_t t) {

@@ -184,11 +184,16 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     // Iteration kind (MFMA method): a pending far contribution makes this a PAIR
     // iteration; otherwise a clean panel (pivots, no missed-pivot check) with far
     // columns beyond the next two panels and rows left below DEFERs its own.
+    // Odd-numbered matrices of a batch start their pairs one iteration later, so
+    // that every iteration of the batch has far work to overlap the next panel
+    // eliminations.
     const bool below = (n - rank0) > Bt;
     const uint32_t resid = (pb < w && below) ? 1u : 0u;
+    const bool stagger = S.iters == 0 && (blockIdx.x & 1);
     const uint32_t kind = dpend ? IT_PAIR
-                          : (defer && pb > 0 && !resid && jend + 2 * BMAX < n && rank0 + pb < n) ? IT_DEFER
-                                                                                                : IT_NORMAL;
+                          : (defer && !stagger && pb > 0 && !resid && jend + 2 * BMAX < n && rank0 + pb < n)
+                              ? IT_DEFER
+                              : IT_NORMAL;
     if (kind == IT_PAIR && tid < BMAX) I.dJ[tid] = S.dJ[tid];
     if (kind == IT_DEFER && tid < pb) S.dJ[tid] = col + Jl[tid];
     if (tid == 0) {
@@ -269,7 +274,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     if (s0 >= pb) {  // uniform over the workgroup: no slots, only zero digit columns
         if (L)
             for (uint32_t idx = threadIdx.x; idx < BMAX * ZSLOTS; idx += blockDim.x)
-                z_digits(S.Zl, L, idx / ZSLOTS, s0 + idx % ZSLOTS, 0, f.p);
+                z_digits(S.Zl + (size_t)par * BMAX * L * KSTEP, L, idx / ZSLOTS, s0 + idx % ZSLOTS, 0, f.p);
         return;
     }
     __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
@@ -341,7 +346,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     __syncthreads();  // the inverses are in LDS
     if (wave == ZSLOTS) return;
     if (s >= pb) {
-        if (L) z_digits(S.Zl, L, lane, s, 0, p);
+        if (L) z_digits(S.Zl + (size_t)par * BMAX * L * KSTEP, L, lane, s, 0, p);
         return;
     }
 
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
         if (lane < (uint32_t)t) acc = add_mod(acc, mul_mod_t<MODE>(f, u, zt), p);
     }
     if (lane < pb) S.Z[lane * BMAX + s] = z;
-    if (L) z_digits(S.Zl, L, lane, s, lane < pb ? z : 0, p);
+    if (L) z_digits(S.Zl + (size_t)par * BMAX * L * KSTEP, L, lane, s, lane < pb ? z : 0, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -366,17 +371,23 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
 // Tile counts of matrix S in this iteration: gather tasks g, near tiles un, far
 // tiles or items uf; adds the update kernels' work to its counters.
 __device__ __forceinline__ void plan_counts(RankMat &S, uint32_t par, uint32_t far_items, uint32_t &g, uint32_t &un,
-                                            uint32_t &uf) {
-    g = un = uf = 0;
+                                            uint32_t &uf, uint32_t &xf) {
+    g = un = uf = xf = 0;
     if (S.done) return;
     const IterState &I = S.it[par];
     const uint32_t O = S.n - I.rank;
     const uint32_t cols = I.cend - I.c0;
     if (O == 0 || cols == 0) return;
     const bool pair = I.kind == IT_PAIR;
-    // gather rows (own and, for a PAIR, the pending panel's columns) and X columns
-    if (I.pb > 0) g = cdiv(O, GATHER_ROWS) + cdiv(cols, XCOLS);
-    else if (pair) g = cdiv(O, GATHER_ROWS);
+    // gather rows (own and, for a PAIR, the pending panel's columns) and X columns;
+    // the MFMA method's far X columns go to the launch after k_finish
+    if (I.pb > 0) {
+        const uint32_t nx = cdiv(cols, XCOLS), nxn = far_items ? x_near_tasks(I) : nx;
+        g = cdiv(O, GATHER_ROWS) + nxn;
+        xf = nx - nxn;
+    } else if (pair) {
+        g = cdiv(O, GATHER_ROWS);
+    }
     uint32_t base, wn, wf;
     update_range(I, 0, base, wn);
     un = part_tiles(O, wn, 0);
@@ -396,44 +407,33 @@ __device__ __forceinline__ void plan_counts(RankMat &S, uint32_t par, uint32_t f
     }
 }
 
-// Inclusive prefix sums of x over the workgroup (PT threads, at most 16 waves).
-template <uint32_t PT>
-__device__ __forceinline__ void block_incl_scan3(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t (*wsum)[3],
-                                                 uint32_t tid) {
+// Inclusive prefix sums of the 4 values v over the workgroup (at most 16 waves).
+__device__ __forceinline__ void block_incl_scan4(uint32_t *v, uint32_t (*wsum)[4], uint32_t tid) {
     const uint32_t lane = tid & 63, w = tid >> 6;
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t ta = __shfl_up(a, off, 64), tb = __shfl_up(b, off, 64), tc = __shfl_up(c, off, 64);
-        if (lane >= off) {
-            a += ta;
-            b += tb;
-            c += tc;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t t = __shfl_up(v[k], off, 64);
+            if (lane >= off) v[k] += t;
         }
     }
-    if (lane == 63) {
-        wsum[w][0] = a;
-        wsum[w][1] = b;
-        wsum[w][2] = c;
-    }
+    if (lane == 63)
+#pragma unroll
+        for (int k = 0; k < 4; k++) wsum[w][k] = v[k];
     __syncthreads();
-    uint32_t pa = 0, pb = 0, pc = 0;
-    for (uint32_t k = 0; k < w; k++) {
-        pa += wsum[k][0];
-        pb += wsum[k][1];
-        pc += wsum[k][2];
-    }
-    a += pa;
-    b += pb;
-    c += pc;
+    for (uint32_t j = 0; j < w; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] += wsum[j][k];
 }
 
 constexpr uint32_t PLAN_THREADS = 1024;
 
 __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off,
-                                                       uint32_t *near_off, uint32_t *far_off, RankPlan *plan,
-                                                       uint32_t par, uint32_t far_items) {
-    __shared__ uint32_t wsum[PLAN_THREADS / 64][3];
-    __shared__ uint32_t base_g, base_n, base_f;
+                                                       uint32_t *near_off, uint32_t *far_off, uint32_t *xf_off,
+                                                       RankPlan *plan, uint32_t par, uint32_t far_items) {
+    __shared__ uint32_t wsum[PLAN_THREADS / 64][4];
+    __shared__ uint32_t base[4];
     const uint32_t tid = threadIdx.x;
     // Commit the last iteration's missed-pivot swap, then the top blocks' new row
     // order (written by k_block_ge of this iteration, which saw the swap).  The
@@ -456,34 +456,31 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t n
         const uint32_t q = idx % BMAX;
         if (!S.done && q < I.Bt) S.perm[I.rank0 + q] = I.top[q];
     }
-    if (tid == 0) base_g = base_n = base_f = 0;
+    if (tid < 4) base[tid] = 0;
     __syncthreads();
+    uint32_t *const offs[4] = {gx_off, near_off, far_off, xf_off};
     for (uint32_t i0 = 0; i0 < nmat; i0 += PLAN_THREADS) {
         const uint32_t i = i0 + tid;
-        uint32_t g = 0, un = 0, uf = 0;
-        if (i < nmat) plan_counts(mats[i], par, far_items, g, un, uf);
-        uint32_t sg = g, sn = un, sf = uf;
-        block_incl_scan3<PLAN_THREADS>(sg, sn, sf, wsum, tid);
-        if (i < nmat) {
-            gx_off[i] = base_g + sg - g;
-            near_off[i] = base_n + sn - un;
-            far_off[i] = base_f + sf - uf;
-        }
+        uint32_t c[4] = {0, 0, 0, 0};
+        if (i < nmat) plan_counts(mats[i], par, far_items, c[0], c[1], c[2], c[3]);
+        uint32_t v[4] = {c[0], c[1], c[2], c[3]};
+        block_incl_scan4(v, wsum, tid);
+        if (i < nmat)
+#pragma unroll
+            for (int k = 0; k < 4; k++) offs[k][i] = base[k] + v[k] - c[k];
         __syncthreads();
-        if (tid == PLAN_THREADS - 1) {
-            base_g += sg;
-            base_n += sn;
-            base_f += sf;
-        }
+        if (tid == PLAN_THREADS - 1)
+#pragma unroll
+            for (int k = 0; k < 4; k++) base[k] += v[k];
         __syncthreads();
     }
     if (tid == 0) {
-        gx_off[nmat] = base_g;
-        near_off[nmat] = base_n;
-        far_off[nmat] = base_f;
-        plan->gx_total = base_g;
-        plan->near_total = base_n;
-        plan->far_total = base_f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) offs[k][nmat] = base[k];
+        plan->gx_total = base[0];
+        plan->near_total = base[1];
+        plan->far_total = base[2];
+        plan->xf_total = base[3];
     }
 }
 
@@ -686,7 +683,8 @@ void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldPara
 }
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_THREADS), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.plan,
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(PLAN_THREADS), 0, st, dmats, nmat, rl.gx_off, rl.near_off, rl.far_off,
+                       rl.xf_off, rl.plan,
                        par, rl.far_items);
 }
 
