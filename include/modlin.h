@@ -95,6 +95,8 @@ typedef struct modlin_seq_desc {
                                    context's captured hipGraph of it */
 #define MODLIN_FLAG_NO_DEFER 4u /* rank, MFMA method: update the far columns once per panel instead
                                    of once per two panels (the same ranks) */
+#define MODLIN_FLAG_U64_STORE 8u /* rank, MFMA method, p < 2^32: keep the working copy in u64 entries
+                                    instead of u32 (the same ranks) */
 
 typedef struct modlin_opts {
     uint32_t method; /* modlin_method.  BM: PLAIN runs one step per pass over c; AUTO and MFMA

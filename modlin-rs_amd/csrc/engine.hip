@@ -123,7 +123,8 @@ struct SetupJob {
     uint32_t *perm;
     uint64_t elems;  // n * n
     uint32_t n;
-    uint32_t form;   // 0: u64 in 16-byte pieces, 1: u64 one by one, 2: u32 widened
+    uint32_t form;   // 0: u64 in 16-byte pieces, 1: u64 one by one, 2: u32 widened to u64,
+                     // 3: u32 kept (u32 working copy), 4: u64 narrowed to u32 (entries below 2^32)
 };
 
 // All matrices of a batch in one launch: workgroup (x, y) covers part x of job y.
@@ -132,7 +133,15 @@ __global__ void k_setup(const SetupJob *jobs) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (J.src) {
-        if (J.form == 2) {
+        if (J.form == 3) {
+            const uint32_t *src = (const uint32_t *)J.src;
+            uint32_t *dst = (uint32_t *)J.M;
+            for (uint64_t i = t0; i < J.elems; i += stride) dst[i] = src[i];
+        } else if (J.form == 4) {
+            const uint64_t *src = (const uint64_t *)J.src;
+            uint32_t *dst = (uint32_t *)J.M;
+            for (uint64_t i = t0; i < J.elems; i += stride) dst[i] = (uint32_t)src[i];
+        } else if (J.form == 2) {
             const uint32_t *src = (const uint32_t *)J.src;
             for (uint64_t i = t0; i < J.elems; i += stride) J.M[i] = src[i];
         } else if (J.form == 1) {
@@ -167,6 +176,9 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     // far updates of two panels at once (MFMA method), unless the caller asks for
     // one per panel
     const bool defer = use_mfma && !(opts && (opts->flags & MODLIN_FLAG_NO_DEFER));
+    // entries below 2^32 are kept as u32 in the working copy (MFMA method): the
+    // trailing updates, which read and write it, move half the bytes
+    const bool m32 = use_mfma && p <= 0xffffffffull && !(opts && (opts->flags & MODLIN_FLAG_U64_STORE));
     const LimbParams lp = make_limb_params(p);
     const size_t KA = (size_t)lp.L * KSTEP;
 
@@ -208,14 +220,16 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             ctx->err = "null matrix pointer";
             return MODLIN_EINVAL;
         }
-        offM[i] = cur; cur = align_up(cur + n * n * 8, A);
+        offM[i] = cur; cur = align_up(cur + n * n * (m32 ? 4 : 8), A);
         offP[i] = cur; cur = align_up(cur + n * 4, A);
         offA[i] = cur; cur = align_up(cur + n * BMAX * 8, A);
         offX[i] = cur; cur = align_up(cur + BMAX * n * 8, A);
         offZ[i] = cur; cur = align_up(cur + BMAX * BMAX * 8, A);
         offE[i] = cur; cur = align_up(cur + ETA_WORDS * 8, A);
+        // stage for host entries whose width differs from the working copy's
         offS[i] = 0;
-        if (mats[i].elem == MODLIN_ELEM_U32) { offS[i] = cur; cur = align_up(cur + n * n * 4, A); }
+        const bool in32 = mats[i].elem == MODLIN_ELEM_U32;
+        if (in32 != m32 && mats[i].mem != MODLIN_MEM_DEVICE) { offS[i] = cur; cur = align_up(cur + n * n * (in32 ? 4 : 8), A); }
         offAl[i] = offXl[i] = offZl[i] = offAlP[i] = offXlP[i] = 0;
         if (use_mfma) {
             // digit columns are read in whole 64-column tiles: each X buffer is
@@ -257,6 +271,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.XlP = defer ? (int8_t *)(base + offXlP[i]) : nullptr;
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
+        S.m32 = m32 ? 1u : 0u;
         S.done = (n == 0) ? 1u : 0u;
         S.pend = 0xffffffffu;
         if (n == 0) continue;
@@ -269,14 +284,15 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         J.perm = S.perm;
         J.elems = elems;
         J.n = (uint32_t)n;
-        J.form = u32 ? 2u : (((uintptr_t)mats[i].data & 15) ? 1u : 0u);
+        J.form = m32 ? (u32 ? 3u : 4u) : (u32 ? 2u : (((uintptr_t)mats[i].data & 15) ? 1u : 0u));
         J.src = mats[i].data;
         if (!dev) {
-            // host entries: staged straight into M (u64) or into the stage (u32)
-            void *dst = u32 ? (void *)(base + offS[i]) : (void *)S.M;
+            // host entries: straight into M when the widths agree, else into the stage
+            const bool same = u32 == m32;
+            void *dst = same ? (void *)S.M : (void *)(base + offS[i]);
             rc = copy_h2d(ctx, dst, mats[i].data, elems * (u32 ? 4 : 8));
             if (rc) return rc;
-            J.src = u32 ? dst : nullptr;
+            J.src = same ? nullptr : dst;
         }
     }
     RankMat *dmats = (RankMat *)(base + 0);

@@ -171,10 +171,10 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 int8_t *out = pass ? S.AlP : S.Al;
                 for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
                     const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
-                    const uint64_t *mrow = S.M + (size_t)S.perm[I.rank + r0 + i] * ld;
+                    const size_t mrow = (size_t)S.perm[I.rank + r0 + i] * ld;
                     uint64_t x[4];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) x[j] = (k4 + j < kb) ? mrow[Jc[k4 + j]] : 0;
+                    for (int j = 0; j < 4; j++) x[j] = (k4 + j < kb) ? m_get(S, mrow + Jc[k4 + j]) : 0;
                     uint32_t w[L];
                     digits4<L>(x, p, w);
                     uint32_t *dst = (uint32_t *)(out + (size_t)(r0 + i) * KA + k4);
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 uint64_t x[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    x[j] = (sl < pb && a4 + j < dpb) ? S.M[(size_t)I.R[sl] * ld + I.dJ[a4 + j]] : 0;
+                    x[j] = (sl < pb && a4 + j < dpb) ? m_get(S, (size_t)I.R[sl] * ld + I.dJ[a4 + j]) : 0;
                 uint32_t w[L];
                 digits4<L>(x, p, w);
 #pragma unroll
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 for (int r = 0; r < 4; r++) {
                     const uint32_t sl = wr + 4 * (lane >> 4) + r;
                     const uint64_t pr = sub_mod(acc_residue<MODE, NC>(f, lp, acc, b, r), lp.k0, p);
-                    y[b][r] = (c < cols && sl < pb) ? sub_mod(S.M[(size_t)I.R[sl] * ld + I.c0 + cb + c], pr, p) : 0;
+                    y[b][r] = (c < cols && sl < pb) ? sub_mod(m_get(S, (size_t)I.R[sl] * ld + I.c0 + cb + c), pr, p) : 0;
                 }
             }
             __syncthreads();  // every wave is done with As and Bs
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 uint64_t x[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    x[j] = (c < cols && s4 + j < pb) ? S.M[(size_t)I.R[s4 + j] * ld + I.c0 + cb + c] : 0;
+                    x[j] = (c < cols && s4 + j < pb) ? m_get(S, (size_t)I.R[s4 + j] * ld + I.c0 + cb + c) : 0;
                 uint32_t w[L];
                 digits4<L>(x, p, w);
 #pragma unroll
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
             for (uint32_t e = tid; e < nr * nc; e += blockDim.x) {
                 const uint32_t i = e / nc, c = e - i * nc;
                 const uint32_t gcol = c0 + cb + c;
-                if (gcol < jend && S.M[(size_t)S.perm[rank + r0 + i] * ld + gcol] != 0) nzcol = min(nzcol, gcol);
+                if (gcol < jend && m_get(S, (size_t)S.perm[rank + r0 + i] * ld + gcol) != 0) nzcol = min(nzcol, gcol);
             }
         } else {
             // stage this tile's digits (already in registers) in LDS
@@ -361,19 +361,21 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
         }
         if (pb) {
             // rows of this lane's outputs and their current values (loaded early)
-            uint64_t *mrow[4];
+            size_t mrow[4];
+            bool rok[4];
             uint64_t cv[2][4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const uint32_t i = wr + 4 * (lane >> 4) + r;
-                mrow[r] = (i < nr) ? S.M + (size_t)S.perm[rank + r0 + i] * ld + c0 + cb : nullptr;
+                rok[r] = i < nr;
+                mrow[r] = rok[r] ? (size_t)S.perm[rank + r0 + i] * ld + c0 + cb : 0;
             }
 #pragma unroll
             for (int b = 0; b < 2; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const uint32_t c = wc + 16 * b + (lane & 15);
-                    cv[b][r] = (mrow[r] && c < nc) ? mrow[r][c] : 0;
+                    cv[b][r] = (rok[r] && c < nc) ? m_get(S, mrow[r] + c) : 0;
                 }
             v4i acc[NC][2];
             limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
@@ -383,9 +385,9 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const uint32_t c = wc + 16 * b + (lane & 15);
-                    if (!mrow[r] || c >= nc) continue;
+                    if (!rok[r] || c >= nc) continue;
                     const uint64_t res = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
-                    mrow[r][c] = res;
+                    m_set(S, mrow[r] + c, res);
                     const uint32_t gcol = c0 + cb + c;
                     if (res != 0 && gcol < jend) nzcol = min(nzcol, gcol);
                 }
@@ -447,17 +449,18 @@ constexpr uint32_t FSTEP = 32;  // columns per step
 // stores go through global pointers that the compiler cannot tell apart from
 // the RankMat array.
 struct FarItem {
-    gptr<uint64_t> Mg;
+    uint64_t *M;  // the working copy (u64 or, when m32, u32 entries), offset by c0 in far_item
+    uint32_t c0;
     gptr<const uint32_t> permg;
     gptr<const v4i> Ag[2], Xg[2];  // panel h: 0 the pending one, 1 this iteration's own
     uint32_t x, r0, nr, rank, ld, cbase, width, C;
 };
 
 // One item with the panels H0 .. H1 (compile-time, so every product's
-// accumulator operand is known).
-template <int L, int MODE, int H0, int H1>
+// accumulator operand is known) and entries of type T.
+template <int L, int MODE, int H0, int H1, typename T>
 __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const FieldParams &f, const LimbParams &lp,
-                                         gptr<uint64_t> tr, uint32_t tid, uint32_t lane, uint32_t wr) {
+                                         gptr<T> tr, uint32_t tid, uint32_t lane, uint32_t wr) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;           // LDS row stride: consecutive columns start 4 banks apart
     constexpr int NC = 2 * L - 1;
@@ -484,13 +487,14 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
         for (int la = 0; la < L; la++) af[h - H0][la] = ga[la * (KSTEP / 16)];
     }
     // This lane's rows; a row outside the band loads and stores through `tr`.
-    gptr<uint64_t> mrow[4];
+    const gptr<T> Mg = to_global((T *)it.M) + it.c0;
+    gptr<T> mrow[4];
     bool rok[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const uint32_t i = wr + 4 * (lane >> 4) + r;
         rok[r] = i < it.nr;
-        mrow[r] = it.Mg + (size_t)it.permg[it.rank + it.r0 + min(i, it.nr - 1)] * it.ld;
+        mrow[r] = Mg + (size_t)it.permg[it.rank + it.r0 + min(i, it.nr - 1)] * it.ld;
     }
     v4i rb[H1 - H0 + 1][NV];
     auto fetch_b = [&](uint32_t j) {
@@ -584,7 +588,7 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
                         v = limb_sub_m61_k2(cv[q][r], lp.kc2, P);
                     else
                         v = sub_mod(add_mod(cv[q][r], lp.k2, f.p), limb_residue_k2<MODE, NC>(f, lp, P), f.p);
-                    *((rok[r] && cok) ? mrow[r] + cb + c : tr) = v;
+                    *((rok[r] && cok) ? mrow[r] + cb + c : tr) = (T)v;
                 }
                 fetch_c(jn, q);
             }
@@ -602,6 +606,7 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
     const uint32_t total = plan->far_total;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const gptr<uint64_t> tr = to_global(trash) + tid;
+    const gptr<uint32_t> tr32 = to_global((uint32_t *)trash) + tid;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
         const RankMat &S = mats[mi];
@@ -615,7 +620,9 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         update_range(I, 1, it.cbase, it.width);
         it.C = cdiv(it.width, UT);
         if ((pb == 0 && !pair) || it.x >= it.C) continue;  // uniform: no pivots, or no tile
-        it.Mg = to_global(S.M) + I.c0;
+        it.M = S.M;
+        it.c0 = I.c0;
+        const bool m32 = uni32(S.m32) != 0;
         it.permg = to_global((const uint32_t *)S.perm);
         it.Ag[0] = to_global((const v4i *)S.AlP);
         it.Ag[1] = to_global((const v4i *)S.Al);
@@ -625,12 +632,21 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         it.nr = min(UT, O - it.r0);
         it.rank = I.rank;
         it.ld = S.ld;
-        if (!pair)
-            far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
-        else if (pb > 0)
-            far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
-        else
-            far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+        if (m32) {  // p < 2^32
+            if (!pair)
+                far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+            else if (pb > 0)
+                far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+            else
+                far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+        } else {
+            if (!pair)
+                far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+            else if (pb > 0)
+                far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+            else
+                far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+        }
     }
 }
 

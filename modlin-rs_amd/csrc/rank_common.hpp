@@ -49,7 +49,7 @@ struct IterState {
 // through perm (logical position -> physical row) so that pivoting swaps
 // indices instead of 8n-byte rows.
 struct RankMat {
-    uint64_t *M;       // working copy, n x n, row stride ld
+    uint64_t *M;       // working copy, n x n, row stride ld; u32 entries when m32
     uint32_t *perm;    // logical -> physical row
     uint64_t *Ab;      // gathered A operand M[O, J]: n x BMAX (row stride BMAX)
     uint64_t *X;       // A^-1 M[R, c0:], BMAX x n (row stride n)
@@ -62,6 +62,7 @@ struct RankMat {
     int8_t *AlP;       // IT_PAIR: int8 limb planes of M[O, J'] for the pending panel's columns J'
     int8_t *XlP;       // IT_DEFER: int8 limb planes of its X (read by its near update and the next PAIR)
     uint32_t n, ld;
+    uint32_t m32;            // M holds u32 entries (p < 2^32, MFMA method): half the update traffic
     uint32_t rank, col;      // pivots found; first column not yet eliminated
     uint32_t first_nz_col;   // smallest panel column with a nonzero residual
     uint32_t done;
@@ -175,6 +176,17 @@ __device__ __forceinline__ bool tile_geom(uint32_t O, uint32_t cbase, uint32_t w
     nr = min(UT, O - r0);
     nc = min(UT, cbase + width - cb);
     return true;
+}
+
+// Entry idx (row * ld + col) of the working copy, in either storage width.
+__device__ __forceinline__ uint64_t m_get(const RankMat &S, size_t idx) {
+    return S.m32 ? (uint64_t)((const uint32_t *)S.M)[idx] : S.M[idx];
+}
+__device__ __forceinline__ void m_set(const RankMat &S, size_t idx, uint64_t v) {
+    if (S.m32)
+        ((uint32_t *)S.M)[idx] = (uint32_t)v;
+    else
+        S.M[idx] = v;
 }
 
 // The X digit planes an iteration writes and its near update reads.

@@ -76,12 +76,12 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     {
         // 16 lanes per block row, 4 panel columns each: all loads issued first.
         const uint32_t q = tid >> 4, c0 = tid & 15;
-        const uint64_t *src = S.M + (size_t)rowid[q < Bt ? q : 0] * ld + col;
+        const size_t src = (size_t)rowid[q < Bt ? q : 0] * ld + col;
         uint64_t v[4];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const uint32_t c = c0 + 16 * m;
-            v[m] = (q < Bt && c < w) ? src[c] : 0;
+            v[m] = (q < Bt && c < w) ? m_get(S, src + c) : 0;
         }
 #pragma unroll
         for (int m = 0; m < 4; m++) U[q][c0 + 16 * m] = v[m];
@@ -647,7 +647,7 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
         __syncthreads();
         uint32_t mine = 0xffffffffu;
         for (uint32_t q = S.rank + tid; q < n; q += blockDim.x) {
-            if (S.M[(size_t)S.perm[q] * S.ld + jstar] != 0) { mine = q; break; }
+            if (m_get(S, (size_t)S.perm[q] * S.ld + jstar) != 0) { mine = q; break; }
         }
         if (mine != 0xffffffffu) atomicMin(&best, mine);
         __syncthreads();

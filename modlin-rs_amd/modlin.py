@@ -32,6 +32,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 FLAG_PROFILE = 1
 FLAG_NO_GRAPH = 2
 FLAG_NO_DEFER = 4
+FLAG_U64_STORE = 8
 P_FLOOR = 1e-307  # stats.rs:16
 LC_NOALLOC_MAX_N = 1 << 20  # MODLIN_LC_NOALLOC_MAX_N
 
@@ -281,13 +282,14 @@ class Context:
         return arr, keep
 
     def rank_batch(self, field: Field, mats: Sequence, method: int = METHOD_AUTO, panel: int = 0,
-                   profile: bool = False, graph: bool = True, defer: bool = True) -> list[int]:
+                   profile: bool = False, graph: bool = True, defer: bool = True, u32_store: bool = True) -> list[int]:
         """Ranks of a batch of square matrices.  defer=False updates the far columns
-        once per panel instead of once per two panels (MFMA method; same ranks)."""
+        once per panel instead of once per two panels; u32_store=False keeps a u64
+        working copy for p < 2^32 (MFMA method; the same ranks either way)."""
         arr, keep = self._descs(mats, True)
         out = np.zeros(max(1, len(mats)), dtype=np.uint64)
         opts = _Opts(method, panel, (FLAG_PROFILE if profile else 0) | (0 if graph else FLAG_NO_GRAPH)
-                     | (0 if defer else FLAG_NO_DEFER), 0)
+                     | (0 if defer else FLAG_NO_DEFER) | (0 if u32_store else FLAG_U64_STORE), 0)
         self._check(self._lib.modlin_rank_batch(self._h, ctypes.byref(field._f), arr, len(mats), ctypes.byref(opts),
                                                 _u64p(out)), "rank_batch")
         return [int(x) for x in out[:len(mats)]]

@@ -253,3 +253,37 @@ def test_deferred_far_update(ctx, p):
         assert _rank_all(ctx, p, mats, modlin.METHOD_MFMA, panel=panel) == want, panel
     got = ctx.rank_batch(modlin.Field(p), mats, method=modlin.METHOD_MFMA, defer=False)
     assert got == want
+
+
+@pytest.mark.parametrize("p", [65521, (1 << 31) - 1, 4294967291, 251])
+def test_u32_working_copy(ctx, p):
+    """For p < 2^32 the MFMA method keeps the working copy in u32 entries: u64 and
+    u32 inputs, in host and device memory, give the oracle's ranks, as does the u64
+    working copy (flag MODLIN_FLAG_U64_STORE)."""
+    rng = np.random.default_rng(p % 1009)
+    mats = [rng.integers(0, p, size=(n, n), dtype=np.uint64) for n in (64, 129, 300, 513)]
+    mats += [workloads.c3(450, p)]
+    r = 70
+    mats.append(modlin.combine_rows(p, rng.integers(0, p, size=(r, 380), dtype=np.uint64),
+                                    rng.integers(0, p, size=(380, r), dtype=np.uint64)))
+    mats += _hidden_pivot_cases(p, rng)[5:]
+    want = [oracle.rank(p, a, a.shape[0]) for a in mats]
+    f = modlin.Field(p)
+    m32 = [a.astype(np.uint32) for a in mats]
+    print(f"u32 store p={p} host u64 / u32", flush=True)
+    assert ctx.rank_batch(f, mats, method=modlin.METHOD_MFMA) == want
+    assert ctx.rank_batch(f, m32, method=modlin.METHOD_MFMA) == want
+    print(f"u32 store p={p} device u64 / u32 / mixed", flush=True)
+    dev = [ctx.to_device(a) for a in mats]
+    dev32 = [ctx.to_device(a) for a in m32]
+    try:
+        assert ctx.rank_batch(f, dev, method=modlin.METHOD_MFMA) == want
+        assert ctx.rank_batch(f, dev32, method=modlin.METHOD_MFMA) == want
+        mixed = [dev[i] if i % 3 == 0 else dev32[i] if i % 3 == 1 else m32[i] for i in range(len(mats))]
+        assert ctx.rank_batch(f, mixed, method=modlin.METHOD_MFMA) == want
+    finally:
+        for d in dev + dev32:
+            d.free()
+    print(f"u32 store p={p} u64 working copy", flush=True)
+    assert ctx.rank_batch(f, mats, method=modlin.METHOD_MFMA, u32_store=False) == want
+    assert ctx.rank_batch(f, m32, method=modlin.METHOD_MFMA, u32_store=False) == want
