@@ -329,6 +329,11 @@ def run(ctx, field, dev, method, steps, warmup):
     return ranks, dt / steps, st, prof_ranks
 
 
+EXTRA_LINES = ("c1_500_mixmax17", "single_8192_p61", "single_8192_p31", "single_16384_p61", "single_16384_p31",
+               "bm_batch_1024", "bm_long_200k", "rank_batch_256x2048_u32_p65521", "c5_mixed_bm200k_rank256x2048")
+FULL_LINES = ("single_32768_p61", "mixmax256_rank_65536")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,6 +346,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 256 MiB copy kernel")
     ap.add_argument("--full", action="store_true",
                     help="also the 32768^2 planted matrix and the 65536^2 MIXMAX-256 rank")
+    ap.add_argument("--only", default="",
+                    help="comma-separated lines to run, for profiling one workload at a time: "
+                         "c2 and/or names of the extra lines (the output then holds only those)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU only; --gpus above 1 is not supported "
@@ -348,11 +356,56 @@ def main():
         sys.exit(2)
     method = {"auto": M.METHOD_AUTO, "plain": M.METHOD_PLAIN, "mfma": M.METHOD_MFMA}[args.method]
 
+    only = set(x for x in args.only.split(",") if x)
+    unknown = only - set(EXTRA_LINES) - set(FULL_LINES) - {"c2"}
+    if unknown:
+        print(f"bench.py: unknown lines {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
     mats = W.c2()
+    ctx = M.Context(0)
+    out = {}
+    if not only or "c2" in only:
+        out = c2_line(ctx, mats, method, args)
+    extras = {
+        "c1_500_mixmax17": lambda: c1_line(ctx),
+        "single_8192_p61": lambda: single_large(ctx, 8192, M61, 2),
+        "single_8192_p31": lambda: single_large(ctx, 8192, (1 << 31) - 1, 2),
+        "single_16384_p61": lambda: single_large(ctx, 16384, M61, 1),
+        "single_16384_p31": lambda: single_large(ctx, 16384, (1 << 31) - 1, 1),
+        "bm_batch_1024": lambda: bm_batch(ctx, 2),
+        "bm_long_200k": lambda: bm_long(ctx, 1),
+        "rank_batch_256x2048_u32_p65521": lambda: rank_batch_u32(ctx, 2),
+        "c5_mixed_bm200k_rank256x2048": lambda: c5_mixed(),
+    }
+    full = {
+        "single_32768_p61": lambda: single_large(ctx, 32768, M61, 1, plain=False, warmup=0),
+        "mixmax256_rank_65536": lambda: mixmax256_rank(ctx),
+    }
+    assert tuple(extras) == EXTRA_LINES and tuple(full) == FULL_LINES
+    for name, fn in extras.items():
+        if (only and name in only) or (not only and not args.no_extras):
+            out[name] = fn()
+    for name, fn in full.items():
+        if (only and name in only) or (not only and args.full):
+            out[name] = fn()
+    if args.copy_calibration:
+        out["copy_calibration"] = copy_calibration(ctx)
+    # release the context (graphs, streams, events, pinned words, arena) while the
+    # runtime and any attached profiler are fully up, not from interpreter teardown
+    ctx.synchronize()
+    ctx.close()
+    if not args.no_cpu and (not only or "c2" in only):
+        out["cpu_baseline"] = cpu_baseline(mats)
+        if not args.no_extras:
+            out["cpu_baseline_bm"] = cpu_bm(W.c4()[0], W.c5_bm())
+    print(json.dumps(out))
+
+
+def c2_line(ctx, mats, method, args):
+    """The headline line: the 64-matrix batch over 2^61-1 from device memory."""
     sides = [m.shape[0] for m in mats]
     cols = sum(sides)
     field = M.Field(M61)
-    ctx = M.Context(0)
     dev = [ctx.to_device(m) for m in mats]
 
     ranks, t_step, st, _ = run(ctx, field, dev, method, args.steps, args.warmup)
@@ -403,26 +456,7 @@ def main():
         out["speedup_vs_plain"] = t_plain / t_step
     for d in dev:
         d.free()
-    if not args.no_extras:
-        out["c1_500_mixmax17"] = c1_line(ctx)
-        out["single_8192_p61"] = single_large(ctx, 8192, M61, 2)
-        out["single_8192_p31"] = single_large(ctx, 8192, (1 << 31) - 1, 2)
-        out["single_16384_p61"] = single_large(ctx, 16384, M61, 1)
-        out["single_16384_p31"] = single_large(ctx, 16384, (1 << 31) - 1, 1)
-        out["bm_batch_1024"] = bm_batch(ctx, 2)
-        out["bm_long_200k"] = bm_long(ctx, 1)
-        out["rank_batch_256x2048_u32_p65521"] = rank_batch_u32(ctx, 2)
-        out["c5_mixed_bm200k_rank256x2048"] = c5_mixed()
-    if args.full:
-        out["single_32768_p61"] = single_large(ctx, 32768, M61, 1, plain=False, warmup=0)
-        out["mixmax256_rank_65536"] = mixmax256_rank(ctx)
-    if args.copy_calibration:
-        out["copy_calibration"] = copy_calibration(ctx)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mats)
-        if not args.no_extras:
-            out["cpu_baseline_bm"] = cpu_bm(W.c4()[0], W.c5_bm())
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

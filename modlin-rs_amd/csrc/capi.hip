@@ -73,6 +73,9 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
         hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_ge, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fin, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_plan, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_gx, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc(&ctx->na_host, mdl::NA_HOST_BYTES, hipHostMallocDefault) != hipSuccess ||
         mdl::noalloc_init(ctx) != MODLIN_OK) {
         (void)hipGetLastError();
@@ -114,6 +117,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     if (ctx->aux) hipStreamSynchronize(ctx->aux);
+    if (ctx->aux2) hipStreamSynchronize(ctx->aux2);
     if (ctx->arena) hipFree(ctx->arena);
     mdl::release_pinned(ctx);
     mdl::release_generators(ctx);
@@ -126,7 +130,10 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (ctx->rank_graph) hipGraphExecDestroy(ctx->rank_graph);
     if (ctx->ev_ge) hipEventDestroy(ctx->ev_ge);
     if (ctx->ev_fin) hipEventDestroy(ctx->ev_fin);
+    if (ctx->ev_plan) hipEventDestroy(ctx->ev_plan);
+    if (ctx->ev_gx) hipEventDestroy(ctx->ev_gx);
     if (ctx->aux) hipStreamDestroy(ctx->aux);
+    if (ctx->aux2) hipStreamDestroy(ctx->aux2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

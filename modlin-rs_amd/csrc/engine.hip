@@ -22,6 +22,7 @@ void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldPara
                   uint32_t L, uint32_t defer);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
+void launch_collect(hipStream_t st, RankMat *dmats, uint32_t nmat, const uint32_t *active, uint64_t *out);
 void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
                          uint32_t par);
 void launch_update_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
@@ -206,6 +207,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + sizeof(RankPlan), A);
     const size_t off_trash = cur;
     cur = align_up(cur + TRASH_WORDS * sizeof(uint64_t), A);
+    const size_t off_out = cur;  // k_collect's output: active, then rank, macs, bytes per matrix
+    cur = align_up(cur + (1 + 3 * count) * sizeof(uint64_t), A);
     for (uint64_t i = 0; i < count; i++) {
         const uint64_t n = mats[i].n;
         if (n > 0xffffffffull / 2 || (n && n > (((uint64_t)1) << 62) / n / 8)) {
@@ -331,8 +334,11 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     hipStream_t aux = ctx->aux;
     // One block of check_every iterations: block elimination and inverse on the
     // aux stream once the previous finish has run; plan, gather, near update and
-    // finish on the main stream; the far update overlapping the next block
-    // elimination.  (check_every is even, so iteration parities repeat per block.)
+    // finish on the main stream; the far columns' X product (MFMA method) on the
+    // second aux stream from the plan on, beside the gather and near update; the
+    // far update once both are done, overlapping the next block elimination.
+    // (check_every is even, so iteration parities repeat per block.)
+    hipStream_t aux2 = ctx->aux2;
     auto enqueue_block = [&](bool prof) -> hipError_t {
         for (int s = 0; s < check_every; s++) {
             const uint32_t par = (uint32_t)(s & 1);
@@ -342,6 +348,14 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             if ((e = hipEventRecord(ctx->ev_ge, aux)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(st, ctx->ev_ge, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);
+            if (use_mfma) {
+                // far X: reads the plan, Z's digits (this parity) and the pivot rows,
+                // none of which the gather, near update or finish below writes
+                if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(aux2, ctx->ev_plan, 0)) != hipSuccess) return e;
+                launch_mfma(aux2, dmats, nmat, f, lp, rl, par, -2);
+                if ((e = hipEventRecord(ctx->ev_gx, aux2)) != hipSuccess) return e;
+            }
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, -1);
             else
@@ -354,7 +368,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             if (prof) hipEventRecord(evs[4 * s + 1], st);
             launch_finish(st, dmats, nmat, dactive, par);
             if ((e = hipEventRecord(ctx->ev_fin, st)) != hipSuccess) return e;
-            if (use_mfma) launch_mfma(st, dmats, nmat, f, lp, rl, par, -2);  // far X, off the panel chain
+            if (use_mfma && (e = hipStreamWaitEvent(st, ctx->ev_gx, 0)) != hipSuccess) return e;
             if (prof) hipEventRecord(evs[4 * s + 2], st);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, 1);
@@ -401,17 +415,36 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     // per panel (fp.rs:251): at every host check, the columns below each matrix's
     // current panel (all n once it is done), never decreasing (a missed pivot moves
     // a panel back).
-    uint64_t total_cols = 0, shown_cols = 0;
-    for (uint64_t i = 0; i < count; i++) total_cols += mats[i].n;
+    uint64_t total_cols = 0, shown_cols = 0, max_n = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        total_cols += mats[i].n;
+        max_n = std::max<uint64_t>(max_n, mats[i].n);
+    }
+    // Without progress reports or per-launch timing, the blocks that a batch with
+    // no missed pivot needs (one panel of b columns per iteration) are queued
+    // before the first host check, so that such a batch makes one round trip: the
+    // active count and the ranks come back in one copy (k_collect).
+    int ahead = 1;
+    if (!progress && !profile)
+        ahead = (int)std::max<uint64_t>(1, ((max_n + b - 1) / b + check_every - 1) / check_every);
+    uint64_t *dout = (uint64_t *)(base + off_out);
+    std::vector<uint64_t> hout(1 + 3 * count);
+    for (uint64_t i = 0; i < count; i++) hout[1 + 3 * i] = hm[i].rank;  // a batch with nothing active
     MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
     while (h_active > 0) {
-        if (use_graph)
-            MDL_HIP(hipGraphLaunch(ctx->rank_graph, st));
-        else
-            MDL_HIP(enqueue_block(profile));
-        iters += check_every;
-        rc = copy_d2h(ctx, &h_active, dactive, sizeof(uint32_t));
+        for (int k = 0; k < ahead; k++) {
+            if (use_graph)
+                MDL_HIP(hipGraphLaunch(ctx->rank_graph, st));
+            else
+                MDL_HIP(enqueue_block(profile));
+            iters += check_every;
+        }
+        ahead = 1;
+        launch_collect(st, dmats, nmat, dactive, dout);
+        MDL_HIP(hipGetLastError());
+        rc = copy_d2h(ctx, hout.data(), dout, hout.size() * sizeof(uint64_t));
         if (rc) return rc;
+        h_active = (uint32_t)hout[0];
         if (profile) {
             for (int s = 0; s < check_every; s++) {
                 float ms0 = 0, ms1 = 0;
@@ -434,14 +467,13 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     }
     if (progress) progress(user, total_cols, total_cols);
     MDL_HIP(hipStreamSynchronize(aux));
+    MDL_HIP(hipStreamSynchronize(aux2));
     for (auto &e : evs) hipEventDestroy(e);
-    rc = copy_d2h(ctx, hm, dmats, count * sizeof(RankMat));
-    if (rc) return rc;
     double macs = 0, ubytes = 0;
     for (uint64_t i = 0; i < count; i++) {
-        ranks[i] = hm[i].rank;
-        macs += (double)hm[i].macs;
-        ubytes += (double)hm[i].upd_bytes;
+        ranks[i] = hout[1 + 3 * i];
+        macs += (double)hout[2 + 3 * i];
+        ubytes += (double)hout[3 + 3 * i];
     }
     ctx->stats.macs = macs;
     ctx->stats.update_bytes = ubytes;

@@ -319,15 +319,21 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
         RankMat &S = mats[cur.mi];
         const IterState &I = S.it[par];
         const uint32_t r0 = cur.r0, cb = cur.cb, nr = cur.nr, nc = cur.nc, pb = cur.pb, ld = S.ld;
-        const uint32_t rank = I.rank, c0 = I.c0, jend = I.jend;
+        const uint32_t rank = I.rank, c0 = I.c0;
         uint32_t nzcol = 0xffffffffu;
+        const uint32_t scan_end = I.scan_end;
         if (pb == 0) {
-            // detection only (no pivots): scan the tile for a nonzero panel column
-            for (uint32_t e = tid; e < nr * nc; e += blockDim.x) {
-                const uint32_t i = e / nc, c = e - i * nc;
-                const uint32_t gcol = c0 + cb + c;
-                if (gcol < jend && m_get(S, (size_t)S.perm[rank + r0 + i] * ld + gcol) != 0) nzcol = min(nzcol, gcol);
-            }
+            // detection only (no pivots): scan the tile for a nonzero column below
+            // scan_end; a tile wholly right of a nonzero column already found has
+            // nothing to add
+            const uint32_t known = __hip_atomic_load(&S.first_nz_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c0 + cb <= known)
+                for (uint32_t e = tid; e < nr * nc; e += blockDim.x) {
+                    const uint32_t i = e / nc, c = e - i * nc;
+                    const uint32_t gcol = c0 + cb + c;
+                    if (gcol < scan_end && m_get(S, (size_t)S.perm[rank + r0 + i] * ld + gcol) != 0)
+                        nzcol = min(nzcol, gcol);
+                }
         } else {
             // stage this tile's digits (already in registers) in LDS
             __syncthreads();
@@ -389,7 +395,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                     const uint64_t res = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
                     m_set(S, mrow[r] + c, res);
                     const uint32_t gcol = c0 + cb + c;
-                    if (res != 0 && gcol < jend) nzcol = min(nzcol, gcol);
+                    if (res != 0 && gcol < scan_end) nzcol = min(nzcol, gcol);
                 }
         }
         if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);

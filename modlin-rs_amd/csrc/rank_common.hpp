@@ -37,7 +37,8 @@ struct IterState {
     uint32_t dpb;            // IT_PAIR: pivots of the pending panel
     uint32_t dcol;           // IT_PAIR: first column the pending panel has not updated (= split)
     uint32_t dxoff;          // IT_PAIR: c0 minus the pending panel's c0 (its X digit columns)
-    uint32_t pad;
+    uint32_t scan_end;       // resid: columns [col, scan_end) are searched for a missed pivot --
+                             // the panel (jend), or, for a panel without pivots, every column left (n)
     uint32_t J[BMAX];        // pivot columns of the panel (absolute)
     uint32_t R[BMAX];        // physical pivot rows, in pivot order
     uint32_t top[BMAX];      // physical rows of the top block in their new order

@@ -204,6 +204,7 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         I.pb = pb;
         I.resid = resid;
         I.kind = kind;
+        I.scan_end = jend;
         if (kind == IT_PAIR) {
             // near part [c0, dcol): this panel only; far part [dcol, n): both panels
             I.split = S.dcol;
@@ -226,9 +227,20 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
             if (pb > 0) {
                 I.c0 = resid ? col : jend;
                 I.cend = n;
+            } else if (resid) {
+                // No pivot in the top block and no update to apply: search every
+                // column left (the rows below the block included), so that a run of
+                // eliminated columns -- the whole rest of a rank-deficient matrix --
+                // costs one pass instead of one iteration per panel.  The first
+                // nonzero column is the reference's next pivot column, the first
+                // nonzero row in it its pivot row (fp.rs:150-163).
+                I.c0 = col;
+                I.cend = n;
+                I.split = n;
+                I.scan_end = n;
             } else {
                 I.c0 = col;
-                I.cend = resid ? jend : col;  // detection only, or nothing at all
+                I.cend = col;  // nothing at all
             }
         }
         S.rank = rank0 + pb;
@@ -561,6 +573,9 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
         uint32_t cbase, cols, r0, cb, nr, nc;
         update_range(I, far, cbase, cols);
         if (!tile_geom(O, cbase, cols, far, lt, r0, cb, nr, nc)) continue;  // padding tile (uniform)
+        // detection only, right of a nonzero column already found: nothing to add
+        if (pb == 0 && I.c0 + cb > __hip_atomic_load(&S.first_nz_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            continue;
         __syncthreads();
         for (uint32_t idx = tid; idx < UT * pb; idx += blockDim.x) {
             uint32_t i = idx / pb, kk = idx - i * pb;
@@ -623,7 +638,7 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
                 uint64_t v = sub_mod(mrow[cc], tot[a][c], p);
                 if (pb > 0) mrow[cc] = v;
                 const uint32_t gcol = I.c0 + cb + cc;
-                if (v != 0 && gcol < I.jend) nzcol = min(nzcol, gcol);
+                if (v != 0 && gcol < I.scan_end) nzcol = min(nzcol, gcol);
             }
         }
         if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
@@ -641,7 +656,7 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n;
     uint32_t jstar = S.first_nz_col;
-    const bool miss = I.resid && jstar < I.jend;
+    const bool miss = I.resid && jstar < I.scan_end;
     if (miss) {
         if (tid == 0) best = 0xffffffffu;
         __syncthreads();
@@ -657,11 +672,26 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
         if (tid == 0 && best != 0xffffffffu && best != S.rank) S.pend = best;
     }
     if (tid == 0) {
-        S.col = miss ? jstar : I.jend;
+        S.col = miss ? jstar : I.scan_end;  // (scan_end = n: no nonzero left, the rank is final)
         if (S.col >= n || S.rank >= n) {
             S.done = 1;
             atomicSub(active, 1u);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_collect: the active count and every matrix's rank and counters, packed for
+// one copy to the host: out[0] = active, out[1 + 3 i ..] = rank, macs, bytes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t nmat, const uint32_t *active,
+                                                 uint64_t *out) {
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) out[0] = *active;
+    for (uint32_t i = tid; i < nmat; i += blockDim.x) {
+        out[1 + 3 * (size_t)i] = mats[i].rank;
+        out[2 + 3 * (size_t)i] = mats[i].macs;
+        out[3 + 3 * (size_t)i] = mats[i].upd_bytes;
     }
 }
 
@@ -690,6 +720,10 @@ void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch
 
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par) {
     hipLaunchKernelGGL(k_finish, dim3(nmat), dim3(1024), 0, st, dmats, active, par);
+}
+
+void launch_collect(hipStream_t st, RankMat *dmats, uint32_t nmat, const uint32_t *active, uint64_t *out) {
+    hipLaunchKernelGGL(k_collect, dim3(1), dim3(256), 0, st, dmats, nmat, active, out);
 }
 
 void launch_gather_plain(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, const RankLaunch &rl,
