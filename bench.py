@@ -25,6 +25,7 @@ size that a counter run uses to scale FETCH_SIZE / WRITE_SIZE to bytes.
 """
 import argparse
 import json
+import platform
 import os
 import sys
 import time
@@ -56,60 +57,154 @@ def _oracle():
     return O
 
 
-def _rank_sample(O, mats, budget_n3):
-    sample, acc = [], 0.0
-    for m in mats:
-        n = m.shape[0]
-        if sample and acc + float(n) ** 3 > budget_n3:
-            break
-        sample.append(m)
-        acc += float(n) ** 3
-    t0 = time.perf_counter()
-    for m in sample:
-        O.rank(M61, m, m.shape[0])
-    dt = time.perf_counter() - t0
-    cols = sum(m.shape[0] for m in sample)
-    return cols / dt, f"first {len(sample)} matrices of the batch (sum n = {cols}, sum n^3 = {acc:.3g}), {dt:.1f} s"
+_NATIVE = []
 
 
-def cpu_baseline(mats, budget_n3=4.0e10, budget_n3_1t=1.0e10):
-    """The oracle's port of fp::rank (OpenMP over rows, as rayon does) on the
-    first matrices of the batch whose sum of n^3 stays within the budget, on all
-    host threads; and a smaller sample on one thread and on all threads (columns/s
-    depend on the sides, so the two thread counts are compared on one sample)."""
-    O = _oracle()
-    cores = O.num_threads()
-    v, sample = _rank_sample(O, mats, budget_n3)
-    va, _ = _rank_sample(O, mats, budget_n3_1t)
-    O.set_num_threads(1)
+def _native_oracle():
+    """The oracle compiled for this host (-march=native), built once per run."""
+    if not _NATIVE:
+        import tempfile
+        O = _oracle()
+        _NATIVE.append(O.load(O.build_native(tempfile.mkdtemp(prefix="modlin_oracle_"))))
+    return _NATIVE[0]
+
+
+def cpu_model() -> str:
     try:
-        v1, sample1 = _rank_sample(O, mats, budget_n3_1t)
-    finally:
-        O.set_num_threads(cores)
-    return {"value": v, "unit": "columns/s", "cores": cores, "kind": "port", "sample": sample,
-            "single_thread": {"value": v1, "unit": "columns/s", "cores": 1, "sample": sample1,
-                              "all_threads_same_sample": va}}
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
-def cpu_bm(seqs, long_seq, budget_s=4.0, long_prefix=20000):
-    """The oracle's port of fp::linear_complexity (one thread, as the reference) on
-    the batch's sequences in order until the time budget is spent, and on the
-    first long_prefix steps of the long sequence (the same first steps as the
-    full run, with l still small, so a lower bound of its time per step)."""
+def _rank_run(O, lib, mats, threads, over_matrices):
+    """Seconds to rank mats with the oracle library lib: one after another with
+    the update's row loop on `threads` OpenMP threads, or (over_matrices) the
+    matrices spread over `threads` host threads, one OpenMP thread each."""
+    import threading
+    if not over_matrices:
+        lib.or_set_num_threads(threads)
+        t0 = time.perf_counter()
+        for m in mats:
+            O.rank(M61, m, m.shape[0], lib=lib)
+        return time.perf_counter() - t0
+    order = sorted(range(len(mats)), key=lambda i: -mats[i].shape[0])  # largest first
+    lock, nxt = threading.Lock(), [0]
+
+    def worker():
+        lib.or_set_num_threads(1)  # this thread's OpenMP team
+        while True:
+            with lock:
+                if nxt[0] >= len(order):
+                    return
+                i = order[nxt[0]]
+                nxt[0] += 1
+            O.rank(M61, mats[i], mats[i].shape[0], lib=lib)
+
+    th = [threading.Thread(target=worker) for _ in range(threads)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(mats, w_target=2e10, w_cap=6e9, full=False):
+    """The oracle's port of fp::rank on the host cores, from a -march=native
+    build compiled here (gcc -O3 -fopenmp).  The sample is the batch's matrices in
+    batch order, each of work W <= w_cap, until their W reaches w_target (the
+    whole batch with full=True), W being the oracle's work count (the dense
+    Schur-complement multiply-accumulates of a matrix, from the batch's fixture):
+      value           the reference's way: one matrix after another, the update's
+                      row loop over all host threads (OpenMP, as rayon in
+                      fp.rs:202-248);
+      batch_parallel  the same matrices spread over the host threads, one each;
+      single_thread   one matrix after another on one thread, on the part of the
+                      sample up to w_target / 4;
+      portable_build  the x86-64-v3 library the tests load, as value.
+    Each leg projects the whole batch's time from its rate per unit of W; the
+    batch's largest matrices parallelise better than the sample's, so for the
+    multi-threaded legs the projection is an upper bound."""
     O = _oracle()
+    fx = fixture("c2_coranks.json")
+    work = fx["work"] if fx else [float(m.shape[0]) ** 3 / 3 for m in mats]
+    w_all = float(sum(work))
+    native = _native_oracle()
+    portable = O.load()
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, native.or_num_threads())
+
+    def pick(target):
+        idx, acc = [], 0.0
+        for i, w in enumerate(work):
+            if full:
+                idx.append(i)
+            elif w <= w_cap and acc < target:
+                idx.append(i)
+                acc += w
+        return idx
+
+    def leg(lib, idx, threads, over):
+        sub = [mats[i] for i in idx]
+        dt = _rank_run(O, lib, sub, threads, over)
+        cols, w = sum(m.shape[0] for m in sub), float(sum(work[i] for i in idx))
+        return {"value": cols / dt, "unit": "columns/s", "cores": threads, "seconds": dt, "matrices": len(idx),
+                "sample": f"{len(idx)} of {len(mats)} matrices (sum n = {cols}, W = {w:.3g} of {w_all:.3g})",
+                "projected_batch_s": dt * w_all / w}
+
+    big, small = pick(w_target), pick(w_target / 4)
+    warm = np.random.default_rng(1).integers(0, M61, size=(256, 256), dtype=np.uint64)
+    for lib in (native, portable):  # thread pools and code paged in before any timing
+        lib.or_set_num_threads(cores)
+        O.rank(M61, warm, 256, lib=lib)
+    ref = leg(native, big, cores, False)
+    out = {"value": ref["value"], "unit": "columns/s", "cores": cores, "kind": "port",
+           "sample": ref["sample"] + " in batch order, each W <= %.3g; one after another, update rows on all "
+                                     "threads" % w_cap,
+           "cpu_model": cpu_model(), "build": "gcc -O3 -march=native -fopenmp (compiled on this host)",
+           "seconds": ref["seconds"], "projected_batch_s": ref["projected_batch_s"],
+           "batch_parallel": leg(native, big, cores, True),
+           "single_thread": leg(native, small, 1, False),
+           "portable_build": dict(leg(portable, big, cores, False), build="gcc -O3 -march=x86-64-v3 -fopenmp")}
+    portable.or_set_num_threads(cores)
+    native.or_set_num_threads(cores)
+    return out
+
+
+def cpu_bm(seqs, long_seq, budget_s=4.0, prefixes=(20000, 40000)):
+    """The oracle's port of fp::linear_complexity (one thread, as the reference),
+    from the -march=native build: on the batch's sequences in order until the time
+    budget is spent, and on two prefixes of the long sequence.  A BM step costs
+    about twice the current length l, and l grows as n / 2 until it reaches the
+    sequence's complexity (65,280 here, at n = 130,560), so the full run's time
+    is projected from the prefixes as t(n) = b n^2 up to 2 l_max, then linear."""
+    O, lib = _oracle(), _native_oracle()
+    lib.or_set_num_threads(1)
     t0, steps, k = time.perf_counter(), 0, 0
     while k < len(seqs) and time.perf_counter() - t0 < budget_s:
-        O.lc(M61, seqs[k])
+        O.lc(M61, seqs[k], lib=lib)
         steps += seqs[k].size
         k += 1
     dt = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    O.lc(M61, long_seq[:long_prefix])
-    dl = time.perf_counter() - t1
+    tp = []
+    for n in prefixes:
+        t1 = time.perf_counter()
+        O.lc(M61, long_seq[:n], lib=lib)
+        tp.append(time.perf_counter() - t1)
+    n0, n1 = prefixes
+    b = (tp[1] - tp[0]) / (n1 * n1 - n0 * n0)
+    knee, n = 2 * 65280, long_seq.size
+    proj = b * knee * knee + (n - knee) * 2 * b * knee
+    lib.or_set_num_threads(lib.or_num_threads())
     return {"bm_batch": {"value": steps / dt, "unit": "steps/s", "cores": 1, "kind": "port",
                          "sample": f"first {k} sequences of the batch ({steps} steps), {dt:.1f} s"},
-            "bm_long": {"value": dl * 1e6 / long_prefix, "unit": "us/step", "cores": 1, "kind": "port",
-                        "sample": f"first {long_prefix} of the 200000 steps, {dl:.2f} s"}}
+            "bm_long": {"value": tp[1] * 1e6 / n1, "unit": "us/step", "cores": 1, "kind": "port",
+                        "sample": f"first {n1} of the {n} steps, {tp[1]:.2f} s (first {n0}: {tp[0]:.2f} s)",
+                        "steps_20k_40k_us_per_step": (tp[1] - tp[0]) * 1e6 / (n1 - n0),
+                        "projected_full_s": proj, "projected_us_per_step": proj * 1e6 / n}}
 
 
 def traffic_per_launch():
@@ -195,18 +290,19 @@ def c1_line(ctx, steps=20):
         out[f"{name}_columns_per_s"] = 500 / t
         out[f"{name}_corank"] = 500 - ranks[0]
     dev[0].free()
-    O = _oracle()
-    cores = O.num_threads()
+    O, lib = _oracle(), _native_oracle()
+    cores = lib.or_num_threads()
     for label, th in (("cpu_port_1t", 1), ("cpu_port_all", cores)):
-        O.set_num_threads(th)
+        lib.or_set_num_threads(th)
         t0 = time.perf_counter()
         reps = 0
         while time.perf_counter() - t0 < 1.0:
-            O.rank(M61, mm, 500)
+            O.rank(M61, mm, 500, lib=lib)
             reps += 1
         out[f"{label}_columns_per_s"] = 500 * reps / (time.perf_counter() - t0)
-    O.set_num_threads(cores)
+    lib.or_set_num_threads(cores)
     out["cpu_threads"] = cores
+    out["cpu_build"] = "gcc -O3 -march=native -fopenmp (compiled on this host)"
     return out
 
 
@@ -346,6 +442,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 256 MiB copy kernel")
     ap.add_argument("--full", action="store_true",
                     help="also the 32768^2 planted matrix and the 65536^2 MIXMAX-256 rank")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the CPU baseline on the whole batch (minutes) instead of a leading sample")
     ap.add_argument("--only", default="",
                     help="comma-separated lines to run, for profiling one workload at a time: "
                          "c2 and/or names of the extra lines (the output then holds only those)")
@@ -395,7 +493,7 @@ def main():
     ctx.synchronize()
     ctx.close()
     if not args.no_cpu and (not only or "c2" in only):
-        out["cpu_baseline"] = cpu_baseline(mats)
+        out["cpu_baseline"] = cpu_baseline(mats, full=args.cpu_full)
         if not args.no_extras:
             out["cpu_baseline_bm"] = cpu_bm(W.c4()[0], W.c5_bm())
     print(json.dumps(out))

@@ -135,7 +135,11 @@ typedef struct modlin_tuning {
     uint32_t bm_coop_wgs; /* workgroups of the cooperative BM, 1..1024 (default: one per CU) */
     uint32_t bm_threads;  /* threads per sequence of the batched BM: 256, 512 or 1024 (default:
                              the most that keep the whole batch resident) */
-    uint32_t reserved[1];
+    uint32_t far_depth;   /* MFMA method: panels per far update, 1 or 2 (default 2: a clean panel
+                             leaves its far columns to the next one, K up to 128 per pass) */
+    uint32_t far_quarters; /* MFMA method, 4 digits or fewer (p < 2^32): 16-column quarters per
+                              B fragment read in the far update, 1 or 2 (default 2) */
+    uint32_t reserved[3];
 } modlin_tuning_t;
 
 /* --- Field (src/fp.rs:49-79) -------------------------------------------- */
@@ -153,6 +157,7 @@ int modlin_ctx_last_stats(const modlin_ctx_t *ctx, modlin_stats_t *out);
 
 /* Tuning.  A table file has one entry per line,
  *     <rank|bm> <mode> <side> <count> far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs bm_threads
+ *         [far_depth far_quarters]
  * with mode m61 (p = 2^61-1), m31 (p = 2^31-1) or gL (other p, L int8 digits per
  * element), side the power of two at or above the largest n of a call (at least
  * 256), count 1 or many; '#' starts a comment.  A call uses the entry of its kind,

@@ -8,8 +8,10 @@ tuning.txt from the library's directory when it is created.
 
 Search: coordinate descent from the built-in defaults, two passes over the
 parameters (far-update workgroups per CU, near-update workgroups per CU, gather
-grid, panel iterations per host check, panel width; threads per sequence of
-the batched Berlekamp-Massey, workgroups of the cooperative one).  Each candidate is timed as the best of `reps`
+grid, panel iterations per host check, panel width, panels per far update (K
+depth 64 or 128), and for p < 2^32 the far update's limb-class grouping
+(quarters per B fragment); threads per sequence of the batched
+Berlekamp-Massey, workgroups of the cooperative one).  Each candidate is timed as the best of `reps`
 runs after one warm-up run.  Launch parameters never change a result: every
 candidate's ranks (or linear complexity) must equal those of the untuned run,
 or the tuner stops with an error.
@@ -34,9 +36,13 @@ RANK_SPACE = {
     "far_per_cu": [2, 4, 6, 8],
     "near_per_cu": [1, 2, 4],
     "gather_wgs": [512, 1024, 2048],
-    "check_every": [8, 16],
+    "check_every": [4, 8, 16],
     "panel": [48, 64],
+    "far_depth": [1, 2],      # K per far pass: one panel (<= 64) or two (<= 128)
 }
+# fields (p < 2^32, 4 digits or fewer): quarters of the far update's column step
+# computed per B fragment read -- how many limb-class accumulator sets are live
+RANK_SPACE_SMALL = dict(RANK_SPACE, far_quarters=[1, 2])
 BM_SPACE = {"bm_coop_wgs": [128, 256, 512]}
 BM_BATCH_SPACE = {"bm_threads": [256, 512, 1024]}
 
@@ -128,12 +134,14 @@ def main():
     mats = W.c2()
     jobs.append((Job(ctx, "rank", M61, [ctx.to_device(m) for m in mats], "64-matrix batch, sides <= 4096, 2^61-1"),
                  RANK_SPACE))
-    for n, p in ((8192, M61), (8192, P31)) + (() if args.quick else ((16384, M61),)):
+    mm, _ = W.c1()
+    jobs.append((Job(ctx, "rank", M61, [ctx.to_device(mm)], "500^2 MIXMAX-17 (rank 68), 2^61-1"), RANK_SPACE))
+    for n, p in ((8192, M61), (8192, P31)) + (() if args.quick else ((16384, M61), (16384, P31))):
         jobs.append((Job(ctx, "rank", p, [ctx.to_device(W.c3(n, p))], f"{n}^2 planted corank, {mode_of(p)}"),
-                     RANK_SPACE))
+                     RANK_SPACE if p == M61 else RANK_SPACE_SMALL))
     small = W.c5_rank(count=256)
     jobs.append((Job(ctx, "rank", 65521, [ctx.to_device(m) for m in small], "256 x 2048^2 u32, p = 65521"),
-                 RANK_SPACE))
+                 RANK_SPACE_SMALL))
     seqs, _ = W.c4()
     # bm_threads sizes the one-step-per-pass kernel (method PLAIN); the default,
     # blocked form has no launch parameter to tune
@@ -144,7 +152,8 @@ def main():
                      BM_SPACE))
 
     lines = ["# Launch parameters chosen by modlin-rs_amd/autotune.py on one MI355X.",
-             "# kind mode side count far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs bm_threads"]
+             "# kind mode side count far_per_cu near_per_cu gather_wgs check_every panel bm_coop_wgs bm_threads"
+             " far_depth far_quarters"]
     for job, space in jobs:
         params, t0, t1 = tune(job, space, args.reps, log)
         lines.append(f"{entry_line(job, params)}   # {job.label}: {t0 * 1e3:.2f} -> {t1 * 1e3:.2f} ms")

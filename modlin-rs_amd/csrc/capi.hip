@@ -92,8 +92,10 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
 
 int modlin_ctx_set_tuning(modlin_ctx_t *ctx, const modlin_tuning_t *t) {
     if (!ctx) return MODLIN_EINVAL;
-    if (t && ((t->check_every & 1) || t->panel > 64 || t->bm_coop_wgs > 1024 || !mdl::bm_threads_ok(t->bm_threads))) {
-        ctx->err = "tuning: check_every must be even, panel <= 64, bm_coop_wgs <= 1024, bm_threads 256, 512 or 1024";
+    if (t && ((t->check_every & 1) || t->panel > 64 || t->bm_coop_wgs > 1024 || !mdl::bm_threads_ok(t->bm_threads) ||
+              t->far_depth > 2 || t->far_quarters > 2)) {
+        ctx->err = "tuning: check_every must be even, panel <= 64, bm_coop_wgs <= 1024, bm_threads 256, 512 or 1024, "
+                   "far_depth and far_quarters 1 or 2";
         return MODLIN_EINVAL;
     }
     ctx->tune_override = t ? *t : modlin_tuning_t{};

@@ -176,7 +176,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     const bool use_mfma = method != MODLIN_METHOD_PLAIN;
     // far updates of two panels at once (MFMA method), unless the caller asks for
     // one per panel
-    const bool defer = use_mfma && !(opts && (opts->flags & MODLIN_FLAG_NO_DEFER));
+    const bool defer = use_mfma && !(opts && (opts->flags & MODLIN_FLAG_NO_DEFER)) && tu.far_depth != 1;
     // entries below 2^32 are kept as u32 in the working copy (MFMA method): the
     // trailing updates, which read and write it, move half the bytes
     const bool m32 = use_mfma && p <= 0xffffffffull && !(opts && (opts->flags & MODLIN_FLAG_U64_STORE));
@@ -322,6 +322,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     // far grid: a multiple of 8 (XCD numbering)
     rl.grid_far = ((uint32_t)ctx->cu_count * (tu.far_per_cu ? tu.far_per_cu : (use_mfma ? 4 : 8)) + 7) / 8 * 8;
     rl.far_items = use_mfma ? 1u : 0u;
+    rl.far_qw = tu.far_quarters ? tu.far_quarters : 2u;
     const int check_every = tu.check_every ? (int)tu.check_every : 8;
     std::vector<hipEvent_t> evs;
     if (profile) {
@@ -382,9 +383,9 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     bool use_graph = !profile && !(opts && (opts->flags & MODLIN_FLAG_NO_GRAPH));
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof key, "%p/%u/%u/%d/%d/%llu/%u/%u/%u/%u/%d", (void *)base, nmat, b, (int)use_mfma,
+        snprintf(key, sizeof key, "%p/%u/%u/%d/%d/%llu/%u/%u/%u/%u/%d/%u", (void *)base, nmat, b, (int)use_mfma,
                  (int)defer, (unsigned long long)p, rl.grid_gx, rl.grid_near, rl.grid_far, (unsigned)lp.L,
-                 check_every);
+                 check_every, rl.far_qw);
         if (!ctx->rank_graph || ctx->rank_graph_key != key) {
             if (ctx->rank_graph) {
                 hipGraphExecDestroy(ctx->rank_graph);

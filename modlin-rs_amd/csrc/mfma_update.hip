@@ -464,7 +464,7 @@ struct FarItem {
 
 // One item with the panels H0 .. H1 (compile-time, so every product's
 // accumulator operand is known) and entries of type T.
-template <int L, int MODE, int H0, int H1, typename T>
+template <int L, int MODE, int QW, int H0, int H1, typename T>
 __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const FieldParams &f, const LimbParams &lp,
                                          gptr<T> tr, uint32_t tid, uint32_t lane, uint32_t wr) {
     constexpr uint32_t KA = L * KSTEP;
@@ -475,7 +475,6 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
     constexpr uint32_t NV = (NCH + FAR_THREADS - 1) / FAR_THREADS;
     constexpr uint32_t TILE = FSTEP * KAP;      // LDS bytes of one panel tile
     constexpr uint32_t BUF = 2 * TILE;
-    constexpr int QW = L >= 5 ? 1 : 2;          // quarters computed together
     const uint32_t koff = 16 * (lane >> 4);
     const uint32_t cbase = it.cbase, width = it.width, x = it.x;
     // Steps: tile x + 8 (j / 2), half j % 2; the last tile's second half only if
@@ -603,7 +602,9 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
     }
 }
 
-template <int L, int MODE>
+// QW: 16-column quarters computed together per B fragment read (the limb-class
+// accumulators of QW quarters are live at once: 2 only for L <= 4).
+template <int L, int MODE, int QW>
 __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, uint32_t nmat, const uint32_t *item_off,
                                                                const RankPlan *plan, FieldParams f, LimbParams lp,
                                                                uint32_t par, uint64_t *trash) {
@@ -640,18 +641,18 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         it.ld = S.ld;
         if (m32) {  // p < 2^32
             if (!pair)
-                far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
             else if (pb > 0)
-                far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
             else
-                far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
         } else {
             if (!pair)
-                far_item<L, MODE, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
             else if (pb > 0)
-                far_item<L, MODE, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
             else
-                far_item<L, MODE, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
         }
     }
 }
@@ -666,9 +667,12 @@ static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const F
     else if (what == 0)
         hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(rl.grid_near), dim3(512), 0, st, dmats, nmat, rl.near_off,
                            rl.plan, f, lp, par, 0u);
+    else if (L <= 4 && rl.far_qw != 1)
+        hipLaunchKernelGGL((k_update_far<L, MODE, (L <= 4 ? 2 : 1)>), dim3(rl.grid_far), dim3(FAR_THREADS), 0, st, dmats,
+                           nmat, rl.far_off, rl.plan, f, lp, par, rl.trash);
     else
-        hipLaunchKernelGGL((k_update_far<L, MODE>), dim3(rl.grid_far), dim3(FAR_THREADS), 0, st, dmats, nmat, rl.far_off,
-                           rl.plan, f, lp, par, rl.trash);
+        hipLaunchKernelGGL((k_update_far<L, MODE, 1>), dim3(rl.grid_far), dim3(FAR_THREADS), 0, st, dmats, nmat,
+                           rl.far_off, rl.plan, f, lp, par, rl.trash);
 }
 
 // what = -1: digit gather and the near X product; -2: the far X product; 0: near

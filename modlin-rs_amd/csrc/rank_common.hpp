@@ -109,6 +109,7 @@ struct RankLaunch {
     RankPlan *plan;
     uint32_t grid_gx, grid_near, grid_far;
     uint32_t far_items;                     // far work counted as (row band, XCD group) items (MFMA)
+    uint32_t far_qw;                        // MFMA far update: quarters per B fragment read (1, or 2 for L <= 4)
     uint64_t *trash;                        // TRASH_WORDS words that lanes with no element load and store
 };
 

@@ -46,13 +46,21 @@ bool parse_tuning(const std::string &path, std::vector<TuneEntry> &out, std::str
         uint64_t side = 0;
         if (!(ss >> kind)) continue;  // blank or comment
         TuneEntry e{};
-        uint32_t v[7];
+        uint32_t v[9] = {};
         bool ok = (kind == "rank" || kind == "bm") && (ss >> mode >> side >> count) && (count == "1" || count == "many");
         for (int i = 0; ok && i < 7; i++) ok = static_cast<bool>(ss >> v[i]);
+        // two optional trailing fields (far_depth far_quarters), both or neither
         std::string extra;
-        if (ok && (ss >> extra)) ok = false;
+        if (ok && (ss >> extra)) {
+            try {
+                v[7] = (uint32_t)std::stoul(extra);
+            } catch (...) {
+                ok = false;
+            }
+            ok = ok && static_cast<bool>(ss >> v[8]) && !(ss >> extra);
+        }
         if (!ok || side == 0 || (side & (side - 1)) || (v[3] & 1) || v[4] > 64 || v[5] > 1024 ||
-            !bm_threads_ok(v[6])) {
+            !bm_threads_ok(v[6]) || v[7] > 2 || v[8] > 2) {
             err = path + ":" + std::to_string(lineno) + ": malformed tuning entry";
             return false;
         }
@@ -67,6 +75,8 @@ bool parse_tuning(const std::string &path, std::vector<TuneEntry> &out, std::str
         e.t.panel = v[4];
         e.t.bm_coop_wgs = v[5];
         e.t.bm_threads = v[6];
+        e.t.far_depth = v[7];
+        e.t.far_quarters = v[8];
         table.push_back(e);
     }
     out.swap(table);
@@ -105,7 +115,7 @@ modlin_tuning_t select_tuning(const std::vector<TuneEntry> &table, const modlin_
     if (override_) {
         const uint32_t *o = &override_->far_per_cu;
         uint32_t *d = &t.far_per_cu;
-        for (int i = 0; i < 7; i++)
+        for (int i = 0; i < 9; i++)
             if (o[i]) d[i] = o[i];
     }
     return t;

@@ -67,11 +67,12 @@ class _Opts(ctypes.Structure):
                 ("coop_min", ctypes.c_uint32)]
 
 
-TUNING_FIELDS = ("far_per_cu", "near_per_cu", "gather_wgs", "check_every", "panel", "bm_coop_wgs", "bm_threads")
+TUNING_FIELDS = ("far_per_cu", "near_per_cu", "gather_wgs", "check_every", "panel", "bm_coop_wgs", "bm_threads",
+                 "far_depth", "far_quarters")
 
 
 class _Tuning(ctypes.Structure):
-    _fields_ = [(name, ctypes.c_uint32) for name in TUNING_FIELDS] + [("reserved", ctypes.c_uint32 * 1)]
+    _fields_ = [(name, ctypes.c_uint32) for name in TUNING_FIELDS] + [("reserved", ctypes.c_uint32 * 3)]
 
 
 class _Stats(ctypes.Structure):

@@ -20,13 +20,23 @@ def build() -> None:
     subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
 
 
-def load() -> ctypes.CDLL:
+def build_native(out_dir: str) -> str:
+    """The same library built for the host it runs on (-march=native), into
+    out_dir: the CPU baseline's build, compiled where it is timed."""
+    out = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
+                    "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"], check=True)
+    return out
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """The default library (built here for x86-64-v3), or the one at path."""
     global _lib
-    if _lib is not None:
+    if path is None and _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    if path is None and not os.path.exists(LIB_PATH):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path or LIB_PATH)
     u64, sz, dbl = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double
     P = ctypes.POINTER(ctypes.c_uint64)
     PD = ctypes.POINTER(ctypes.c_double)
@@ -65,7 +75,8 @@ def load() -> ctypes.CDLL:
     lib.or_mix2.restype = lib.or_mix3.restype = u64
     lib.or_num_threads.restype = ctypes.c_int
     lib.or_set_num_threads.argtypes = [ctypes.c_int]
-    _lib = lib
+    if path is None:
+        _lib = lib
     return lib
 
 
@@ -94,11 +105,11 @@ def inv(p: int, a: int) -> int:
     return load().or_inv(p, a)
 
 
-def rank(p: int, m: np.ndarray, n: int, with_trace: bool = False):
+def rank(p: int, m: np.ndarray, n: int, with_trace: bool = False, lib: ctypes.CDLL | None = None):
     a = np.array(m, dtype=np.uint64, copy=True).reshape(-1)
     pc = np.zeros(max(1, n), dtype=np.uint64)
     w = ctypes.c_double()
-    r = load().or_rank(p, _p(a), n, _p(pc), ctypes.byref(w))
+    r = (lib or load()).or_rank(p, _p(a), n, _p(pc), ctypes.byref(w))
     if with_trace:
         return r, [int(x) for x in pc[:r]], w.value
     return r
@@ -114,10 +125,10 @@ def rank_gf2(m: np.ndarray, n: int) -> int:
     return load().or_rank_gf2(_p(a), n)
 
 
-def lc(p: int, s: np.ndarray, with_stats: bool = False):
+def lc(p: int, s: np.ndarray, with_stats: bool = False, lib: ctypes.CDLL | None = None):
     a = np.ascontiguousarray(s, dtype=np.uint64)
     st = (ctypes.c_double * 3)()
-    r = load().or_lc(p, _p(a), a.size, st)
+    r = (lib or load()).or_lc(p, _p(a), a.size, st)
     if r < 0:
         raise ValueError("empty sequence")
     if with_stats:

@@ -318,3 +318,26 @@ def test_pivot_free_panels(ctx, p, method):
     print(f"pivot-free panels p={p} method={method}", flush=True)
     assert _rank_all(ctx, p, mats, method) == want
     assert _rank_all(ctx, p, mats, method, panel=17) == want
+
+
+@pytest.mark.parametrize("tuning", [dict(far_depth=1), dict(far_depth=2, far_quarters=1), dict(far_quarters=1),
+                                    dict(far_depth=1, far_quarters=1, panel=40)])
+@pytest.mark.parametrize("p", [M61, (1 << 31) - 1, 65521])
+def test_far_update_forms(ctx, p, tuning):
+    """The far update's K depth (one or two panels per pass) and limb-class
+    grouping (quarters per B fragment, for 4 digits or fewer) give the oracle's
+    ranks: mixed sizes, planted and low-rank matrices, hidden pivots."""
+    rng = np.random.default_rng(p % 7919)
+    mats = [rng.integers(0, p, size=(n, n), dtype=np.uint64) for n in (200, 333, 600)]
+    mats += [workloads.c3(700, p)]
+    r = 100
+    mats.append(modlin.combine_rows(p, rng.integers(0, p, size=(r, 500), dtype=np.uint64),
+                                    rng.integers(0, p, size=(500, r), dtype=np.uint64)))
+    mats += _hidden_pivot_cases(p, rng)[10:]
+    want = [oracle.rank(p, a, a.shape[0]) for a in mats]
+    print("far form", p, tuning, flush=True)
+    ctx.set_tuning(**tuning)
+    try:
+        assert _rank_all(ctx, p, mats, modlin.METHOD_MFMA) == want
+    finally:
+        ctx.set_tuning()
