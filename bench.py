@@ -149,6 +149,9 @@ def cpu_baseline(mats, w_target=2e10, w_cap=6e9, full=False):
 
     def leg(lib, idx, threads, over):
         sub = [mats[i] for i in idx]
+        if full:
+            print(f"cpu baseline: {len(sub)} matrices, {threads} threads{' over matrices' if over else ''}",
+                  file=sys.stderr, flush=True)
         dt = _rank_run(O, lib, sub, threads, over)
         cols, w = sum(m.shape[0] for m in sub), float(sum(work[i] for i in idx))
         return {"value": cols / dt, "unit": "columns/s", "cores": threads, "seconds": dt, "matrices": len(idx),

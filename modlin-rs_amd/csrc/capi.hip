@@ -70,12 +70,10 @@ int modlin_ctx_create(int device, modlin_ctx_t **out) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipEventCreate(&ctx->evA) != hipSuccess || hipEventCreate(&ctx->evB) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_ge, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_fin, hipEventDisableTiming) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->aux2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->far_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_plan, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_gx, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_gather, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_far, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc(&ctx->na_host, mdl::NA_HOST_BYTES, hipHostMallocDefault) != hipSuccess ||
         mdl::noalloc_init(ctx) != MODLIN_OK) {
         (void)hipGetLastError();
@@ -118,8 +116,7 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (ctx->aux) hipStreamSynchronize(ctx->aux);
-    if (ctx->aux2) hipStreamSynchronize(ctx->aux2);
+    if (ctx->far_stream) hipStreamSynchronize(ctx->far_stream);
     if (ctx->arena) hipFree(ctx->arena);
     mdl::release_pinned(ctx);
     mdl::release_generators(ctx);
@@ -130,12 +127,10 @@ void modlin_ctx_destroy(modlin_ctx_t *ctx) {
     if (ctx->evA) hipEventDestroy(ctx->evA);
     if (ctx->evB) hipEventDestroy(ctx->evB);
     if (ctx->rank_graph) hipGraphExecDestroy(ctx->rank_graph);
-    if (ctx->ev_ge) hipEventDestroy(ctx->ev_ge);
-    if (ctx->ev_fin) hipEventDestroy(ctx->ev_fin);
     if (ctx->ev_plan) hipEventDestroy(ctx->ev_plan);
-    if (ctx->ev_gx) hipEventDestroy(ctx->ev_gx);
-    if (ctx->aux) hipStreamDestroy(ctx->aux);
-    if (ctx->aux2) hipStreamDestroy(ctx->aux2);
+    if (ctx->ev_gather) hipEventDestroy(ctx->ev_gather);
+    if (ctx->ev_far) hipEventDestroy(ctx->ev_far);
+    if (ctx->far_stream) hipStreamDestroy(ctx->far_stream);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -332,31 +332,29 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     double upd_ms = 0, upd_launches = 0;
     uint32_t h_active = active;
     uint64_t iters = 0;
-    hipStream_t aux = ctx->aux;
-    // One block of check_every iterations: block elimination and inverse on the
-    // aux stream once the previous finish has run; plan, gather, near update and
-    // finish on the main stream; the far columns' X product (MFMA method) on the
-    // second aux stream from the plan on, beside the gather and near update; the
-    // far update once both are done, overlapping the next block elimination.
-    // (check_every is even, so iteration parities repeat per block.)
-    hipStream_t aux2 = ctx->aux2;
+    // One block of check_every iterations.  The panel chain runs on the main
+    // stream: block elimination and inverse, plan, gather, near update, finish,
+    // then the next block elimination.  The far part runs on the far stream: the
+    // far columns' X product (MFMA method) from the plan on, beside the gather and
+    // near update; the far update once the near update is done (its digit rows
+    // are written, and the two do not share the GPU), overlapping the finish and
+    // the next block elimination; the next plan (which commits
+    // the row order the far update reads through, and rewrites the work queue
+    // it takes items from) waits for it.  (check_every is even, so iteration
+    // parities repeat per block.)
+    hipStream_t fs = ctx->far_stream;
     auto enqueue_block = [&](bool prof) -> hipError_t {
         for (int s = 0; s < check_every; s++) {
             const uint32_t par = (uint32_t)(s & 1);
             hipError_t e;
-            if ((e = hipStreamWaitEvent(aux, ctx->ev_fin, 0)) != hipSuccess) return e;
-            launch_block(aux, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
-            if ((e = hipEventRecord(ctx->ev_ge, aux)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(st, ctx->ev_ge, 0)) != hipSuccess) return e;
+            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
+            if ((e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);
-            if (use_mfma) {
-                // far X: reads the plan, Z's digits (this parity) and the pivot rows,
-                // none of which the gather, near update or finish below writes
-                if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
-                if ((e = hipStreamWaitEvent(aux2, ctx->ev_plan, 0)) != hipSuccess) return e;
-                launch_mfma(aux2, dmats, nmat, f, lp, rl, par, -2);
-                if ((e = hipEventRecord(ctx->ev_gx, aux2)) != hipSuccess) return e;
-            }
+            if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(fs, ctx->ev_plan, 0)) != hipSuccess) return e;
+            // far X: reads the plan, Z's digits (this parity) and the pivot rows, none
+            // of which the gather, near update or finish writes
+            if (use_mfma) launch_mfma(fs, dmats, nmat, f, lp, rl, par, -2);
             if (use_mfma)
                 launch_mfma(st, dmats, nmat, f, lp, rl, par, -1);
             else
@@ -367,17 +365,21 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             else
                 launch_update_plain(st, dmats, nmat, f, rl, par, 0);
             if (prof) hipEventRecord(evs[4 * s + 1], st);
+            if ((e = hipEventRecord(ctx->ev_gather, st)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(fs, ctx->ev_gather, 0)) != hipSuccess) return e;
             launch_finish(st, dmats, nmat, dactive, par);
-            if ((e = hipEventRecord(ctx->ev_fin, st)) != hipSuccess) return e;
-            if (use_mfma && (e = hipStreamWaitEvent(st, ctx->ev_gx, 0)) != hipSuccess) return e;
-            if (prof) hipEventRecord(evs[4 * s + 2], st);
+            // far update: columns the finish and the next block elimination do not touch
+            if (prof) hipEventRecord(evs[4 * s + 2], fs);
             if (use_mfma)
-                launch_mfma(st, dmats, nmat, f, lp, rl, par, 1);
+                launch_mfma(fs, dmats, nmat, f, lp, rl, par, 1);
             else
-                launch_update_plain(st, dmats, nmat, f, rl, par, 1);
-            if (prof) hipEventRecord(evs[4 * s + 3], st);
+                launch_update_plain(fs, dmats, nmat, f, rl, par, 1);
+            if (prof) hipEventRecord(evs[4 * s + 3], fs);
+            if ((e = hipEventRecord(ctx->ev_far, fs)) != hipSuccess) return e;
         }
-        return hipGetLastError();
+        // the far stream joins the main stream (the block ends with nothing pending)
+        hipError_t e = hipStreamWaitEvent(st, ctx->ev_far, 0);
+        return e != hipSuccess ? e : hipGetLastError();
     };
     // hipGraph of the block (not while profiling: those events are timed per launch).
     bool use_graph = !profile && !(opts && (opts->flags & MODLIN_FLAG_NO_GRAPH));
@@ -395,7 +397,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
-                hipError_t e1 = hipEventRecord(ctx->ev_fin, st);
+                hipError_t e1 = hipEventRecord(ctx->ev_far, st);
                 if (e1 == hipSuccess) e1 = enqueue_block(false);
                 e = hipStreamEndCapture(st, &graph);
                 if (e == hipSuccess) e = e1;
@@ -431,7 +433,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     uint64_t *dout = (uint64_t *)(base + off_out);
     std::vector<uint64_t> hout(1 + 3 * count);
     for (uint64_t i = 0; i < count; i++) hout[1 + 3 * i] = hm[i].rank;  // a batch with nothing active
-    MDL_HIP(hipEventRecord(ctx->ev_fin, st));  // set-up done
+    MDL_HIP(hipEventRecord(ctx->ev_far, st));  // set-up done
     while (h_active > 0) {
         for (int k = 0; k < ahead; k++) {
             if (use_graph)
@@ -467,8 +469,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         }
     }
     if (progress) progress(user, total_cols, total_cols);
-    MDL_HIP(hipStreamSynchronize(aux));
-    MDL_HIP(hipStreamSynchronize(aux2));
+    MDL_HIP(hipStreamSynchronize(fs));
     for (auto &e : evs) hipEventDestroy(e);
     double macs = 0, ubytes = 0;
     for (uint64_t i = 0; i < count; i++) {

@@ -22,10 +22,10 @@ struct modlin_ctx {
     void *arena = nullptr;
     size_t arena_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evA = nullptr, evB = nullptr;
-    // second stream for the look-ahead panel elimination, third for the far
-    // columns' X product, and their handshake events
-    hipStream_t aux = nullptr, aux2 = nullptr;
-    hipEvent_t ev_ge = nullptr, ev_fin = nullptr, ev_plan = nullptr, ev_gx = nullptr;
+    // second stream of the rank loop (the far part of each update) and its
+    // handshake events
+    hipStream_t far_stream = nullptr;
+    hipEvent_t ev_plan = nullptr, ev_gather = nullptr, ev_far = nullptr;
     // captured 8-iteration launch block of the rank loop, reused while its
     // arguments (arena, batch shape, field, method) are unchanged
     hipGraphExec_t rank_graph = nullptr;
