@@ -483,12 +483,14 @@ def main():
         "mixmax256_rank_65536": lambda: mixmax256_rank(ctx),
     }
     assert tuple(extras) == EXTRA_LINES and tuple(full) == FULL_LINES
-    for name, fn in extras.items():
-        if (only and name in only) or (not only and not args.no_extras):
+    if out:
+        print(f"bench: c2 done, {out['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    for name, fn in list(extras.items()) + list(full.items()):
+        want = not args.no_extras if name in extras else args.full
+        if (only and name in only) or (not only and want):
+            t0 = time.perf_counter()
             out[name] = fn()
-    for name, fn in full.items():
-        if (only and name in only) or (not only and args.full):
-            out[name] = fn()
+            print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(ctx)
     # release the context (graphs, streams, events, pinned words, arena) while the
