@@ -69,7 +69,9 @@ struct RankMat {
     uint32_t done;
     uint32_t iters;
     uint32_t pend;           // missed pivot: position whose row moves to `rank`, applied by
-                             // the next k_plan (0xffffffff: none); see k_finish
+                             // the next k_plan (0xffffffff: none); see finish_iter
+    uint32_t fin;            // iterations whose finish (finish_iter) has run: the last one's
+                             // is pending while fin < iters
     uint32_t dpend;          // a DEFER iteration's far contribution is pending
     uint32_t dpb, dcol, dc0; // its pivots, first column not updated, and base column c0
     uint32_t dJ[BMAX];       // its pivot columns
@@ -89,15 +91,17 @@ struct RankPlan {
 
 // Record of one block elimination, written by k_block_ge and replayed by
 // k_block_z (u64 offsets into RankMat::Eta): the multiplier e of every block row
-// at every step, the pivot rows' entries U[r][s] in the pivot columns (upper
-// triangle), and per step the pivot, the pivot row's own-row coefficient and the
-// swapped position.
-constexpr uint32_t ETA_E = 0;                          // [step][row]
-constexpr uint32_t ETA_U = BMAX * BMAX;                // [pivot row][pivot slot]
+// at every step (zero for rows that take no step: the pivot rows and rows with a
+// zero entry), the pivot rows' entries U[r][s] in the pivot columns (upper
+// triangle, pivot slots), and per step the pivot, the pivot row's own-row
+// coefficient and the pivot row's position in the block (rows do not move).
+// Every value is canonical (in [0, p)).
+constexpr uint32_t ETA_E = 0;                          // [step][block row]
+constexpr uint32_t ETA_U = BMAX * BMAX;                // [pivot slot][pivot slot]
 constexpr uint32_t ETA_PI = 2 * BMAX * BMAX;           // [step]
 constexpr uint32_t ETA_OWNK = ETA_PI + BMAX;           // [step]
-constexpr uint32_t ETA_SWQ = ETA_OWNK + BMAX;          // [step]
-constexpr uint32_t ETA_WORDS = ETA_SWQ + BMAX;
+constexpr uint32_t ETA_Q = ETA_OWNK + BMAX;            // [step]
+constexpr uint32_t ETA_WORDS = ETA_Q + BMAX;
 constexpr uint32_t ZG = 16;                            // workgroups per matrix in k_block_z
 constexpr uint32_t ZSLOTS = BMAX / ZG;                 // slot waves per workgroup (plus one inverse wave)
 

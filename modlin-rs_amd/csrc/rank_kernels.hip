@@ -3,22 +3,24 @@
 //
 // One iteration handles one column panel [col, jend) of every active matrix:
 //
-//   k_block_ge   one workgroup per matrix.  Loads the top block (the next BMAX
-//                logical rows) of the panel into LDS and eliminates it with the
-//                reference's first-nonzero pivot rule (fp.rs:150-163): a wave
-//                ballot + ffs finds the pivot row, rows are swapped in LDS, and the
-//                elimination is fraction-free (pi*x - e*y), so no modular inverse
-//                sits on the per-column critical path.  The elimination is
-//                Gauss-Jordan and tracks the row operations E, so that at the end
-//                E A = D (diagonal) for A = M[R, J], and Z = A^-1 = D^-1 E needs
-//                only pb independent inverses (Fermat, fp.rs:111-113).
+//   k_block_ge   one workgroup per matrix.  Runs the previous iteration's finish
+//                (advance the panel, handle a pivot the top block missed), loads
+//                the top block (the next BMAX logical rows) of the panel into LDS
+//                and eliminates it in place: a wave ballot + ffs finds the pivot
+//                row (the first block row not yet a pivot with a nonzero in the
+//                column), and the elimination is fraction-free (pi*x - e*y), so no
+//                modular inverse sits on the per-column critical path.  It records
+//                the row operations for k_block_z.
+//   k_block_z    Z = A^-1 for A = M[R, J] from that record; pb independent
+//                inverses (Fermat, fp.rs:111-113) off the critical chain.
 //   k_plan       prefix sums of the per-matrix tile counts (the work queue).
 //   k_gather_x   Ab = M[O, J] (gathered multipliers' source) and
 //                X = Z M[R, c0:] (the reference's U-block solve, fp.rs:180-200).
 //   k_update     S = M[O, c0:] - Ab X mod p, the trailing update (fp.rs:202-248),
 //                with lazy 128-bit accumulation and one Barrett reduction per
 //                safe_batch products, exactly as the reference accumulates.
-//   k_finish     advances the panel; handles a pivot the top block missed.
+//   k_finish     the last iteration's finish, before the host reads the active
+//                count (the others run at the start of the next k_block_ge).
 //
 // Exactness.  For any pivot rows R and columns J with A = M[R, J] invertible,
 // rank(M) = |R| + rank(M[O,:] - M[O,J] A^-1 M[R,:]) (Schur complement).  The top
@@ -39,21 +41,84 @@ __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Panel steps keep 2^61 - 1 residues lazily in [0, p + 7] (one fold per step, no
+// final compare); other moduli stay canonical.  Records are canonical.
+template <int MODE>
+__device__ __forceinline__ bool nz_t(uint64_t v) {
+    return MODE == MODE_M61 ? (v != 0 && v != P61) : v != 0;
+}
+template <int MODE>
+__device__ __forceinline__ uint64_t canon_t(uint64_t v) {
+    return MODE == MODE_M61 ? (v >= P61 ? v - P61 : v) : v;
+}
+// pi x - e y for canonical pi, e (x, y lazy for 2^61 - 1)
+template <int MODE>
+__device__ __forceinline__ uint64_t ff_lazy_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {
+    if constexpr (MODE == MODE_M61)
+        return ff_step_m61_lazy(pi, x, e, y);
+    else
+        return ff_step_t<MODE>(f, pi, x, e, y);
+}
+
 // ---------------------------------------------------------------------------
-// k_block_ge: pivot search and fraction-free LU of the panel part of the top block
+// finish_iter: the end of an iteration (parity par) of matrix S, once its near
+// update has run -- advance the panel, bring a missed pivot to the top of the
+// block, retire a finished matrix.  It runs once per iteration: at the start of
+// the next k_block_ge, or in k_finish before the host reads the active count
+// (S.fin < S.iters: the last iteration's finish is pending).  Every thread of the
+// workgroup calls it; it ends with a barrier.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32_t par, uint32_t &best) {
+    const uint32_t tid = threadIdx.x;
+    if (S.done || S.fin == S.iters) return;  // uniform: nothing written yet
+    const IterState &I = S.it[par];
+    const uint32_t n = S.n;
+    const uint32_t jstar = S.first_nz_col;
+    const bool miss = I.resid && jstar < I.scan_end;
+    if (miss) {
+        if (tid == 0) best = 0xffffffffu;
+        __syncthreads();
+        uint32_t mine = 0xffffffffu;
+        for (uint32_t q = S.rank + tid; q < n; q += blockDim.x) {
+            if (m_get(S, (size_t)S.perm[q] * S.ld + jstar) != 0) { mine = q; break; }
+        }
+        if (mine != 0xffffffffu) atomicMin(&best, mine);
+        __syncthreads();
+        // The far part of this iteration's update still reads perm (its digit rows
+        // follow the current order), so the swap to position rank is only recorded
+        // here; the next k_block_ge reads through it and the next k_plan commits it.
+        if (tid == 0 && best != 0xffffffffu && best != S.rank) S.pend = best;
+    }
+    __syncthreads();  // every thread has read the state
+    if (tid == 0) {
+        S.col = miss ? jstar : I.scan_end;  // (scan_end = n: no nonzero left, the rank is final)
+        S.fin = S.iters;
+        if (S.col >= n || S.rank >= n) {
+            S.done = 1;
+            atomicSub(active, 1u);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// k_block_ge: pivot search and fraction-free elimination of the panel part of
+// the top block
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
-                                                   uint32_t defer) {
+                                                   uint32_t defer, uint32_t *active) {
     RankMat &S = mats[blockIdx.x];
-    if (S.done) return;
     // One LDS row per block row: [0, BMAX) panel entries T, then own (the
     // coefficient on the row's own original row).
     constexpr uint32_t COWN = BMAX, UW = BMAX + 1;
     __shared__ uint64_t U[BMAX][UW];
     __shared__ uint64_t Et[BMAX][BMAX];     // multiplier e of each block row at each step
     __shared__ uint64_t piK[BMAX], ownK[BMAX];
-    __shared__ uint32_t rowid[BMAX], Jl[BMAX], swq[BMAX];
+    __shared__ uint32_t rowid[BMAX], Jl[BMAX], Qk[BMAX];
+    __shared__ uint32_t best;
+    finish_iter(S, active, par ^ 1u, best);  // the previous iteration's
+    if (S.done) return;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
@@ -61,9 +126,8 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     const uint32_t jend = min(col + b, n);
     const uint32_t w = jend - col;
     const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
-    const uint64_t p = f.p;
 
-    // Rows of the top block; a missed pivot found by the last k_finish moves to
+    // Rows of the top block; a missed pivot found by the last finish moves to
     // position rank0 (the swap itself is committed to perm by k_plan).
     const uint32_t pend = S.pend;
     if (tid < Bt) {
@@ -85,102 +149,89 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         }
 #pragma unroll
         for (int m = 0; m < 4; m++) U[q][c0 + 16 * m] = v[m];
-        if (c0 == 0) U[q][COWN] = q < Bt ? 1 % p : 0;
+        if (c0 == 0) U[q][COWN] = q < Bt ? 1 % f.p : 0;
     }
     __syncthreads();
 
-    // Wave v owns the panel columns v, v + 16, v + 32, v + 48 (and, at step j, the
-    // own column if v = j mod 16); lane = block row.  A step reads column j (for the
-    // ballot, and as this lane's multiplier e), and speculatively its own columns of
-    // rows `lane` and k, all at once; the pivot comes by readlane.
+    // Rows stay where they are (lane = block row).  The pivot of column j is the
+    // first block row that is not yet a pivot and has a nonzero in column j: any
+    // pivot choice gives the rank (the reference takes the first nonzero row at or
+    // below the current position and swaps it up, fp.rs:150-163).  Wave v owns the
+    // panel columns v, v + 16, v + 32, v + 48 (and, at step j, the own column if
+    // v = j mod 16).  A step reads column j (for the ballot, and as this lane's
+    // multiplier e) and the lane's own columns, all before the ballot; the pivot
+    // row's entries come from its lane by readlane.
     const uint32_t lane = tid & 63, wave = uni32(tid >> 6);
+    const bool inblk = lane < Bt;
+    uint64_t piv = 0;  // block rows that are pivots (the same in every lane)
     uint32_t k = 0;
     for (uint32_t j = 0; j < w && k < Bt; j++) {
-        uint64_t colv = U[lane][j];
+        const uint64_t colv = U[lane][j];
         const bool own = wave == (j & 15);
-        uint64_t xv[5], yv[4];
+        uint64_t xv[5];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const uint32_t c = wave + 16 * m;
-            if (c > j && c < w) {  // uniform over the wave
-                xv[m] = U[lane][c];
-                yv[m] = U[k][c];
-            }
+            if (c > j && c < w) xv[m] = U[lane][c];  // uniform over the wave
         }
         if (own) xv[4] = U[lane][COWN];
-        // Pivot search: first block position q >= k with a nonzero in column j
-        // (rows at or beyond Bt are zero).  Every wave evaluates the same ballot.
-        const uint64_t mask = __ballot(lane >= k && colv != 0);
+        const bool free_row = inblk && !((piv >> lane) & 1);
+        const uint64_t mask = __ballot(free_row && nz_t<MODE>(colv));
         if (mask == 0) continue;  // uniform: no pivot in this column
         const uint32_t q = __ffsll((unsigned long long)mask) - 1;
-        if (q != k) {  // swap block positions k and q (fp.rs:158-162), then read again
-            __syncthreads();  // every wave has read its values before rows move
-            for (uint32_t c = tid; c < UW; c += blockDim.x) {
-                uint64_t t = U[k][c]; U[k][c] = U[q][c]; U[q][c] = t;
-            }
-            if (tid == 0) {
-                uint32_t r = rowid[k]; rowid[k] = rowid[q]; rowid[q] = r;
-            }
-            __syncthreads();
-            colv = U[lane][j];
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const uint32_t c = wave + 16 * m;
-                if (c > j && c < w) {
-                    xv[m] = U[lane][c];
-                    yv[m] = U[k][c];
-                }
-            }
-            if (own) xv[4] = U[lane][COWN];
-        }
-        // Fraction-free elimination below the pivot (fp.rs:166-174 without the
-        // division): x <- pi x - e y on the panel columns after j for every row
-        // under k with e = U[row][j] != 0; own scales by pi.  Row k is only read
-        // and column j is never written, so no barrier separates reads and writes.
-        const uint64_t pi = lane_bcast(colv, k);
-        const uint64_t e = colv;
-        const bool act = lane > k && lane < Bt;
+        // Fraction-free step on the other free rows with e != 0 (fp.rs:166-174
+        // without the division): x <- pi x - e y on the panel columns after j, own
+        // scaled by pi.  Row q and column j are only read, so no barrier separates
+        // the reads from the writes.
+        const uint64_t pi = canon_t<MODE>(lane_bcast(colv, q));
+        const uint64_t e = canon_t<MODE>(colv);
+        const bool act = free_row && lane != q && e != 0;
         if (wave == 0 && act) Et[k][lane] = e;
-        if (act && e != 0) {
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const uint32_t c = wave + 16 * m;
-                if (c > j && c < w) U[lane][c] = ff_step_t<MODE>(f, pi, xv[m], e, yv[m]);
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = wave + 16 * m;
+            if (c > j && c < w) {
+                const uint64_t y = lane_bcast(xv[m], q);
+                if (act) U[lane][c] = ff_lazy_t<MODE>(f, pi, xv[m], e, y);
             }
-            if (own) U[lane][COWN] = ff_step_t<MODE>(f, pi, xv[4], e, 0);
         }
+        if (own && act) U[lane][COWN] = ff_lazy_t<MODE>(f, pi, xv[4], e, 0);
         if (tid == 0) {
             Jl[k] = j;
             piK[k] = pi;
-            ownK[k] = U[k][COWN];
-            swq[k] = q;
+            ownK[k] = canon_t<MODE>(U[q][COWN]);  // row q takes no step now
+            Qk[k] = q;
         }
         __syncthreads();
+        piv |= 1ull << q;
         k++;
     }
     const uint32_t pb = k;
 
     // The record for k_block_z, which builds A^-1 from it.  A pivot row is final
-    // once it is the pivot, so U[r][s] = U[r][J_s] for s > r.
+    // once it is the pivot, so U[r][s] = U[Q[r]][J_s] for s > r.
     uint64_t *eta = S.Eta;
     for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) {
         const uint32_t r = idx / BMAX, s = idx % BMAX;
         eta[ETA_E + idx] = Et[r][s];
-        eta[ETA_U + idx] = (s > r && s < pb) ? U[r][Jl[s]] : 0;
+        eta[ETA_U + idx] = (s > r && s < pb) ? canon_t<MODE>(U[Qk[r]][Jl[s]]) : 0;
     }
     if (tid < pb) {  // (the pivots' inverses are computed by k_block_z, off this kernel's path)
         eta[ETA_PI + tid] = piK[tid];
         eta[ETA_OWNK + tid] = ownK[tid];
-        eta[ETA_SWQ + tid] = swq[tid];
+        eta[ETA_Q + tid] = Qk[tid];
     }
-    // The new order of the block is applied to perm by k_plan, once the previous
+    // The new order of the block -- the pivot rows in pivot order, then the other
+    // rows in block order -- is applied to perm by k_plan, once the previous
     // iteration's update (which reads perm) has finished.
     IterState &I = S.it[par];
-    if (tid < Bt) I.top[tid] = rowid[tid];
     if (tid < pb) {
+        const uint32_t r = rowid[Qk[tid]];
         I.J[tid] = col + Jl[tid];
-        I.R[tid] = rowid[tid];
+        I.R[tid] = r;
+        I.top[tid] = r;
     }
+    if (tid < Bt && !((piv >> tid) & 1)) I.top[pb + __popcll(~piv & ((1ull << tid) - 1))] = rowid[tid];
     // Iteration kind (MFMA method): a pending far contribution makes this a PAIR
     // iteration; otherwise a clean panel (pivots, no missed-pivot check) with far
     // columns beyond the next two panels and rows left below DEFERs its own.
@@ -253,16 +304,17 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
 // k_block_z: Z = A^-1 for the pivot block A = M[R, J] of the last elimination.
 //
 // The elimination's row operations, applied to the identity, give F with
-// F A = U (upper triangular, diagonal = the pivots).  In pivot-slot coordinates
-// F is lower triangular: F[r][r] = own[r] and F[r][s] (s < r) starts at step s as
-// -e own[s] and is then carried through the later steps (x <- pi x - e y, with
-// the recorded swaps).  Then Z = U^-1 F by back substitution.  Every slot s
-// (a column of F and of Z) is independent: one wave per slot, lane = block row,
-// with the step's pivot row broadcast from its lane -- no barriers.  The chain
-// is latency-bound, so the per-step scalars (pivot, swap, own coefficient) sit
-// in the lane registers of their step and are read with readlane.  One more
-// wave per workgroup inverts the pivots (Fermat, in lockstep) while the slot
-// waves run their forward chains; the back substitution waits for it.
+// F A = U (upper triangular in pivot-slot coordinates, diagonal = the pivots).
+// F's column for slot s (the pivot row q_s's coefficient in every block row)
+// starts at step s as own[s] in row q_s and is then carried through the later
+// steps (x <- pi x - e y, y the step's pivot row).  Then Z = U^-1 F.  Every slot
+// (a column of F and of Z) is independent: one wave per slot, lane = block row
+// for the forward chain, with the step's pivot row broadcast from its lane -- no
+// barriers.  One more wave per workgroup inverts the pivots (Fermat, in
+// lockstep) while the slot waves run their forward chains.  The back
+// substitution then runs on V = D^-1 U (unit diagonal) and g = D^-1 f, so that a
+// step of its dependent chain is one product added to lazy sums:
+// z_t = g_t - sum_{u > t} V[t][u] z_u.
 // ---------------------------------------------------------------------------
 
 // Balanced signed digits of Z[k][s] for the matrix-core X product (L > 0):
@@ -290,9 +342,10 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
         return;
     }
     __shared__ uint64_t Ek[BMAX][BMAX];  // multipliers of steps s0.. (rows: step)
-    __shared__ uint64_t Ur[BMAX][BMAX + 1];  // U[r][t], t > r (padded: the back substitution reads columns)
+    __shared__ uint64_t Ur[BMAX][BMAX + 1];  // U[r][t], t > r, then V (padded: the back substitution reads columns)
     __shared__ uint64_t piK[BMAX], dinv[BMAX], ownK[BMAX];
-    __shared__ uint32_t swq[BMAX];
+    __shared__ uint64_t Fcol[ZSLOTS][BMAX];  // F's column of each slot wave, by block row
+    __shared__ uint32_t qk[BMAX];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const uint64_t *eta = S.Eta;
     const uint64_t p = f.p;
@@ -326,51 +379,60 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     if (tid < BMAX) {
         piK[tid] = tid < pb ? eta[ETA_PI + tid] : 1;
         ownK[tid] = tid < pb ? eta[ETA_OWNK + tid] : 0;
-        swq[tid] = tid < pb ? (uint32_t)eta[ETA_SWQ + tid] : tid;
+        qk[tid] = tid < pb ? (uint32_t)eta[ETA_Q + tid] : 0;
     }
     __syncthreads();
     const uint32_t s = s0 + wave;
-    uint64_t fv = 0;
     if (wave == ZSLOTS) {
         // the inverse wave
         dinv[lane] = lane < pb ? inv_fermat_t<MODE>(f, piK[lane]) : 0;
     } else if (s < pb) {
-        // Forward: ev = F[lane][s].
-        const uint32_t my_q = swq[lane];
-        const uint64_t my_pi = piK[lane], my_own = ownK[lane];
-        uint64_t ev = 0;
+        // Forward: ev = F[lane][q_s] for block row `lane`.  The step's pivot row,
+        // pivot and multiplier come from lane registers (readlane) and one LDS read
+        // issued a step ahead.
+        const uint32_t my_q = qk[lane];
+        const uint64_t my_pi = piK[lane];
+        const uint32_t qs = (uint32_t)__builtin_amdgcn_readlane((int)my_q, (int)s);
+        uint64_t ev = lane == qs ? ownK[s] : 0;
         uint64_t e_next = Ek[s][lane];
         for (uint32_t k = s; k < pb; k++) {
             const uint64_t e = e_next;
             if (k + 1 < pb) e_next = Ek[k + 1][lane];
             const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)my_q, (int)k);
-            if (q != k) {
-                const uint64_t vk = lane_bcast(ev, k), vq = lane_bcast(ev, q);
-                ev = (lane == k) ? vq : (lane == q) ? vk : ev;
-            }
-            const uint64_t yk = (k == s) ? lane_bcast(my_own, k) : lane_bcast(ev, k);
-            const uint64_t x = (k == s) ? 0 : ev;
+            const uint64_t y = lane_bcast(ev, q);
             const uint64_t pik = lane_bcast(my_pi, k);
-            if (lane > k && e != 0) ev = ff_step_t<MODE>(f, pik, x, e, yk);
+            if (e != 0) ev = ff_lazy_t<MODE>(f, pik, ev, e, y);
         }
-        fv = lane > s ? ev : (lane == s ? my_own : 0);
+        Fcol[wave][lane] = canon_t<MODE>(ev);
     }
-    __syncthreads();  // the inverses are in LDS
+    __syncthreads();  // the inverses and F's columns are in LDS
+    // V = D^-1 U (the rows of U scaled by the inverse pivots), in place
+    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) {
+        const uint32_t r = idx / BMAX, t = idx % BMAX;
+        if (t > r && t < pb) Ur[r][t] = mul_mod_t<MODE>(f, dinv[r], Ur[r][t]);
+    }
+    __syncthreads();
     if (wave == ZSLOTS) return;
     if (s >= pb) {
         if (L) z_digits(S.Zl + (size_t)par * BMAX * L * KSTEP, L, lane, s, 0, p);
         return;
     }
 
-    // Back substitution, column-oriented: once z[t] is known every lane l < t adds
-    // U[l][t] z[t] to its sum; lane t-1 is then complete.
-    const uint64_t my_dinv = dinv[lane];
+    // Back substitution, column-oriented, lane = slot t: once z_t is known every
+    // lane u < t adds V[u][t] z_t to its lazy sum; lane t-1 is then complete.
+    const uint64_t fv = (lane >= s && lane < pb) ? Fcol[wave][qk[lane]] : 0;
+    const uint64_t g = lane < pb ? mul_mod_t<MODE>(f, dinv[lane], fv) : 0;
     uint64_t acc = 0, z = 0;
     for (int t = (int)pb - 1; t >= 0; t--) {
-        const uint64_t u = Ur[lane][t];
-        if (lane == (uint32_t)t) z = mul_mod_t<MODE>(f, my_dinv, sub_mod(fv, acc, p));
+        const uint64_t v = Ur[lane][t];
+        if (lane == (uint32_t)t) z = sub_mod(g, canon_t<MODE>(acc), p);
         const uint64_t zt = lane_bcast(z, (uint32_t)t);
-        if (lane < (uint32_t)t) acc = add_mod(acc, mul_mod_t<MODE>(f, u, zt), p);
+        if (lane < (uint32_t)t) {
+            if constexpr (MODE == MODE_M61)
+                acc = fold_m61_64(acc + fold_m61_64(mul_m61_lazy(v, (uint32_t)zt & 0x7fffffffu, (uint32_t)(zt >> 31))));
+            else
+                acc = add_mod(acc, mul_mod_t<MODE>(f, v, zt), p);
+        }
     }
     if (lane < pb) S.Z[lane * BMAX + s] = z;
     if (L) z_digits(S.Zl + (size_t)par * BMAX * L * KSTEP, L, lane, s, lane < pb ? z : 0, p);
@@ -392,7 +454,7 @@ __device__ __forceinline__ void plan_counts(RankMat &S, uint32_t par, uint32_t f
     if (O == 0 || cols == 0) return;
     const bool pair = I.kind == IT_PAIR;
     // gather rows (own and, for a PAIR, the pending panel's columns) and X columns;
-    // the MFMA method's far X columns go to the launch after k_finish
+    // the MFMA method's far X columns go to a launch of their own beside the gather
     if (I.pb > 0) {
         const uint32_t nx = cdiv(cols, XCOLS), nxn = far_items ? x_near_tasks(I) : nx;
         g = cdiv(O, GATHER_ROWS) + nxn;
@@ -646,38 +708,12 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
 }
 
 // ---------------------------------------------------------------------------
-// k_finish: advance the panel; bring a missed pivot to the top of the block
+// k_finish: the pending finish of the last iteration (parity par), before the
+// host reads the active count; the other iterations' run inside k_block_ge
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active, uint32_t par) {
-    RankMat &S = mats[blockIdx.x];
-    if (S.done) return;
     __shared__ uint32_t best;
-    const IterState &I = S.it[par];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = S.n;
-    uint32_t jstar = S.first_nz_col;
-    const bool miss = I.resid && jstar < I.scan_end;
-    if (miss) {
-        if (tid == 0) best = 0xffffffffu;
-        __syncthreads();
-        uint32_t mine = 0xffffffffu;
-        for (uint32_t q = S.rank + tid; q < n; q += blockDim.x) {
-            if (m_get(S, (size_t)S.perm[q] * S.ld + jstar) != 0) { mine = q; break; }
-        }
-        if (mine != 0xffffffffu) atomicMin(&best, mine);
-        __syncthreads();
-        // The far part of this iteration's update still reads perm (its digit rows
-        // follow the current order), so the swap to position rank is only recorded
-        // here; the next k_block_ge reads through it and the next k_plan commits it.
-        if (tid == 0 && best != 0xffffffffu && best != S.rank) S.pend = best;
-    }
-    if (tid == 0) {
-        S.col = miss ? jstar : I.scan_end;  // (scan_end = n: no nonzero left, the rank is final)
-        if (S.col >= n || S.rank >= n) {
-            S.done = 1;
-            atomicSub(active, 1u);
-        }
-    }
+    finish_iter(mats[blockIdx.x], active, par, best);
 }
 
 // ---------------------------------------------------------------------------
@@ -699,15 +735,15 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 // Host launchers
 // ---------------------------------------------------------------------------
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer) {
+                  uint32_t L, uint32_t defer, uint32_t *active) {
     if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
         hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
         hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
         hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     }
 }
