@@ -19,7 +19,7 @@ namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer, uint32_t *active);
+                  uint32_t L, uint32_t defer);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
 void launch_collect(hipStream_t st, RankMat *dmats, uint32_t nmat, const uint32_t *active, uint64_t *out);
@@ -333,12 +333,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     uint32_t h_active = active;
     uint64_t iters = 0;
     // One block of check_every iterations.  The panel chain runs on the main
-    // stream: block elimination (which first finishes the iteration before) and
-    // inverse, plan, gather, near update, then the next block elimination.  The
-    // far part runs on the far stream: the far columns' X product (MFMA method)
-    // from the plan on, beside the gather and near update; the far update once
-    // the near update is done (its digit rows are written, and the two do not
-    // share the GPU), overlapping the next block elimination; the next plan (which commits
+    // stream: block elimination and inverse, plan, gather, near update, finish,
+    // then the next block elimination.  The far part runs on the far stream: the
+    // far columns' X product (MFMA method) from the plan on, beside the gather and
+    // near update; the far update once the near update is done (its digit rows
+    // are written, and the two do not share the GPU), overlapping the finish and
+    // the next block elimination; the next plan (which commits
     // the row order the far update reads through, and rewrites the work queue
     // it takes items from) waits for it.  (check_every is even, so iteration
     // parities repeat per block.)
@@ -347,7 +347,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         for (int s = 0; s < check_every; s++) {
             const uint32_t par = (uint32_t)(s & 1);
             hipError_t e;
-            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u, dactive);
+            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
             if ((e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
             launch_plan(st, dmats, nmat, rl, par);
             if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
@@ -367,7 +367,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             if (prof) hipEventRecord(evs[4 * s + 1], st);
             if ((e = hipEventRecord(ctx->ev_gather, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(fs, ctx->ev_gather, 0)) != hipSuccess) return e;
-            // far update: columns the next block elimination does not touch
+            launch_finish(st, dmats, nmat, dactive, par);
+            // far update: columns the finish and the next block elimination do not touch
             if (prof) hipEventRecord(evs[4 * s + 2], fs);
             if (use_mfma)
                 launch_mfma(fs, dmats, nmat, f, lp, rl, par, 1);
@@ -442,8 +443,6 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             iters += check_every;
         }
         ahead = 1;
-        // the last iteration's finish, then the active count and the ranks
-        launch_finish(st, dmats, nmat, dactive, (uint32_t)((check_every - 1) & 1));
         launch_collect(st, dmats, nmat, dactive, dout);
         MDL_HIP(hipGetLastError());
         rc = copy_d2h(ctx, hout.data(), dout, hout.size() * sizeof(uint64_t));

@@ -3,9 +3,7 @@
 //
 // One iteration handles one column panel [col, jend) of every active matrix:
 //
-//   k_block_ge   one workgroup per matrix.  Runs the previous iteration's finish
-//                (advance the panel, handle a pivot the top block missed), loads
-//                the top block (the next BMAX logical rows) of the panel into LDS
+//   k_block_ge   one workgroup per matrix.  Loads the top block (the next BMAX logical rows) of the panel into LDS
 //                and eliminates it in place: a wave ballot + ffs finds the pivot
 //                row (the first block row not yet a pivot with a nonzero in the
 //                column), and the elimination is fraction-free (pi*x - e*y), so no
@@ -19,8 +17,7 @@
 //   k_update     S = M[O, c0:] - Ab X mod p, the trailing update (fp.rs:202-248),
 //                with lazy 128-bit accumulation and one Barrett reduction per
 //                safe_batch products, exactly as the reference accumulates.
-//   k_finish     the last iteration's finish, before the host reads the active
-//                count (the others run at the start of the next k_block_ge).
+//   k_finish     advances the panel; handles a pivot the top block missed.
 //
 // Exactness.  For any pivot rows R and columns J with A = M[R, J] invertible,
 // rank(M) = |R| + rank(M[O,:] - M[O,J] A^-1 M[R,:]) (Schur complement).  The top
@@ -63,10 +60,9 @@ __device__ __forceinline__ uint64_t ff_lazy_t(const FieldParams &f, uint64_t pi,
 // ---------------------------------------------------------------------------
 // finish_iter: the end of an iteration (parity par) of matrix S, once its near
 // update has run -- advance the panel, bring a missed pivot to the top of the
-// block, retire a finished matrix.  It runs once per iteration: at the start of
-// the next k_block_ge, or in k_finish before the host reads the active count
-// (S.fin < S.iters: the last iteration's finish is pending).  Every thread of the
-// workgroup calls it; it ends with a barrier.
+// block, retire a finished matrix.  It runs once per iteration (S.fin < S.iters:
+// the last iteration's finish is pending).  Every thread of the workgroup calls
+// it; it ends with a barrier.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32_t par, uint32_t &best) {
     const uint32_t tid = threadIdx.x;
@@ -107,7 +103,7 @@ __device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
-                                                   uint32_t defer, uint32_t *active) {
+                                                   uint32_t defer) {
     RankMat &S = mats[blockIdx.x];
     // One LDS row per block row: [0, BMAX) panel entries T, then own (the
     // coefficient on the row's own original row).
@@ -116,8 +112,6 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     __shared__ uint64_t Et[BMAX][BMAX];     // multiplier e of each block row at each step
     __shared__ uint64_t piK[BMAX], ownK[BMAX];
     __shared__ uint32_t rowid[BMAX], Jl[BMAX], Qk[BMAX];
-    __shared__ uint32_t best;
-    finish_iter(S, active, par ^ 1u, best);  // the previous iteration's
     if (S.done) return;
 
     const uint32_t tid = threadIdx.x;
@@ -127,7 +121,7 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
     const uint32_t w = jend - col;
     const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
 
-    // Rows of the top block; a missed pivot found by the last finish moves to
+    // Rows of the top block; a missed pivot found by the last k_finish moves to
     // position rank0 (the swap itself is committed to perm by k_plan).
     const uint32_t pend = S.pend;
     if (tid < Bt) {
@@ -708,8 +702,7 @@ __global__ __launch_bounds__(256) void k_update_valu(RankMat *mats, uint32_t nma
 }
 
 // ---------------------------------------------------------------------------
-// k_finish: the pending finish of the last iteration (parity par), before the
-// host reads the active count; the other iterations' run inside k_block_ge
+// k_finish: the end of the iteration (parity par), after its near update
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active, uint32_t par) {
     __shared__ uint32_t best;
@@ -735,15 +728,15 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 // Host launchers
 // ---------------------------------------------------------------------------
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer, uint32_t *active) {
+                  uint32_t L, uint32_t defer) {
     if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer, active);
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     }
 }
