@@ -9,9 +9,11 @@ each step copies them into the library's workspace (the reference destroys its
 input too) and runs the whole batch through one launch sequence.
 
 Prints one JSON line.  value = total matrix columns ranked per second
-(sum of sides / step time).  `roofline` refers to the trailing-update kernel,
-timed with HIP events on the library's stream; `traffic` is its HBM bytes per
-launch from the committed rocprofv3 counter summary; `cpu_baseline` is the
+(sum of sides / step time).  `roofline` refers to the dominant kernel, the
+far-column trailing update k_update_far, timed with HIP events on the stream
+it is launched on; `traffic` is its HBM bytes per launch from the committed
+rocprofv3 counter summary; `near_far_update` gives the same figures for the
+near and far update launches together; `cpu_baseline` is the
 oracle's OpenMP port of fp.rs on a bounded sample of the same batch on this
 host.  Extra keys report the plain (VALU) path on the same batch, the single
 8192 x 8192 planted-corank matrix over 2^61-1 on both paths, the batched
@@ -210,15 +212,16 @@ def cpu_bm(seqs, long_seq, budget_s=4.0, prefixes=(20000, 40000)):
                         "projected_full_s": proj, "projected_us_per_step": proj * 1e6 / n}}
 
 
-def traffic_per_launch():
+def traffic_per_launch(key="far_traffic_bytes_per_launch"):
     """HBM bytes per launch of the MFMA trailing update on this workload, from
     the committed counter summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
-    same counters on a copy kernel of known size), or None."""
+    same counters on a copy kernel of known size), or None.  `key` selects the
+    far-column update alone or the mean over near and far launches."""
     path = os.path.join(ROOT, "profiles", "update_kernels_batched_rank_64_p61_counters.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -254,6 +257,9 @@ def single_large(ctx, n, p, steps, plain=True, warmup=1):
         out[f"{name}_update_ms"] = st["update_ms"]
         if name == "mfma" and st["update_ms"]:
             out["mfma_update_frac"] = 2.0 * L * L * st["macs"] / (st["update_ms"] / 1e3) / 1e12 / PEAK_INT8_TOPS
+            if st["far_ms"]:
+                out["mfma_far_update_frac"] = (2.0 * L * L * st["far_macs"] / (st["far_ms"] / 1e3) / 1e12
+                                               / PEAK_INT8_TOPS)
     if plain:
         out["speedup_mfma_vs_plain"] = out["mfma_columns_per_s"] / out["plain_columns_per_s"]
     out["corank_ok"] = all(out[f"{name}_corank"] == 64 for name, _ in methods)
@@ -518,11 +524,19 @@ def c2_line(ctx, mats, method, args):
 
     L = limb_count(M61) if st["limb_count"] else 0
     upd_s = st["update_ms"] / 1e3
+    near_far = None
     if L:
-        ops = 2.0 * L * L * st["macs"]
-        achieved = ops / upd_s / 1e12
+        # int8 operations per F_p multiply-accumulate: 2 per product of L x L digit pairs
+        far_ops = 2.0 * L * L * st["far_macs"]
+        achieved = far_ops / (st["far_ms"] / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                "frac": achieved / PEAK_INT8_TOPS, "traffic": traffic_per_launch()}
+                "frac": achieved / PEAK_INT8_TOPS, "traffic": traffic_per_launch(),
+                "kernel": "k_update_far", "launches_per_step": st["far_launches"],
+                "avg_us": st["far_ms"] * 1e3 / max(1, st["far_launches"])}
+        all_ops = 2.0 * L * L * st["macs"]
+        near_far = {"kernels": "k_update_mfma + k_update_far", "achieved": all_ops / upd_s / 1e12,
+                    "unit": "TOP/s", "frac": all_ops / upd_s / 1e12 / PEAK_INT8_TOPS,
+                    "traffic": traffic_per_launch("traffic_bytes_per_launch")}
     else:
         # VALU update: algorithmic HBM bytes = 16 per updated element (read + write)
         # + 8 per operand element of each tile (counted by the kernel)
@@ -547,6 +561,7 @@ def c2_line(ctx, mats, method, args):
                                "p=2^61-1, half xoshiro256** uniform, half MIXMAX-17",
                    "matrices": len(mats), "sum_sides": cols, "max_side": max(sides), "method": args.method},
         "roofline": roof,
+        "near_far_update": near_far,
         "update_ms_per_step": st["update_ms"],
         "update_launches_per_step": st["update_launches"],
         "update_avg_us": st["update_ms"] * 1e3 / max(1, st["update_launches"]),

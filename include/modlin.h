@@ -121,6 +121,9 @@ typedef struct modlin_stats {
                                element (read + write) + 8 per operand element per tile */
     double graph;           /* 1 if the rank loop replayed a captured hipGraph */
     double bm_coop;         /* workgroups of the cooperative BM kernel (0: one per sequence) */
+    double far_macs;        /* the part of macs in the far-column updates */
+    double far_ms;          /* summed duration of the far-column update kernels (PROFILE) */
+    double far_launches;    /* number of far-column update launches (PROFILE) */
 } modlin_stats_t;
 
 /* Launch parameters of the rank and BM engines.  A zero field means "from the

@@ -207,8 +207,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur = align_up(cur + sizeof(RankPlan), A);
     const size_t off_trash = cur;
     cur = align_up(cur + TRASH_WORDS * sizeof(uint64_t), A);
-    const size_t off_out = cur;  // k_collect's output: active, then rank, macs, bytes per matrix
-    cur = align_up(cur + (1 + 3 * count) * sizeof(uint64_t), A);
+    const size_t off_out = cur;  // k_collect's output: active, then rank, macs, bytes, far macs per matrix
+    cur = align_up(cur + (1 + 4 * count) * sizeof(uint64_t), A);
     for (uint64_t i = 0; i < count; i++) {
         const uint64_t n = mats[i].n;
         if (n > 0xffffffffull / 2 || (n && n > (((uint64_t)1) << 62) / n / 8)) {
@@ -329,7 +329,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         evs.resize(4 * check_every);
         for (auto &e : evs) MDL_HIP(hipEventCreate(&e));
     }
-    double upd_ms = 0, upd_launches = 0;
+    double upd_ms = 0, upd_launches = 0, far_ms = 0, far_launches = 0;
     uint32_t h_active = active;
     uint64_t iters = 0;
     // One block of check_every iterations.  The panel chain runs on the main
@@ -431,8 +431,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     if (!progress && !profile)
         ahead = (int)std::max<uint64_t>(1, ((max_n + b - 1) / b + check_every - 1) / check_every);
     uint64_t *dout = (uint64_t *)(base + off_out);
-    std::vector<uint64_t> hout(1 + 3 * count);
-    for (uint64_t i = 0; i < count; i++) hout[1 + 3 * i] = hm[i].rank;  // a batch with nothing active
+    std::vector<uint64_t> hout(1 + 4 * count);
+    for (uint64_t i = 0; i < count; i++) hout[1 + 4 * i] = hm[i].rank;  // a batch with nothing active
     MDL_HIP(hipEventRecord(ctx->ev_far, st));  // set-up done
     while (h_active > 0) {
         for (int k = 0; k < ahead; k++) {
@@ -455,6 +455,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
                 MDL_HIP(hipEventElapsedTime(&ms1, evs[4 * s + 2], evs[4 * s + 3]));
                 upd_ms += ms0 + ms1;
                 upd_launches += 2;
+                far_ms += ms1;
+                far_launches += 1;
             }
         }
         if (progress && h_active > 0) {
@@ -471,13 +473,17 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     if (progress) progress(user, total_cols, total_cols);
     MDL_HIP(hipStreamSynchronize(fs));
     for (auto &e : evs) hipEventDestroy(e);
-    double macs = 0, ubytes = 0;
+    double macs = 0, ubytes = 0, fmacs = 0;
     for (uint64_t i = 0; i < count; i++) {
-        ranks[i] = hout[1 + 3 * i];
-        macs += (double)hout[2 + 3 * i];
-        ubytes += (double)hout[3 + 3 * i];
+        ranks[i] = hout[1 + 4 * i];
+        macs += (double)hout[2 + 4 * i];
+        ubytes += (double)hout[3 + 4 * i];
+        fmacs += (double)hout[4 + 4 * i];
     }
     ctx->stats.macs = macs;
+    ctx->stats.far_macs = fmacs;
+    ctx->stats.far_ms = far_ms;
+    ctx->stats.far_launches = far_launches;
     ctx->stats.update_bytes = ubytes;
     ctx->stats.update_ms = upd_ms;
     ctx->stats.update_launches = upd_launches;

@@ -227,27 +227,75 @@ MDL_HD uint64_t limb_residue_k2(const FieldParams &f, const LimbParams &lp, cons
     return reduce128(f, th, tl);
 }
 
-// (cv - sum_s 256^s P_s) mod 2^61 - 1 for cv in [0, p), from the 15 class
-// accumulators of two accumulated panels of L = 8 digits (every class started
-// at KOFF2); kc2 = k2 + 2p.  Class s has weight 2^(8 s mod 61): classes 0-4 and
-// 8-12 (exponents up to 35) stay below 2^61, classes 5-7 and 13-14 are rotated
-// (the bits from 61 up come back at bit 0).  The 15 terms add up below 2^64.
-MDL_HD uint64_t limb_sub_m61_k2(uint64_t cv, uint64_t kc2, const int32_t *acc) {
-    uint64_t x = 0;
-#pragma unroll
-    for (int c = 0; c < 15; c++) {
-        const uint64_t v = (uint32_t)acc[c];
-        const int e = (8 * c) % 61;
-        if (e + 26 <= 61)
-            x += v << e;
-        else
-            x += ((v << e) & P61) + (v >> (61 - e));
+// Recombination of the 15 class sums P_s of 2^61 - 1 (L = 8), each in [0, 2^26)
+// (two accumulated panels, every class started at KOFF2), into a 64-bit value
+// congruent to S' = sum_s 2^(8 s mod 61) P_s.  Classes c and c + 8 (c < 7) have
+// the weights 2^(8c) and 2^(8c + 3) (2^64 == 2^3), so t_c = P_c + 8 P_{c+8} <
+// 2^29.2 carries the weight 2^(8c); class 7 (2^56) stands alone.  The parts:
+//   lo = 2^8 t_1 + 2^16 t_2 + 2^24 t_3                                 < 2^53.2
+//   hi = t_4 + 2^8 (t_5 mod 2^21) + 2^16 (t_6 mod 2^13) + 2^24 (P_7 mod 2^5)
+//                                                      < 2^31.1, weight 2^32
+//   w  = t_0 + (t_5 >> 21) + (t_6 >> 13) + (P_7 >> 5)                  < 2^30
+// (t 2^(8c) for c = 5, 6 and P_7 2^56 are split at bit 61, whose weight is 1),
+// and x = lo + 2^32 hi + w < 2^64.  Mostly 32-bit work: about half the
+// instructions of rotating every class into a 64-bit sum.  The pairs can be
+// added in any order (the update kernels add each as soon as both classes are
+// complete).
+struct M61Sum {
+    uint64_t lo;
+    uint32_t hi, w;
+};
+
+MDL_HD M61Sum m61_sum_init() { return M61Sum{0, 0, 0}; }
+
+// Adds classes C and C + 8 (C < 7).
+template <int C>
+MDL_HD void m61_sum_pair(M61Sum &a, uint32_t pc, uint32_t pc8) {
+    static_assert(C >= 0 && C < 7, "class pairs (0, 8) .. (6, 14)");
+    const uint32_t t = pc + (pc8 << 3);
+    if constexpr (C == 0) {
+        a.w += t;
+    } else if constexpr (C <= 3) {
+        a.lo += (uint64_t)t << (8 * C);
+    } else if constexpr (C == 4) {
+        a.hi += t;
+    } else {
+        constexpr int S = 61 - 8 * C;  // bits of t below 2^61: 21 or 13
+        a.hi += (t & ((1u << S) - 1u)) << (8 * C - 32);
+        a.w += t >> S;
     }
+}
+
+// Adds class 7 (weight 2^56).
+MDL_HD void m61_sum_class7(M61Sum &a, uint32_t p7) {
+    a.hi += (p7 & 31u) << 24;
+    a.w += p7 >> 5;
+}
+
+// (cv - (S' - k2)) mod 2^61 - 1, canonical, for cv in [0, p); kc2 = k2 + 2p.
+MDL_HD uint64_t m61_sum_sub(uint64_t cv, uint64_t kc2, const M61Sum &a) {
+    const uint64_t x = a.lo + (((uint64_t)a.hi << 32) | a.w);
     const uint64_t y = (x & P61) + (x >> 61);  // <= p + 7, congruent to S'
     // r = cv + k2 + 2p - y in [p - 7, 4p), congruent to cv - (S' - k2)
     uint64_t r = cv + kc2 - y;
     r = (r & P61) + (r >> 61);  // <= p + 3
     return r >= P61 ? r - P61 : r;
+}
+
+// (cv - sum_s 256^s P_s) mod 2^61 - 1 for cv in [0, p), from the 15 class
+// accumulators of two accumulated panels of L = 8 digits (every class started
+// at KOFF2, so each lies in [0, 2^26)); kc2 = k2 + 2p.
+MDL_HD uint64_t limb_sub_m61_k2(uint64_t cv, uint64_t kc2, const int32_t *acc) {
+    M61Sum a = m61_sum_init();
+    m61_sum_pair<0>(a, (uint32_t)acc[0], (uint32_t)acc[8]);
+    m61_sum_pair<1>(a, (uint32_t)acc[1], (uint32_t)acc[9]);
+    m61_sum_pair<2>(a, (uint32_t)acc[2], (uint32_t)acc[10]);
+    m61_sum_pair<3>(a, (uint32_t)acc[3], (uint32_t)acc[11]);
+    m61_sum_pair<4>(a, (uint32_t)acc[4], (uint32_t)acc[12]);
+    m61_sum_pair<5>(a, (uint32_t)acc[5], (uint32_t)acc[13]);
+    m61_sum_pair<6>(a, (uint32_t)acc[6], (uint32_t)acc[14]);
+    m61_sum_class7(a, (uint32_t)acc[7]);
+    return m61_sum_sub(cv, kc2, a);
 }
 
 }  // namespace mdl

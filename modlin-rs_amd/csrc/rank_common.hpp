@@ -80,6 +80,7 @@ struct RankMat {
     // update kernels).
     alignas(128) unsigned long long macs;  // F_p multiply-accumulates of the trailing updates
     unsigned long long upd_bytes;          // algorithmic HBM bytes of the trailing updates
+    unsigned long long far_macs;           // the part of macs in the far parts
 };
 
 // Work plan of one iteration: per-matrix tile offsets (prefix sums) of the

@@ -471,6 +471,7 @@ __device__ __forceinline__ void plan_counts(RankMat &S, uint32_t par, uint32_t f
         // column tiles).
         const unsigned long long R = cdiv(O, UT), C = cdiv(wn, UT) + cdiv(wf, UT);
         S.macs += (unsigned long long)O * wn * I.pb + (unsigned long long)O * wf * kfar;
+        S.far_macs += (unsigned long long)O * wf * kfar;
         S.upd_bytes += 16ull * O * (wn + wf) + 8ull * kfar * (O * C + (unsigned long long)(wn + wf) * R);
     }
 }
@@ -711,16 +712,17 @@ __global__ __launch_bounds__(1024) void k_finish(RankMat *mats, uint32_t *active
 
 // ---------------------------------------------------------------------------
 // k_collect: the active count and every matrix's rank and counters, packed for
-// one copy to the host: out[0] = active, out[1 + 3 i ..] = rank, macs, bytes
+// one copy to the host: out[0] = active, out[1 + 4 i ..] = rank, macs, bytes, far macs
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t nmat, const uint32_t *active,
                                                  uint64_t *out) {
     const uint32_t tid = threadIdx.x;
     if (tid == 0) out[0] = *active;
     for (uint32_t i = tid; i < nmat; i += blockDim.x) {
-        out[1 + 3 * (size_t)i] = mats[i].rank;
-        out[2 + 3 * (size_t)i] = mats[i].macs;
-        out[3 + 3 * (size_t)i] = mats[i].upd_bytes;
+        out[1 + 4 * (size_t)i] = mats[i].rank;
+        out[2 + 4 * (size_t)i] = mats[i].macs;
+        out[3 + 4 * (size_t)i] = mats[i].upd_bytes;
+        out[4 + 4 * (size_t)i] = mats[i].far_macs;
     }
 }
 

@@ -78,7 +78,7 @@ class _Tuning(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_double) for name in
                 ("macs", "update_ms", "update_launches", "limb_count", "iterations", "bm_bytes", "bm_ms",
-                 "total_ms", "update_bytes", "graph", "bm_coop")]
+                 "total_ms", "update_bytes", "graph", "bm_coop", "far_macs", "far_ms", "far_launches")]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)

@@ -147,6 +147,39 @@ static void run_k2(uint64_t p, int trials, int kind) {
     }
 }
 
+// limb_sub_m61_k2 over its whole input domain: every class value in [0, 2^26)
+// (uniform, all at the maximum 2^26 - 1, all zero, and each class alone at its
+// maximum), against (cv - sum_s 2^(8 s) P_s + k2) mod p in 128-bit integers.
+static void m61_k2_domain(int trials) {
+    const LimbParams lp = make_limb_params(P61);
+    auto check = [&](const int32_t *acc, uint64_t cv) {
+        u128 s = 0;
+        for (int c = 0; c < 15; c++) s = (s + (u128)(uint32_t)acc[c] * lp.pow256[c]) % P61;
+        const uint64_t sv = (uint64_t)((s + P61 - lp.k2) % P61);  // S' - k2
+        const uint64_t exp = cv >= sv ? cv - sv : cv + P61 - sv;
+        const uint64_t got = limb_sub_m61_k2(cv, lp.kc2, acc);
+        if (got != exp) {
+            if (bad < 8) printf("limb_sub_m61_k2 domain mismatch cv=%llu\n", (unsigned long long)cv);
+            bad++;
+        }
+    };
+    int32_t acc[15];
+    for (int tr = 0; tr < trials; tr++) {
+        for (int c = 0; c < 15; c++) acc[c] = (int32_t)(rng() & ((1u << 26) - 1));
+        check(acc, rng() % P61);
+    }
+    for (uint64_t cv : {(uint64_t)0, (uint64_t)1, P61 - 1, lp.k2, (uint64_t)(rng() % P61)}) {
+        for (int c = 0; c < 15; c++) acc[c] = (1 << 26) - 1;
+        check(acc, cv);
+        for (int c = 0; c < 15; c++) acc[c] = 0;
+        check(acc, cv);
+        for (int one = 0; one < 15; one++) {
+            for (int c = 0; c < 15; c++) acc[c] = c == one ? (1 << 26) - 1 : 0;
+            check(acc, cv);
+        }
+    }
+}
+
 template <int L>
 static void generic(uint64_t p, int trials) {
     for (int kind = 0; kind < 5; kind++) {
@@ -163,6 +196,7 @@ int main() {
         run_k2<MODE_M61, 8>(P61, kind ? 50 : 2 * n, kind);
         run_k2<MODE_M31, 4>(P31, kind ? 50 : 2 * n, kind);
     }
+    m61_k2_domain(20 * n);
     const uint64_t primes[] = {2ull, 3ull, 5ull, 7ull, 127ull, 131ull, 251ull, 257ull, 65521ull, 65537ull,
                                16777213ull, 4294967291ull, 1099511627689ull, 281474976710597ull,
                                72057594037927931ull, P31, P61, 4611686018427387847ull, 9223372036854775783ull};
