@@ -53,6 +53,14 @@ __device__ __forceinline__ TileMeta tile_meta(RankMat *mats, uint32_t nmat, cons
     return m;
 }
 
+// Class accumulators of 2^61 - 1 start at KOFF2 (limb_sub_m61_k2: 32-bit class
+// pairs, about half the VALU work of the pair words); the other moduli at
+// HOFF / 0 (limb_residue's pair words, cheaper there for 4 digits or more).
+template <int MODE, int L>
+constexpr bool kind_k2() {
+    return MODE == MODE_M61 && L == 8;
+}
+
 // Residue of the product for element (b, r) of this lane's accumulators.
 template <int MODE, int NC>
 __device__ __forceinline__ uint64_t acc_residue(const FieldParams &f, const LimbParams &lp, v4i (*acc)[2], int b,
@@ -60,7 +68,11 @@ __device__ __forceinline__ uint64_t acc_residue(const FieldParams &f, const Limb
     int32_t P[NC];
 #pragma unroll
     for (int s = 0; s < NC; s++) P[s] = acc[s][b][r];
-    return limb_residue<MODE, NC>(f, lp, P);
+    if constexpr (kind_k2<MODE, (NC + 1) / 2>()) {
+        const uint64_t neg = limb_sub_m61_k2(0, lp.kc2, P);  // (0 - product) mod p
+        return neg == 0 ? 0 : P61 - neg;
+    }
+    return sub_mod(limb_residue<MODE, NC>(f, lp, P), lp.k0, f.p);
 }
 
 // (cv - product) mod p for element (b, r) of this lane's accumulators.
@@ -70,7 +82,7 @@ __device__ __forceinline__ uint64_t acc_sub(const FieldParams &f, const LimbPara
     int32_t P[NC];
 #pragma unroll
     for (int s = 0; s < NC; s++) P[s] = acc[s][b][r];
-    if constexpr (MODE == MODE_M61 && NC == 15) return limb_sub_m61(cv, lp.kc, P);
+    if constexpr (kind_k2<MODE, (NC + 1) / 2>()) return limb_sub_m61_k2(cv, lp.kc2, P);
     return sub_mod(add_mod(cv, lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
 }
 
@@ -96,14 +108,15 @@ __device__ __forceinline__ void digits4(const uint64_t *x, uint64_t p, uint32_t 
 
 // One 64 x 64 x 64 limb product on the matrix cores: acc[s][b] = init_s +
 // sum_{la+lb=s} A_la B_lb for this wave's 16 x 32 block (rows wr.., columns wc..),
-// with init_s = HOFF for the even classes and 0 for the odd ones (limb.hpp): the
-// first product of each class takes its initial value as the MFMA's C operand.
-// As / Bs hold the digit rows (K-contiguous, L planes of 64 bytes, row stride KAP).
-template <int L, uint32_t KAP>
+// with init_s = HOFF for the even classes and 0 for the odd ones (limb.hpp), or
+// KOFF2 for every class (K2): the first product of each class takes its initial
+// value as the MFMA's C operand.  As / Bs hold the digit rows (K-contiguous, L
+// planes of 64 bytes, row stride KAP).
+template <int L, uint32_t KAP, bool K2>
 __device__ __forceinline__ void limb_tile_mma(const int8_t *As, const int8_t *Bs, uint32_t lane, uint32_t wr,
                                               uint32_t wc, v4i (*acc)[2]) {
-    constexpr int H = (int)HOFF;
-    const v4i hoff = {H, H, H, H}, zero = {0, 0, 0, 0};
+    constexpr int H = K2 ? KOFF2 : (int)HOFF;
+    const v4i hoff = {H, H, H, H}, zero = {K2 ? KOFF2 : 0, K2 ? KOFF2 : 0, K2 ? KOFF2 : 0, K2 ? KOFF2 : 0};
     v4i bf[L][2];
     const uint32_t koff = 16 * (lane >> 4);
 #pragma unroll
@@ -212,7 +225,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             }
             __syncthreads();
             v4i acc[NC][2];
-            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+            limb_tile_mma<L, KAP, kind_k2<MODE, L>()>(As, Bs, lane, wr, wc, acc);
             // Y = M[R[s]][c] - product (mod p) for this lane's 4 slots and 2 columns
             uint64_t y[2][4];
 #pragma unroll
@@ -221,7 +234,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const uint32_t sl = wr + 4 * (lane >> 4) + r;
-                    const uint64_t pr = sub_mod(acc_residue<MODE, NC>(f, lp, acc, b, r), lp.k0, p);
+                    const uint64_t pr = acc_residue<MODE, NC>(f, lp, acc, b, r);
                     y[b][r] = (c < cols && sl < pb) ? sub_mod(m_get(S, (size_t)I.R[sl] * ld + I.c0 + cb + c), pr, p) : 0;
                 }
             }
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
         }
         __syncthreads();
         v4i acc[NC][2];
-        limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+        limb_tile_mma<L, KAP, kind_k2<MODE, L>()>(As, Bs, lane, wr, wc, acc);
         // X = sum_s 256^s P_s mod p, then its digits (4 consecutive k per lane and column)
         int8_t *xl = own_xl(S, I);
 #pragma unroll
@@ -268,7 +281,7 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
             const uint32_t c = wc + 16 * b + (lane & 15);
             uint64_t x[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) x[r] = sub_mod(acc_residue<MODE, NC>(f, lp, acc, b, r), lp.k0, p);
+            for (int r = 0; r < 4; r++) x[r] = acc_residue<MODE, NC>(f, lp, acc, b, r);
             if (c < cols) {
                 uint32_t dw[L];
                 digits4<L>(x, p, dw);
@@ -327,13 +340,33 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
             // scan_end; a tile wholly right of a nonzero column already found has
             // nothing to add
             const uint32_t known = __hip_atomic_load(&S.first_nz_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (c0 + cb <= known)
-                for (uint32_t e = tid; e < nr * nc; e += blockDim.x) {
-                    const uint32_t i = e / nc, c = e - i * nc;
-                    const uint32_t gcol = c0 + cb + c;
-                    if (gcol < scan_end && m_get(S, (size_t)S.perm[rank + r0 + i] * ld + gcol) != 0)
-                        nzcol = min(nzcol, gcol);
+            // thread: column tid % 64, rows tid / 64 + 8 k; every load of the tile
+            // is issued before any is tested (a rank-deficient matrix's rest is
+            // scanned whole in one iteration, so this is a streaming read)
+            const uint32_t c = tid & 63, i0 = tid >> 6, gcol = c0 + cb + c;
+            if (c0 + cb <= known && c < nc && gcol < scan_end) {
+                size_t idx[UT / 8];
+#pragma unroll
+                for (uint32_t k = 0; k < UT / 8; k++) {
+                    const uint32_t i = i0 + 8 * k;
+                    idx[k] = (size_t)S.perm[rank + r0 + min(i, nr - 1)] * ld + gcol;
                 }
+                bool any = false;
+                if (S.m32) {
+                    uint32_t v[UT / 8];
+#pragma unroll
+                    for (uint32_t k = 0; k < UT / 8; k++) v[k] = ((const uint32_t *)S.M)[idx[k]];
+#pragma unroll
+                    for (uint32_t k = 0; k < UT / 8; k++) any |= i0 + 8 * k < nr && v[k] != 0;
+                } else {
+                    uint64_t v[UT / 8];
+#pragma unroll
+                    for (uint32_t k = 0; k < UT / 8; k++) v[k] = S.M[idx[k]];
+#pragma unroll
+                    for (uint32_t k = 0; k < UT / 8; k++) any |= i0 + 8 * k < nr && v[k] != 0;
+                }
+                if (any) nzcol = gcol;
+            }
         } else {
             // stage this tile's digits (already in registers) in LDS
             __syncthreads();
@@ -384,7 +417,7 @@ __global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nma
                     cv[b][r] = (rok[r] && c < nc) ? m_get(S, mrow[r] + c) : 0;
                 }
             v4i acc[NC][2];
-            limb_tile_mma<L, KAP>(As, Bs, lane, wr, wc, acc);
+            limb_tile_mma<L, KAP, kind_k2<MODE, L>()>(As, Bs, lane, wr, wc, acc);
             // epilogue: C - sum_s 256^s P_s (mod p)
 #pragma unroll
             for (int b = 0; b < 2; b++)
