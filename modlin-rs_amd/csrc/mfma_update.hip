@@ -182,15 +182,31 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                 const uint32_t *Jc = pass ? I.dJ : I.J;
                 const uint32_t kb = pass ? I.dpb : pb;
                 int8_t *out = pass ? S.AlP : S.Al;
-                for (uint32_t idx = tid; idx < rows * (KSTEP / 4); idx += blockDim.x) {
-                    const uint32_t i = idx / (KSTEP / 4), k4 = (idx - i * (KSTEP / 4)) * 4;
-                    const size_t mrow = (size_t)S.perm[I.rank + r0 + i] * ld;
-                    uint64_t x[4];
+                // GATHER_ROWS x 16 units of 4 entries, two per thread: every load of
+                // both (row order, then entries) is issued before any digit is formed
+                static_assert(GATHER_ROWS * (KSTEP / 4) <= 2 * 512, "two units per thread");
+                uint32_t iu[2], ku[2];
+                bool ok[2];
+                size_t mrow[2];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) x[j] = (k4 + j < kb) ? m_get(S, mrow + Jc[k4 + j]) : 0;
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t idx = tid + 512 * u;
+                    iu[u] = idx / (KSTEP / 4);
+                    ku[u] = (idx - iu[u] * (KSTEP / 4)) * 4;
+                    ok[u] = iu[u] < rows;
+                    mrow[u] = (size_t)S.perm[I.rank + r0 + (ok[u] ? iu[u] : 0)] * ld;
+                }
+                uint64_t x[2][4];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) x[u][j] = (ok[u] && ku[u] + j < kb) ? m_get(S, mrow[u] + Jc[ku[u] + j]) : 0;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    if (!ok[u]) continue;
                     uint32_t w[L];
-                    digits4<L>(x, p, w);
-                    uint32_t *dst = (uint32_t *)(out + (size_t)(r0 + i) * KA + k4);
+                    digits4<L>(x[u], p, w);
+                    uint32_t *dst = (uint32_t *)(out + (size_t)(r0 + iu[u]) * KA + ku[u]);
 #pragma unroll
                     for (int la = 0; la < L; la++) dst[la * (KSTEP / 4)] = w[la];
                 }
