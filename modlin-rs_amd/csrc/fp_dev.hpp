@@ -347,6 +347,38 @@ MDL_HD uint64_t inv_fermat_t(const FieldParams &f, uint64_t a) {
     return r;
 }
 
+// 1 / a mod 2^61 - 1 for a in [1, p): a^(p-2) (Fermat, fp.rs:111-113) by an
+// addition chain instead of square-and-multiply.  With x_k = a^(2^k - 1):
+// x_{j+k} = x_j^(2^k) x_k builds x_2, x_3, x_6, x_12, x_24, x_48, x_54, x_57,
+// x_59, and p - 2 = 2^61 - 3 = 4 (2^59 - 1) + 1 gives the inverse as x_59^4 a:
+// 60 squarings and 10 products (square-and-multiply takes 60 and 59), lazy in
+// [0, 2^61 + 8) until the end.
+MDL_HD uint64_t sq_m61_lazy(uint64_t x) {  // x < 2^61 + 8 -> x^2 in [0, 2^61 + 8)
+    const uint32_t x0 = (uint32_t)x & 0x7fffffffu, x1 = (uint32_t)(x >> 31);
+    const uint64_t s = (uint64_t)x0 * x0 + (uint64_t)x1 * (2u * x1);  // x^2 == s + 2 x0 x1 2^31
+    const uint64_t t = ((uint64_t)x0 * x1) << 1;                         // < 2^62
+    return fold_m61_64(s + (t >> 30) + ((t & 0x3fffffffull) << 31));
+}
+MDL_HD uint64_t sqn_m61_lazy(uint64_t x, int n) {
+    for (int i = 0; i < n; i++) x = sq_m61_lazy(x);
+    return x;
+}
+MDL_HD uint64_t mulf_m61_lazy(uint64_t x, uint64_t y) {
+    return fold_m61_64(mul_m61_lazy(x, (uint32_t)y & 0x7fffffffu, (uint32_t)(y >> 31)));
+}
+MDL_HD uint64_t inv_m61(uint64_t a) {
+    const uint64_t x2 = mulf_m61_lazy(sq_m61_lazy(a), a);
+    const uint64_t x3 = mulf_m61_lazy(sq_m61_lazy(x2), a);
+    const uint64_t x6 = mulf_m61_lazy(sqn_m61_lazy(x3, 3), x3);
+    const uint64_t x12 = mulf_m61_lazy(sqn_m61_lazy(x6, 6), x6);
+    const uint64_t x24 = mulf_m61_lazy(sqn_m61_lazy(x12, 12), x12);
+    const uint64_t x48 = mulf_m61_lazy(sqn_m61_lazy(x24, 24), x24);
+    const uint64_t x54 = mulf_m61_lazy(sqn_m61_lazy(x48, 6), x6);
+    const uint64_t x57 = mulf_m61_lazy(sqn_m61_lazy(x54, 3), x3);
+    const uint64_t x59 = mulf_m61_lazy(sqn_m61_lazy(x57, 2), x2);
+    return canon_m61(mulf_m61_lazy(sqn_m61_lazy(x59, 2), a));
+}
+
 // Inverse of a in [1, p) by the binary extended Euclidean algorithm (Knuth,
 // TAOCP vol. 2, 4.5.2): about 2 bitlen(p) short steps instead of the ~2 bitlen(p)
 // dependent Barrett products of Fermat.  Any exact inverse is bit-identical to

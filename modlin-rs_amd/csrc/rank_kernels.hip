@@ -379,7 +379,10 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
     const uint32_t s = s0 + wave;
     if (wave == ZSLOTS) {
         // the inverse wave
-        dinv[lane] = lane < pb ? inv_fermat_t<MODE>(f, piK[lane]) : 0;
+        if constexpr (MODE == MODE_M61)
+            dinv[lane] = lane < pb ? inv_m61(piK[lane]) : 0;
+        else
+            dinv[lane] = lane < pb ? inv_fermat_t<MODE>(f, piK[lane]) : 0;
     } else if (s < pb) {
         // Forward: ev = F[lane][q_s] for block row `lane`.  The step's pivot row,
         // pivot and multiplier come from lane registers (readlane) and one LDS read

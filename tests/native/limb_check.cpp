@@ -180,6 +180,22 @@ static void m61_k2_domain(int trials) {
     }
 }
 
+// The addition-chain inverse of 2^61 - 1 against the binary extended Euclid,
+// and a * inv(a) == 1, on random and edge values.
+static void m61_inverse(int trials) {
+    auto check = [&](uint64_t a) {
+        const uint64_t r = inv_m61(a);
+        if (r != inv_mod_bin(a, P61) || (uint64_t)(((u128)a * r) % P61) != 1) {
+            if (bad < 8) printf("inv_m61 mismatch a=%llu\n", (unsigned long long)a);
+            bad++;
+        }
+    };
+    const uint64_t edge[] = {1, 2, 3, P61 - 1, P61 - 2, (P61 + 1) / 2, (uint64_t)1 << 60, ((uint64_t)1 << 31) - 1,
+                             (uint64_t)1 << 31, (uint64_t)0x7fffffff << 30};
+    for (uint64_t a : edge) check(a);
+    for (int i = 0; i < trials; i++) check(1 + rng() % (P61 - 1));
+}
+
 template <int L>
 static void generic(uint64_t p, int trials) {
     for (int kind = 0; kind < 5; kind++) {
@@ -197,6 +213,7 @@ int main() {
         run_k2<MODE_M31, 4>(P31, kind ? 50 : 2 * n, kind);
     }
     m61_k2_domain(20 * n);
+    m61_inverse(n);
     const uint64_t primes[] = {2ull, 3ull, 5ull, 7ull, 127ull, 131ull, 251ull, 257ull, 65521ull, 65537ull,
                                16777213ull, 4294967291ull, 1099511627689ull, 281474976710597ull,
                                72057594037927931ull, P31, P61, 4611686018427387847ull, 9223372036854775783ull};
