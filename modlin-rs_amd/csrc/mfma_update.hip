@@ -9,13 +9,17 @@
 //   * A X = sum_s 256^s P_s with P_s = sum_{la+lb=s} A_la X_lb, s < 2L-1: each
 //     digit pair is one int8 GEMM with K = 64 (the panel), accumulated exactly in
 //     int32 per shift class (|P_s| <= 8 * 64 * 2^14 = 2^23);
-//   * the epilogue (limb_residue, limb.hpp) joins neighbouring classes into
-//     exact 32-bit pair words -- the even classes' accumulators start at an
-//     offset that keeps them non-negative -- adds the pair words as two 32-bit
-//     word strings and reduces the sum once (Mersenne folds, or the weights
-//     2^(32 i) mod p and the reference's Barrett reduction, fp.rs:71-79).  Every
-//     result is the canonical residue, so the update equals the VALU path and
-//     the CPU reference bit for bit.
+//   * the epilogue (limb.hpp) forms the residue of sum_s 256^s P_s.  For
+//     2^61 - 1 every class starts at KOFF2 (non-negative 26-bit sums) and
+//     classes c and c + 8, whose weights differ by 2^3, are added as one 32-bit
+//     value before the Mersenne folds (limb_sub_m61_k2).  For the other moduli
+//     one panel's classes are joined into exact 32-bit pair words -- the even
+//     classes' accumulators start at an offset that keeps them non-negative --
+//     added as two 32-bit word strings and reduced once (Mersenne folds for
+//     2^31 - 1, or the weights 2^(32 i) mod p and the reference's Barrett
+//     reduction, fp.rs:71-79); two panels' classes (the far update) go through
+//     limb_residue_k2.  Every result is the canonical residue, so the update
+//     equals the VALU path and the CPU reference bit for bit.
 //
 // k_gather_split  digit planes of A = M[O, J] (rows x L*64, K-contiguous) and of
 //                 X = Z M[R, c0:] (columns x L*64, K-contiguous), fusing the X
