@@ -42,7 +42,6 @@ struct LimbParams {
     uint64_t pow256[16];  // 256^l mod p
     uint64_t pow32[5];    // 2^(32 i) mod p: weights of the recombined 32-bit words
     uint64_t k0;          // sum_{t < L} HOFF 2^(16 t) mod p (the pair words' offsets)
-    uint64_t kc;          // k0 + 2p (limb_sub_m61)
     uint64_t k2;          // sum_{s < 2L-1} KOFF2 256^s mod p (the offsets of limb_residue_k2)
     uint64_t kc2;         // k2 + 2p (limb_sub_m61_k2)
     uint32_t L;           // digits per element
@@ -81,30 +80,12 @@ inline LimbParams make_limb_params(uint64_t p) {
         w = (w * 65536) % p;
     }
     lp.k0 = (uint64_t)k0;
-    lp.kc = (uint64_t)(k0 + 2 * (u128)p);
     u128 k2 = 0;
     for (uint32_t c = 0; c + 1 < 2 * lp.L; c++) k2 = (k2 + (u128)KOFF2 % p * lp.pow256[c]) % p;
     lp.k2 = (uint64_t)k2;
     lp.kc2 = (uint64_t)(k2 + 2 * (u128)p);
     return lp;
 }
-
-// a + b + cin with the carry out (one add-with-carry instruction on the GPU).
-MDL_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout) {
-#if defined(__clang__)
-    unsigned co;
-    const uint32_t r = __builtin_addc(a, b, cin, &co);
-    cout = co;
-    return r;
-#else
-    const uint64_t v = (uint64_t)a + b + cin;
-    cout = (uint32_t)(v >> 32);
-    return (uint32_t)v;
-#endif
-}
-
-// (hi:lo) >> 16, low 32 bits.
-MDL_HD uint32_t funnel16(uint32_t hi, uint32_t lo) { return (lo >> 16) | (hi << 16); }
 
 // S = sum_t h_t 2^(16 t) as five 32-bit words (NH <= 8 pair words).
 template <int NH>
@@ -163,38 +144,6 @@ MDL_HD uint64_t limb_residue(const FieldParams &f, const LimbParams &lp, const i
         add128(th, tl, ph, pl);
     }
     return f.k >= 49 ? reduce_prod(f, th, tl) : reduce128(f, th, tl);
-}
-
-// (cv - sum_s 256^s P_s) mod 2^61 - 1 for cv in [0, p), from the 15 class
-// accumulators of L = 8 digits (HOFF-offset even classes); kc = k0 + 2p.  The
-// same sum as limb_residue in 32-bit add-with-carry steps, with the final
-// reduction shared by the subtraction.
-MDL_HD uint64_t limb_sub_m61(uint64_t cv, uint64_t kc, const int32_t *acc) {
-    uint32_t h[8];
-#pragma unroll
-    for (int t = 0; t < 7; t++) h[t] = (uint32_t)acc[2 * t] + ((uint32_t)acc[2 * t + 1] << 8);
-    h[7] = (uint32_t)acc[14];
-    // S = (h6 h4 h2 h0) + 2^16 (h7 h5 h3 h1) as five words s0..s4 (s4 <= 2^16)
-    uint32_t c;
-    const uint32_t s0 = addc32(h[0], h[1] << 16, 0, c);
-    const uint32_t s1 = addc32(h[2], funnel16(h[3], h[1]), c, c);
-    const uint32_t s2 = addc32(h[4], funnel16(h[5], h[3]), c, c);
-    const uint32_t s3 = addc32(h[6], funnel16(h[7], h[5]), c, c);
-    const uint32_t s4 = (h[7] >> 16) + c;
-    // x = (lo & p) + (lo >> 61) + ((hi << 3) & p) + (hi >> 58) + 2^6 s4 < 2^62 + 2^24,
-    // lo = (s1 s0), hi = (s3 s2): 2^64 == 8 and 2^128 == 2^6
-    const uint32_t small = (s1 >> 29) + (s3 >> 26) + (s4 << 6);
-    uint32_t xl = addc32(s0, s2 << 3, 0, c);
-    uint32_t xh = (s1 & 0x1fffffffu) + (((s3 << 3) | (s2 >> 29)) & 0x1fffffffu) + c;
-    xl = addc32(xl, small, 0, c);
-    xh += c;
-    // y = (x & p) + (x >> 61) <= p + 3
-    const uint32_t yl = addc32(xl, xh >> 29, 0, c);
-    const uint32_t yh = (xh & 0x1fffffffu) + c;
-    // r = cv + k0 + 2p - y in [p - 3, 4p), congruent to cv - (S - k0)
-    uint64_t r = cv + kc - (((uint64_t)yh << 32) | yl);
-    r = (r & P61) + (r >> 61);  // <= p + 3
-    return r >= P61 ? r - P61 : r;
 }
 
 // Residue mod p of S' = sum_s 256^s P'_s from the class accumulators of two
