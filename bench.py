@@ -534,7 +534,7 @@ def c2_line(ctx, mats, method, args):
                 "kernel": "k_update_far", "launches_per_step": st["far_launches"],
                 "avg_us": st["far_ms"] * 1e3 / max(1, st["far_launches"])}
         all_ops = 2.0 * L * L * st["macs"]
-        near_far = {"kernels": "k_update_mfma + k_update_far", "achieved": all_ops / upd_s / 1e12,
+        near_far = {"kernels": "k_update_near + k_update_far", "achieved": all_ops / upd_s / 1e12,
                     "unit": "TOP/s", "frac": all_ops / upd_s / 1e12 / PEAK_INT8_TOPS,
                     "traffic": traffic_per_launch("traffic_bytes_per_launch")}
     else:

@@ -24,10 +24,12 @@
 // k_gather_split  digit planes of A = M[O, J] (rows x L*64, K-contiguous) and of
 //                 X = Z M[R, c0:] (columns x L*64, K-contiguous), fusing the X
 //                 product (the reference's U-block solve) with the digit split.
-// k_update_mfma   one 64 x 64 output tile per workgroup iteration: both digit
-//                 tiles are staged in LDS with 16-byte loads, 8 waves compute
-//                 16 x 32 each with v_mfma_i32_16x16x64_i8 (lane l holds row or
-//                 column l % 16 and K bytes 16 (l / 16) .. +16 of a 64-byte step).
+// k_update_near   the near part: one 64 x 64 output tile per item, on the far
+//                 update's machinery (k_update_far: row digits in registers,
+//                 column digits staged in LDS, v_mfma_i32_16x16x64_i8 with lane l
+//                 holding row or column l % 16 and K bytes 16 (l / 16) .. +16 of
+//                 a 64-byte step).
+// k_update_far    the far part, by row band and XCD group, one or two panels.
 #include "rank_common.hpp"
 
 namespace mdl {
@@ -77,17 +79,6 @@ __device__ __forceinline__ uint64_t acc_residue(const FieldParams &f, const Limb
         return neg == 0 ? 0 : P61 - neg;
     }
     return sub_mod(limb_residue<MODE, NC>(f, lp, P), lp.k0, f.p);
-}
-
-// (cv - product) mod p for element (b, r) of this lane's accumulators.
-template <int MODE, int NC>
-__device__ __forceinline__ uint64_t acc_sub(const FieldParams &f, const LimbParams &lp, uint64_t cv, v4i (*acc)[2],
-                                            int b, int r) {
-    int32_t P[NC];
-#pragma unroll
-    for (int s = 0; s < NC; s++) P[s] = acc[s][b][r];
-    if constexpr (kind_k2<MODE, (NC + 1) / 2>()) return limb_sub_m61_k2(cv, lp.kc2, P);
-    return sub_mod(add_mod(cv, lp.k0, f.p), limb_residue<MODE, NC>(f, lp, P), f.p);
 }
 
 // Balanced signed digits of four consecutive elements, packed per digit plane:
@@ -313,149 +304,6 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
     }
 }
 
-template <int L, int MODE>
-__global__ __launch_bounds__(512) void k_update_mfma(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
-                                                     const RankPlan *plan, FieldParams f, LimbParams lp, uint32_t par,
-                                                     uint32_t far) {
-    constexpr uint32_t KA = L * KSTEP;
-    constexpr uint32_t KAP = KA + 16;    // LDS row stride: consecutive rows start 4 banks apart
-    constexpr int NC = 2 * L - 1;        // shift classes
-    constexpr uint32_t CH = KA / 16;     // 16-byte chunks per digit row
-    constexpr uint32_t NCH = UT * CH;              // chunks per operand tile
-    constexpr uint32_t NV = (NCH + 511) / 512;     // chunks per thread per operand tile
-    __shared__ __attribute__((aligned(16))) int8_t As[UT * KAP];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[UT * KAP];
-    const uint32_t total = far ? plan->far_total : plan->near_total;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
-    const uint32_t wr = (wave >> 1) * 16, wc = (wave & 1) * 32;
-
-    // operand prefetch registers: the next tile's digit chunks
-    v4i ra[NV], rb[NV];
-    uint32_t t = blockIdx.x;
-    TileMeta cur;
-    if (t < total) {
-        cur = tile_meta(mats, nmat, upd_off, t, par, far);
-        if (cur.pb) {
-            const RankMat &S = mats[cur.mi];
-            const v4i *ga = (const v4i *)(S.Al + (size_t)cur.r0 * KA);
-            const v4i *gb = (const v4i *)(own_xl(S, S.it[par]) + (size_t)cur.cb * KA);
-#pragma unroll
-            for (uint32_t j = 0; j < NV; j++) {
-                if (NCH % 512 == 0 || tid + j * 512 < NCH) {
-                    ra[j] = ga[tid + j * 512];
-                    rb[j] = gb[tid + j * 512];
-                }
-            }
-        }
-    }
-    for (; t < total; t += gridDim.x) {
-        RankMat &S = mats[cur.mi];
-        const IterState &I = S.it[par];
-        const uint32_t r0 = cur.r0, cb = cur.cb, nr = cur.nr, nc = cur.nc, pb = cur.pb, ld = S.ld;
-        const uint32_t rank = I.rank, c0 = I.c0;
-        uint32_t nzcol = 0xffffffffu;
-        const uint32_t scan_end = I.scan_end;
-        if (pb == 0) {
-            // detection only (no pivots): scan the tile for a nonzero column below
-            // scan_end; a tile wholly right of a nonzero column already found has
-            // nothing to add
-            const uint32_t known = __hip_atomic_load(&S.first_nz_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // thread: column tid % 64, rows tid / 64 + 8 k; every load of the tile
-            // is issued before any is tested (a rank-deficient matrix's rest is
-            // scanned whole in one iteration, so this is a streaming read)
-            const uint32_t c = tid & 63, i0 = tid >> 6, gcol = c0 + cb + c;
-            if (c0 + cb <= known && c < nc && gcol < scan_end) {
-                size_t idx[UT / 8];
-#pragma unroll
-                for (uint32_t k = 0; k < UT / 8; k++) {
-                    const uint32_t i = i0 + 8 * k;
-                    idx[k] = (size_t)S.perm[rank + r0 + min(i, nr - 1)] * ld + gcol;
-                }
-                bool any = false;
-                if (S.m32) {
-                    uint32_t v[UT / 8];
-#pragma unroll
-                    for (uint32_t k = 0; k < UT / 8; k++) v[k] = ((const uint32_t *)S.M)[idx[k]];
-#pragma unroll
-                    for (uint32_t k = 0; k < UT / 8; k++) any |= i0 + 8 * k < nr && v[k] != 0;
-                } else {
-                    uint64_t v[UT / 8];
-#pragma unroll
-                    for (uint32_t k = 0; k < UT / 8; k++) v[k] = S.M[idx[k]];
-#pragma unroll
-                    for (uint32_t k = 0; k < UT / 8; k++) any |= i0 + 8 * k < nr && v[k] != 0;
-                }
-                if (any) nzcol = gcol;
-            }
-        } else {
-            // stage this tile's digits (already in registers) in LDS
-            __syncthreads();
-#pragma unroll
-            for (uint32_t j = 0; j < NV; j++) {
-                const uint32_t v = tid + j * 512, row = v / CH, ch = v - row * CH;
-                if (NCH % 512 == 0 || v < NCH) {
-                    *(v4i *)(As + row * KAP + ch * 16) = ra[j];
-                    *(v4i *)(Bs + row * KAP + ch * 16) = rb[j];
-                }
-            }
-            __syncthreads();
-        }
-        // prefetch the next tile's digits while this one is computed
-        const uint32_t tn = t + gridDim.x;
-        TileMeta nxt;
-        if (tn < total) {
-            nxt = tile_meta(mats, nmat, upd_off, tn, par, far);
-            if (nxt.pb) {
-                const RankMat &Sn = mats[nxt.mi];
-                const v4i *ga = (const v4i *)(Sn.Al + (size_t)nxt.r0 * KA);
-                const v4i *gb = (const v4i *)(own_xl(Sn, Sn.it[par]) + (size_t)nxt.cb * KA);
-#pragma unroll
-                for (uint32_t j = 0; j < NV; j++) {
-                    if (NCH % 512 == 0 || tid + j * 512 < NCH) {
-                        ra[j] = ga[tid + j * 512];
-                        rb[j] = gb[tid + j * 512];
-                    }
-                }
-            }
-        }
-        if (pb) {
-            // rows of this lane's outputs and their current values (loaded early)
-            size_t mrow[4];
-            bool rok[4];
-            uint64_t cv[2][4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const uint32_t i = wr + 4 * (lane >> 4) + r;
-                rok[r] = i < nr;
-                mrow[r] = rok[r] ? (size_t)S.perm[rank + r0 + i] * ld + c0 + cb : 0;
-            }
-#pragma unroll
-            for (int b = 0; b < 2; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint32_t c = wc + 16 * b + (lane & 15);
-                    cv[b][r] = (rok[r] && c < nc) ? m_get(S, mrow[r] + c) : 0;
-                }
-            v4i acc[NC][2];
-            limb_tile_mma<L, KAP, kind_k2<MODE, L>()>(As, Bs, lane, wr, wc, acc);
-            // epilogue: C - sum_s 256^s P_s (mod p)
-#pragma unroll
-            for (int b = 0; b < 2; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const uint32_t c = wc + 16 * b + (lane & 15);
-                    if (!rok[r] || c >= nc) continue;
-                    const uint64_t res = acc_sub<MODE, NC>(f, lp, cv[b][r], acc, b, r);
-                    m_set(S, mrow[r] + c, res);
-                    const uint32_t gcol = c0 + cb + c;
-                    if (res != 0 && gcol < scan_end) nzcol = min(nzcol, gcol);
-                }
-        }
-        if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
-        cur = nxt;
-    }
-}
-
 // The same limb product for a 16 x 16 block with the roles of the operands
 // swapped in the register file: the wave's L A digit fragments (its 16 rows) are
 // held, and the B fragment of its 16 columns is read per digit plane lb.
@@ -513,13 +361,16 @@ struct FarItem {
     gptr<const uint32_t> permg;
     gptr<const v4i> Ag[2], Xg[2];  // panel h: 0 the pending one, 1 this iteration's own
     uint32_t x, r0, nr, rank, ld, cbase, width, C;
+    uint32_t scan_end;  // NEAR: nonzero results in columns c0 + c < scan_end are reported
 };
 
 // One item with the panels H0 .. H1 (compile-time, so every product's
-// accumulator operand is known) and entries of type T.
-template <int L, int MODE, int QW, int H0, int H1, typename T>
+// accumulator operand is known) and entries of type T.  NEAR: the item is the
+// single column tile x (the near part's 64 x 64 tiles), and the first column
+// with a nonzero result below it.scan_end lowers nzcol (the missed-pivot check).
+template <int L, int MODE, int QW, int H0, int H1, typename T, bool NEAR = false>
 __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const FieldParams &f, const LimbParams &lp,
-                                         gptr<T> tr, uint32_t tid, uint32_t lane, uint32_t wr) {
+                                         gptr<T> tr, uint32_t tid, uint32_t lane, uint32_t wr, uint32_t &nzcol) {
     constexpr uint32_t KA = L * KSTEP;
     constexpr uint32_t KAP = KA + 16;           // LDS row stride: consecutive columns start 4 banks apart
     constexpr int NC = 2 * L - 1;
@@ -532,7 +383,7 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
     const uint32_t cbase = it.cbase, width = it.width, x = it.x;
     // Steps: tile x + 8 (j / 2), half j % 2; the last tile's second half only if
     // it holds a column.
-    const uint32_t ntile = (it.C - x + 7) / 8;
+    const uint32_t ntile = NEAR ? 1u : (it.C - x + 7) / 8;
     const uint32_t last_cb = cbase + (x + 8 * (ntile - 1)) * UT;
     const uint32_t nstep = 2 * ntile - (last_cb + FSTEP < cbase + width ? 0u : 1u);
     auto step_cb = [&](uint32_t j) { return cbase + (x + 8 * (j >> 1)) * UT + FSTEP * (j & 1); };
@@ -647,6 +498,10 @@ __device__ __forceinline__ void far_item(const FarItem &it, int8_t *Bs, const Fi
                     else
                         v = sub_mod(add_mod(cv[q][r], lp.k2, f.p), limb_residue_k2<MODE, NC>(f, lp, P), f.p);
                     *((rok[r] && cok) ? mrow[r] + cb + c : tr) = (T)v;
+                    if constexpr (NEAR) {
+                        const uint32_t gcol = it.c0 + cb + c;
+                        if (rok[r] && cok && v != 0 && gcol < it.scan_end) nzcol = min(nzcol, gcol);
+                    }
                 }
                 fetch_c(jn, q);
             }
@@ -667,6 +522,7 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
     const gptr<uint64_t> tr = to_global(trash) + tid;
     const gptr<uint32_t> tr32 = to_global((uint32_t *)trash) + tid;
+    uint32_t nz_unused = 0;  // far items do not report nonzero columns
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const uint32_t mi = find_mat(item_off, nmat, item);
         const RankMat &S = mats[mi];
@@ -694,19 +550,100 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         it.ld = S.ld;
         if (m32) {  // p < 2^32
             if (!pair)
-                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16, nz_unused);
             else if (pb > 0)
-                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr32, tid, lane, wave * 16, nz_unused);
             else
-                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr32, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr32, tid, lane, wave * 16, nz_unused);
         } else {
             if (!pair)
-                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 1, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16, nz_unused);
             else if (pb > 0)
-                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 1>(it, Bs, f, lp, tr, tid, lane, wave * 16, nz_unused);
             else
-                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16);
+                far_item<L, MODE, QW, 0, 0>(it, Bs, f, lp, tr, tid, lane, wave * 16, nz_unused);
         }
+    }
+}
+
+// Near part of the update on the far update's machinery: one 64 x 64 tile per
+// item (the near part is one or two tiles wide and O rows tall), the wave's row
+// digits in registers, the tile's two 32-column halves of X digits staged in LDS
+// (double-buffered), C values fetched a half ahead; nonzero results below
+// scan_end are reported (the missed-pivot check).  A tile of an iteration
+// without pivots is only scanned for a nonzero.
+template <int L, int MODE, int QW>
+__global__ __launch_bounds__(FAR_THREADS, 2) void k_update_near(RankMat *mats, uint32_t nmat, const uint32_t *upd_off,
+                                                                const RankPlan *plan, FieldParams f, LimbParams lp,
+                                                                uint32_t par, uint64_t *trash) {
+    constexpr uint32_t KA = L * KSTEP;
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2 * 2 * FSTEP * (KA + 16)];
+    const uint32_t total = plan->near_total;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni32(tid >> 6);
+    const gptr<uint64_t> tr = to_global(trash) + tid;
+    const gptr<uint32_t> tr32 = to_global((uint32_t *)trash) + tid;
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t mi = find_mat(upd_off, nmat, item);
+        RankMat &S = mats[mi];
+        const IterState &I = S.it[par];
+        const uint32_t lt = item - upd_off[mi];
+        const uint32_t O = S.n - I.rank;
+        uint32_t cbase, width, r0, cb, nr, nc;
+        update_range(I, 0, cbase, width);
+        if (!tile_geom(O, cbase, width, 0, lt, r0, cb, nr, nc)) continue;  // uniform: padding tile
+        uint32_t nzcol = 0xffffffffu;
+        if (I.pb == 0) {
+            // detection only: thread = column tid % 64, rows tid / 64 + 4 k, every
+            // load issued before any test; a tile right of a nonzero column already
+            // found has nothing to add
+            const uint32_t known = __hip_atomic_load(&S.first_nz_col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t c = tid & 63, i0 = tid >> 6, gcol = I.c0 + cb + c;
+            if (I.c0 + cb <= known && c < nc && gcol < I.scan_end) {
+                constexpr uint32_t RK = UT / (FAR_THREADS / 64);
+                size_t idx[RK];
+#pragma unroll
+                for (uint32_t k = 0; k < RK; k++) {
+                    const uint32_t i = i0 + (FAR_THREADS / 64) * k;
+                    idx[k] = (size_t)S.perm[I.rank + r0 + min(i, nr - 1)] * S.ld + gcol;
+                }
+                bool any = false;
+                if (S.m32) {
+                    uint32_t v[RK];
+#pragma unroll
+                    for (uint32_t k = 0; k < RK; k++) v[k] = ((const uint32_t *)S.M)[idx[k]];
+#pragma unroll
+                    for (uint32_t k = 0; k < RK; k++) any |= i0 + (FAR_THREADS / 64) * k < nr && v[k] != 0;
+                } else {
+                    uint64_t v[RK];
+#pragma unroll
+                    for (uint32_t k = 0; k < RK; k++) v[k] = S.M[idx[k]];
+#pragma unroll
+                    for (uint32_t k = 0; k < RK; k++) any |= i0 + (FAR_THREADS / 64) * k < nr && v[k] != 0;
+                }
+                if (any) nzcol = gcol;
+            }
+        } else {
+            FarItem it;
+            it.x = (cb - cbase) / UT;
+            it.cbase = cbase;
+            it.width = width;
+            it.C = cdiv(width, UT);
+            it.M = S.M;
+            it.c0 = I.c0;
+            it.permg = to_global((const uint32_t *)S.perm);
+            it.Ag[0] = it.Ag[1] = to_global((const v4i *)S.Al);
+            it.Xg[0] = it.Xg[1] = to_global((const v4i *)own_xl(S, I));
+            it.r0 = r0;
+            it.nr = nr;
+            it.rank = I.rank;
+            it.ld = S.ld;
+            it.scan_end = I.scan_end;
+            if (uni32(S.m32))
+                far_item<L, MODE, QW, 1, 1, uint32_t, true>(it, Bs, f, lp, tr32, tid, lane, wave * 16, nzcol);
+            else
+                far_item<L, MODE, QW, 1, 1, uint64_t, true>(it, Bs, f, lp, tr, tid, lane, wave * 16, nzcol);
+        }
+        if (I.resid && nzcol != 0xffffffffu) atomicMin(&S.first_nz_col, nzcol);
     }
 }
 
@@ -717,9 +654,12 @@ static void launch_mfma_L(hipStream_t st, RankMat *dmats, uint32_t nmat, const F
     if (what < 0)  // -1: rows and near X; -2: far X
         hipLaunchKernelGGL((k_gather_split<L, MODE>), dim3(rl.grid_gx), dim3(512), 0, st, dmats, nmat,
                            what == -1 ? rl.gx_off : rl.xf_off, rl.plan, f, lp, par, what == -1 ? 0u : 1u);
+    else if (what == 0 && L <= 4 && rl.far_qw != 1)
+        hipLaunchKernelGGL((k_update_near<L, MODE, (L <= 4 ? 2 : 1)>), dim3(rl.grid_near), dim3(FAR_THREADS), 0, st,
+                           dmats, nmat, rl.near_off, rl.plan, f, lp, par, rl.trash);
     else if (what == 0)
-        hipLaunchKernelGGL((k_update_mfma<L, MODE>), dim3(rl.grid_near), dim3(512), 0, st, dmats, nmat, rl.near_off,
-                           rl.plan, f, lp, par, 0u);
+        hipLaunchKernelGGL((k_update_near<L, MODE, 1>), dim3(rl.grid_near), dim3(FAR_THREADS), 0, st, dmats, nmat,
+                           rl.near_off, rl.plan, f, lp, par, rl.trash);
     else if (L <= 4 && rl.far_qw != 1)
         hipLaunchKernelGGL((k_update_far<L, MODE, (L <= 4 ? 2 : 1)>), dim3(rl.grid_far), dim3(FAR_THREADS), 0, st, dmats,
                            nmat, rl.far_off, rl.plan, f, lp, par, rl.trash);
