@@ -300,6 +300,24 @@ MDL_HD uint64_t ff_step_m61_lazy(uint64_t pi, uint64_t x, uint64_t e, uint64_t y
     return lin2_m61_lazy(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)ne & 0x7fffffffu,
                          (uint32_t)(ne >> 31));
 }
+// lin2_m61_lazy for throughput: t first, then s accumulated by multiply-adds onto
+// t's two parts, r = (t >> 30) + (t mod 2^30) 2^31 + s < 2^33 + 2^61 + 1.5 2^63 < 2^64.
+// Eight multiply-adds and no separate 64-bit adds, at a dependent depth of eight:
+// for code that interleaves several independent ones.
+MDL_HD uint64_t lin2_m61_lazy_acc(uint64_t x, uint64_t y, uint32_t b0, uint32_t b1, uint32_t d0, uint32_t d1) {
+    const uint32_t x0 = (uint32_t)x & 0x7fffffffu, x1 = (uint32_t)(x >> 31);
+    const uint32_t y0 = (uint32_t)y & 0x7fffffffu, y1 = (uint32_t)(y >> 31);
+    uint64_t t = (uint64_t)x0 * b1;
+    t += (uint64_t)x1 * b0;
+    t += (uint64_t)y0 * d1;
+    t += (uint64_t)y1 * d0;
+    uint64_t r = (t >> 30) + ((t & 0x3fffffffull) << 31);
+    r += (uint64_t)x0 * b0;
+    r += (uint64_t)x1 * (2u * b1);
+    r += (uint64_t)y0 * d0;
+    r += (uint64_t)y1 * (2u * d1);
+    return (r & P61) + (r >> 61);  // <= p + 7
+}
 
 template <int MODE>
 MDL_HD uint64_t ff_step_t(const FieldParams &f, uint64_t pi, uint64_t x, uint64_t e, uint64_t y) {

@@ -101,16 +101,104 @@ __device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32
 // k_block_ge: pivot search and fraction-free elimination of the panel part of
 // the top block
 // ---------------------------------------------------------------------------
+// The block elimination runs on GE_WAVES waves, two per SIMD: the step loop is
+// bound by VALU issue, every wave repeats the pivot search, and two waves per
+// SIMD are what keeps a SIMD issuing.  Wave v keeps the panel columns v + 8 m.
+constexpr uint32_t GE_WAVES = 8, GE_COLS = BMAX / GE_WAVES;
+
+// One fraction-free step of a block row: x <- pi x - e y where the row takes it
+// (act), x's residue where it does not.  For 2^61 - 1 the choice is folded into
+// the coefficients, (b, d) = (pi, p - e) or (1, 0), in 31-bit halves per lane:
+// lin2_m61_lazy_acc then needs no select, and with (1, 0) it returns x for
+// x < 2^61 and x - p for x in [2^61, p + 7].  Results stay in [0, p + 7].
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
-                                                   uint32_t defer) {
+struct FFStep {
+    uint64_t pi, e;
+    bool act;
+    uint32_t b0, b1, d0, d1;
+    __device__ __forceinline__ FFStep(uint64_t pi_, uint64_t e_, bool act_)
+        : pi(pi_), e(e_), act(act_), b0(0), b1(0), d0(0), d1(0) {
+        if constexpr (MODE == MODE_M61) {
+            const uint64_t ne = P61 - e;
+            b0 = act ? ((uint32_t)pi & 0x7fffffffu) : 1u;
+            b1 = act ? (uint32_t)(pi >> 31) : 0u;
+            d0 = act ? ((uint32_t)ne & 0x7fffffffu) : 0u;
+            d1 = act ? (uint32_t)(ne >> 31) : 0u;
+        }
+    }
+    __device__ __forceinline__ uint64_t operator()(const FieldParams &f, uint64_t x, uint64_t y) const {
+        if constexpr (MODE == MODE_M61)
+            return lin2_m61_lazy_acc(x, y, b0, b1, d0, d1);
+        else
+            return act ? ff_step_t<MODE>(f, pi, x, e, y) : x;
+    }
+};
+
+// A step on the wave's panel columns wave + 8 m for m >= M0 (those after the
+// pivot column) and, with OWN, on the own column.  Their products are independent
+// and interleave (for 2^61 - 1 written stage by stage across the columns, so the
+// multiply-add chains interleave in the instruction stream too).
+template <int MODE, int M0, bool OWN>
+__device__ __forceinline__ void ge_columns(const FieldParams &f, const FFStep<MODE> &st,
+                                           uint64_t (&x)[GE_COLS], uint64_t &own, uint32_t q) {
+    constexpr int NC = GE_COLS, NS = OWN ? NC + 1 : NC;  // slot NC: the own column (y = 0)
+    uint64_t y[NC];
+#pragma unroll
+    for (int m = M0; m < NC; m++) y[m] = lane_bcast(x[m], q);
+    if constexpr (MODE == MODE_M61) {
+        uint32_t x0[NS], x1[NS], y0[NS], y1[NS];
+        uint64_t t[NS], s[NS];
+#pragma unroll
+        for (int m = M0; m < NS; m++) {
+            const uint64_t xv = m < NC ? x[m] : own, yv = m < NC ? y[m] : 0;
+            x0[m] = (uint32_t)xv & 0x7fffffffu;
+            x1[m] = (uint32_t)(xv >> 31);
+            y0[m] = (uint32_t)yv & 0x7fffffffu;
+            y1[m] = (uint32_t)(yv >> 31);
+        }
+#pragma unroll
+        for (int m = M0; m < NS; m++) t[m] = (uint64_t)x0[m] * st.b1;
+#pragma unroll
+        for (int m = M0; m < NS; m++) s[m] = (uint64_t)x0[m] * st.b0;
+#pragma unroll
+        for (int m = M0; m < NS; m++) t[m] += (uint64_t)x1[m] * st.b0;
+#pragma unroll
+        for (int m = M0; m < NS; m++) s[m] += (uint64_t)x1[m] * (2u * st.b1);
+#pragma unroll
+        for (int m = M0; m < NS; m++) t[m] += (uint64_t)y0[m] * st.d1;
+#pragma unroll
+        for (int m = M0; m < NS; m++) s[m] += (uint64_t)y0[m] * st.d0;
+#pragma unroll
+        for (int m = M0; m < NS; m++) t[m] += (uint64_t)y1[m] * st.d0;
+#pragma unroll
+        for (int m = M0; m < NS; m++) s[m] += (uint64_t)y1[m] * (2u * st.d1);
+#pragma unroll
+        for (int m = M0; m < NS; m++) {
+            // s < 1.5 2^63, t < 2^63: the sum below is < 2^64 (lin2_m61_lazy_acc)
+            const uint64_t v = s[m] + (t[m] >> 30) + ((t[m] & 0x3fffffffull) << 31);
+            const uint64_t z = (v & P61) + (v >> 61);  // <= p + 7
+            if (m < NC)
+                x[m] = z;
+            else
+                own = z;
+        }
+    } else {
+#pragma unroll
+        for (int m = M0; m < NC; m++) x[m] = st(f, x[m], y[m]);
+        if constexpr (OWN) own = st(f, own, 0);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
+                                                            uint32_t defer) {
     RankMat &S = mats[blockIdx.x];
-    // One LDS row per block row: [0, BMAX) panel entries T, then own (the
-    // coefficient on the row's own original row).
+    // U: one padded row per block row: the block as loaded, then, after the
+    // elimination, its final state (panel columns, and the own column COWN).
     constexpr uint32_t COWN = BMAX, UW = BMAX + 1;
     __shared__ uint64_t U[BMAX][UW];
-    __shared__ uint64_t Et[BMAX][BMAX];     // multiplier e of each block row at each step
-    __shared__ uint64_t piK[BMAX], ownK[BMAX];
+    __shared__ uint64_t colb[2][BMAX];  // the column of the next pivot search, double-buffered
+    __shared__ uint64_t piK[BMAX];
     __shared__ uint32_t rowid[BMAX], Jl[BMAX], Qk[BMAX];
     if (S.done) return;
 
@@ -129,90 +217,124 @@ __global__ __launch_bounds__(1024) void k_block_ge(RankMat *mats, FieldParams f,
         const uint32_t src = (pend == 0xffffffffu) ? pos : (pos == rank0 ? pend : (pos == pend ? rank0 : pos));
         rowid[tid] = S.perm[src];
     }
-    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
     __syncthreads();
     {
-        // 16 lanes per block row, 4 panel columns each: all loads issued first.
-        const uint32_t q = tid >> 4, c0 = tid & 15;
+        // GE_WAVES lanes per block row, GE_COLS panel columns each: all loads issued first.
+        const uint32_t q = tid / GE_WAVES, c0 = tid % GE_WAVES;
         const size_t src = (size_t)rowid[q < Bt ? q : 0] * ld + col;
-        uint64_t v[4];
+        uint64_t v[GE_COLS];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const uint32_t c = c0 + 16 * m;
+        for (int m = 0; m < (int)GE_COLS; m++) {
+            const uint32_t c = c0 + GE_WAVES * m;
             v[m] = (q < Bt && c < w) ? m_get(S, src + c) : 0;
         }
 #pragma unroll
-        for (int m = 0; m < 4; m++) U[q][c0 + 16 * m] = v[m];
-        if (c0 == 0) U[q][COWN] = q < Bt ? 1 % f.p : 0;
+        for (int m = 0; m < (int)GE_COLS; m++) U[q][c0 + GE_WAVES * m] = v[m];
     }
     __syncthreads();
 
     // Rows stay where they are (lane = block row).  The pivot of column j is the
     // first block row that is not yet a pivot and has a nonzero in column j: any
     // pivot choice gives the rank (the reference takes the first nonzero row at or
-    // below the current position and swaps it up, fp.rs:150-163).  Wave v owns the
-    // panel columns v, v + 16, v + 32, v + 48 (and, at step j, the own column if
-    // v = j mod 16).  A step reads column j (for the ballot, and as this lane's
-    // multiplier e) and the lane's own columns, all before the ballot; the pivot
-    // row's entries come from its lane by readlane.
+    // below the current position and swaps it up, fp.rs:150-163).  Wave v keeps
+    // the panel columns v + 8 m (m < 8) in registers for the whole elimination,
+    // and wave 0 also the own column (the coefficient on the row's own original
+    // row), so the waves' counts of columns after the pivot column differ by at
+    // most one at every step.  The pivot row's entries come from its lane by
+    // readlane; only the column of the next search goes through LDS: its owner
+    // publishes it at the end of a step, and one barrier per step separates that
+    // from the reads (two buffers, so the next column's write cannot meet a late
+    // read of this one).
     const uint32_t lane = tid & 63, wave = uni32(tid >> 6);
     const bool inblk = lane < Bt;
+    uint64_t x[GE_COLS];
+#pragma unroll
+    for (int m = 0; m < (int)GE_COLS; m++) x[m] = U[lane][wave + GE_WAVES * m];
+    uint64_t own = inblk ? 1 % f.p : 0;
+    if (wave == 0) colb[0][lane] = x[0];
+    __syncthreads();  // (also: every read of the loaded block precedes the final state's writes)
+
+    uint64_t *eta = S.Eta;
     uint64_t piv = 0;  // block rows that are pivots (the same in every lane)
     uint32_t k = 0;
     for (uint32_t j = 0; j < w && k < Bt; j++) {
-        const uint64_t colv = U[lane][j];
-        const bool own = wave == (j & 15);
-        uint64_t xv[5];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const uint32_t c = wave + 16 * m;
-            if (c > j && c < w) xv[m] = U[lane][c];  // uniform over the wave
-        }
-        if (own) xv[4] = U[lane][COWN];
+        const uint64_t colv = colb[j & 1][lane];
         const bool free_row = inblk && !((piv >> lane) & 1);
         const uint64_t mask = __ballot(free_row && nz_t<MODE>(colv));
-        if (mask == 0) continue;  // uniform: no pivot in this column
-        const uint32_t q = __ffsll((unsigned long long)mask) - 1;
-        // Fraction-free step on the other free rows with e != 0 (fp.rs:166-174
-        // without the division): x <- pi x - e y on the panel columns after j, own
-        // scaled by pi.  Row q and column j are only read, so no barrier separates
-        // the reads from the writes.
-        const uint64_t pi = canon_t<MODE>(lane_bcast(colv, q));
-        const uint64_t e = canon_t<MODE>(colv);
-        const bool act = free_row && lane != q && e != 0;
-        if (wave == 0 && act) Et[k][lane] = e;
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const uint32_t c = wave + 16 * m;
-            if (c > j && c < w) {
-                const uint64_t y = lane_bcast(xv[m], q);
-                if (act) U[lane][c] = ff_lazy_t<MODE>(f, pi, xv[m], e, y);
+        if (mask != 0) {  // uniform (else no pivot in this column)
+            const uint32_t q = __ffsll((unsigned long long)mask) - 1;
+            // Fraction-free step on the other free rows with e != 0 (fp.rs:166-174
+            // without the division): x <- pi x - e y on the panel columns after j,
+            // own scaled by pi.  A pivot row takes no step from then on, so its
+            // entries after its column, and its own, are final: the record reads
+            // them at the end.
+            const uint64_t pi = canon_t<MODE>(lane_bcast(colv, q));
+            const uint64_t e = canon_t<MODE>(colv);
+            const bool act = free_row && lane != q && e != 0;
+            const FFStep<MODE> st(pi, e, act);
+            // this wave's columns after j: wave + 8 m for m >= mlo
+            const uint32_t mlo = j < wave ? 0u : (j - wave) / GE_WAVES + 1;
+            if (wave == 0) {
+                eta[ETA_E + k * BMAX + lane] = act ? e : 0;  // the step's multipliers
+                if (lane == 0) {
+                    Jl[k] = j;
+                    piK[k] = pi;
+                    Qk[k] = q;
+                }
+                switch (mlo) {
+                    case 0: ge_columns<MODE, 0, true>(f, st, x, own, q); break;
+                    case 1: ge_columns<MODE, 1, true>(f, st, x, own, q); break;
+                    case 2: ge_columns<MODE, 2, true>(f, st, x, own, q); break;
+                    case 3: ge_columns<MODE, 3, true>(f, st, x, own, q); break;
+                    case 4: ge_columns<MODE, 4, true>(f, st, x, own, q); break;
+                    case 5: ge_columns<MODE, 5, true>(f, st, x, own, q); break;
+                    case 6: ge_columns<MODE, 6, true>(f, st, x, own, q); break;
+                    case 7: ge_columns<MODE, 7, true>(f, st, x, own, q); break;
+                    default: ge_columns<MODE, 8, true>(f, st, x, own, q); break;
+                }
+            } else {
+                switch (mlo) {
+                    case 0: ge_columns<MODE, 0, false>(f, st, x, own, q); break;
+                    case 1: ge_columns<MODE, 1, false>(f, st, x, own, q); break;
+                    case 2: ge_columns<MODE, 2, false>(f, st, x, own, q); break;
+                    case 3: ge_columns<MODE, 3, false>(f, st, x, own, q); break;
+                    case 4: ge_columns<MODE, 4, false>(f, st, x, own, q); break;
+                    case 5: ge_columns<MODE, 5, false>(f, st, x, own, q); break;
+                    case 6: ge_columns<MODE, 6, false>(f, st, x, own, q); break;
+                    case 7: ge_columns<MODE, 7, false>(f, st, x, own, q); break;
+                    default: break;
+                }
             }
+            piv |= 1ull << q;
+            k++;
         }
-        if (own && act) U[lane][COWN] = ff_lazy_t<MODE>(f, pi, xv[4], e, 0);
-        if (tid == 0) {
-            Jl[k] = j;
-            piK[k] = pi;
-            ownK[k] = canon_t<MODE>(U[q][COWN]);  // row q takes no step now
-            Qk[k] = q;
+        const uint32_t jn = j + 1;
+        if (jn < w && wave == jn % GE_WAVES) {
+            uint64_t v = x[0];
+#pragma unroll
+            for (int m = 1; m < (int)GE_COLS; m++)
+                if (jn / GE_WAVES == (uint32_t)m) v = x[m];
+            colb[jn & 1][lane] = v;
         }
         __syncthreads();
-        piv |= 1ull << q;
-        k++;
     }
     const uint32_t pb = k;
+    // the final state, for the record
+#pragma unroll
+    for (int m = 0; m < (int)GE_COLS; m++) U[lane][wave + GE_WAVES * m] = x[m];
+    if (wave == 0) U[lane][COWN] = own;
+    __syncthreads();
 
-    // The record for k_block_z, which builds A^-1 from it.  A pivot row is final
-    // once it is the pivot, so U[r][s] = U[Q[r]][J_s] for s > r.
-    uint64_t *eta = S.Eta;
+    // The record for k_block_z, which builds A^-1 from it: the multipliers (above,
+    // by step), the pivot rows' entries at the later pivot columns, U[r][s] =
+    // entry of pivot row r at column J_s for s > r, and the pivot rows' own.
     for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) {
         const uint32_t r = idx / BMAX, s = idx % BMAX;
-        eta[ETA_E + idx] = Et[r][s];
         eta[ETA_U + idx] = (s > r && s < pb) ? canon_t<MODE>(U[Qk[r]][Jl[s]]) : 0;
     }
     if (tid < pb) {  // (the pivots' inverses are computed by k_block_z, off this kernel's path)
         eta[ETA_PI + tid] = piK[tid];
-        eta[ETA_OWNK + tid] = ownK[tid];
+        eta[ETA_OWNK + tid] = canon_t<MODE>(U[Qk[tid]][COWN]);
         eta[ETA_Q + tid] = Qk[tid];
     }
     // The new order of the block -- the pivot rows in pivot order, then the other
@@ -735,13 +857,13 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
                   uint32_t L, uint32_t defer) {
     if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
         hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
     }
 }

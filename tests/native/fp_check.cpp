@@ -48,6 +48,13 @@ static void run(uint64_t p, int n) {
                 if (bad < 8) printf("ff_step_m61_lazy mismatch\n");
                 bad++;
             }
+            const uint64_t ne = P61 - e;
+            const uint64_t za = lin2_m61_lazy_acc(xl, yl, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31),
+                                                  (uint32_t)ne & 0x7fffffffu, (uint32_t)(ne >> 31));
+            if (za > P61 + 7 || za % P61 != want) {
+                if (bad < 8) printf("lin2_m61_lazy_acc mismatch\n");
+                bad++;
+            }
             const uint64_t l = lin2_m61(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31), (uint32_t)e & 0x7fffffffu,
                                         (uint32_t)(e >> 31));
             if (l != (uint64_t)(((u128)pi * x + (u128)e * y) % p)) {
@@ -81,7 +88,31 @@ static void run_sum(uint64_t p, int n) {
     }
 }
 
+// The lazy 2^61 - 1 steps at the ends of their operand ranges: x, y up to p + 7,
+// pi up to p - 1, e = 0 (so d = p) and e up to p - 1; every pair of corners.
+static void m61_lazy_corners() {
+    const uint64_t xs[] = {0, 1, 7, P61 - 1, P61, P61 + 1, P61 + 7, 0x7fffffffull, 0x80000000ull,
+                           (1ull << 60) + 12345, P61 - 0x7fffffffull};
+    const uint64_t cs[] = {0, 1, 2, P61 - 1, P61 - 2, 0x7fffffffull, 0x80000000ull, 1ull << 60, (1ull << 61) - 0x80000000ull};
+    for (uint64_t x : xs)
+        for (uint64_t y : xs)
+            for (uint64_t pi : cs)
+                for (uint64_t e : cs) {
+                    const uint64_t want =
+                        (uint64_t)(((u128)pi * (x % P61) + (u128)(P61 - e) % P61 * (y % P61)) % P61);
+                    const uint64_t z = ff_step_m61_lazy(pi, x, e, y);
+                    const uint64_t ne = P61 - e;
+                    const uint64_t za = lin2_m61_lazy_acc(x, y, (uint32_t)pi & 0x7fffffffu, (uint32_t)(pi >> 31),
+                                                          (uint32_t)ne & 0x7fffffffu, (uint32_t)(ne >> 31));
+                    if (z > P61 + 7 || z % P61 != want || za > P61 + 7 || za % P61 != want) {
+                        if (bad < 8) printf("lazy 2^61-1 corner mismatch\n");
+                        bad++;
+                    }
+                }
+}
+
 int main() {
+    m61_lazy_corners();
     run_sum<MODE_M61>(P61, 20000);
     run_sum<MODE_M31>(P31, 5000);
     run_sum<MODE_GENERIC>(4611686018427387847ull, 5000);
