@@ -41,10 +41,6 @@ __device__ __forceinline__ uint64_t lane_bcast(uint64_t v, uint32_t src) {
 // Panel steps keep 2^61 - 1 residues lazily in [0, p + 7] (one fold per step, no
 // final compare); other moduli stay canonical.  Records are canonical.
 template <int MODE>
-__device__ __forceinline__ bool nz_t(uint64_t v) {
-    return MODE == MODE_M61 ? (v != 0 && v != P61) : v != 0;
-}
-template <int MODE>
 __device__ __forceinline__ uint64_t canon_t(uint64_t v) {
     return MODE == MODE_M61 ? (v >= P61 ? v - P61 : v) : v;
 }
@@ -258,9 +254,9 @@ __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, Field
     uint64_t piv = 0;  // block rows that are pivots (the same in every lane)
     uint32_t k = 0;
     for (uint32_t j = 0; j < w && k < Bt; j++) {
-        const uint64_t colv = colb[j & 1][lane];
+        const uint64_t colv = colb[j & 1][lane];  // canonical (published so)
         const bool free_row = inblk && !((piv >> lane) & 1);
-        const uint64_t mask = __ballot(free_row && nz_t<MODE>(colv));
+        const uint64_t mask = __ballot(free_row && colv != 0);
         if (mask != 0) {  // uniform (else no pivot in this column)
             const uint32_t q = __ffsll((unsigned long long)mask) - 1;
             // Fraction-free step on the other free rows with e != 0 (fp.rs:166-174
@@ -268,8 +264,8 @@ __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, Field
             // own scaled by pi.  A pivot row takes no step from then on, so its
             // entries after its column, and its own, are final: the record reads
             // them at the end.
-            const uint64_t pi = canon_t<MODE>(lane_bcast(colv, q));
-            const uint64_t e = canon_t<MODE>(colv);
+            const uint64_t pi = lane_bcast(colv, q);
+            const uint64_t e = colv;
             const bool act = free_row && lane != q && e != 0;
             const FFStep<MODE> st(pi, e, act);
             // this wave's columns after j: wave + 8 m for m >= mlo
@@ -314,7 +310,7 @@ __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, Field
 #pragma unroll
             for (int m = 1; m < (int)GE_COLS; m++)
                 if (jn / GE_WAVES == (uint32_t)m) v = x[m];
-            colb[jn & 1][lane] = v;
+            colb[jn & 1][lane] = canon_t<MODE>(v);
         }
         __syncthreads();
     }
