@@ -185,7 +185,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
 
     // Workspace layout (256-byte aligned regions).
     std::vector<size_t> offM(count), offP(count), offA(count), offX(count), offZ(count), offE(count), offS(count),
-        offAl(count), offXl(count), offZl(count), offAlP(count), offXlP(count);
+        offAl(count), offXl(count), offZl(count), offAlP(count), offXlP(count), offR(count);
     size_t cur = 0;
     const size_t A = 256;
     // the matrices' state, their set-up jobs and the active count, copied to the
@@ -195,16 +195,21 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     cur += count * sizeof(SetupJob);
     const size_t off_active = cur;
     cur = align_up(cur + sizeof(uint32_t), A);
-    const size_t off_gx = cur;
-    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
-    const size_t off_upd = cur;
-    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
-    const size_t off_far = cur;
-    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
-    const size_t off_xf = cur;
-    cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
-    const size_t off_plan = cur;
-    cur = align_up(cur + sizeof(RankPlan), A);
+    // the work plan, one per iteration parity (a far update reads its iteration's
+    // while the next k_plan writes the other)
+    size_t off_gx[2], off_upd[2], off_far[2], off_xf[2], off_plan[2];
+    for (int q = 0; q < 2; q++) {
+        off_gx[q] = cur;
+        cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+        off_upd[q] = cur;
+        cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+        off_far[q] = cur;
+        cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+        off_xf[q] = cur;
+        cur = align_up(cur + (count + 1) * sizeof(uint32_t), A);
+        off_plan[q] = cur;
+        cur = align_up(cur + sizeof(RankPlan), A);
+    }
     const size_t off_trash = cur;
     cur = align_up(cur + TRASH_WORDS * sizeof(uint64_t), A);
     const size_t off_out = cur;  // k_collect's output: active, then rank, macs, bytes, far macs per matrix
@@ -233,11 +238,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         offS[i] = 0;
         const bool in32 = mats[i].elem == MODLIN_ELEM_U32;
         if (in32 != m32 && mats[i].mem != MODLIN_MEM_DEVICE) { offS[i] = cur; cur = align_up(cur + n * n * (in32 ? 4 : 8), A); }
-        offAl[i] = offXl[i] = offZl[i] = offAlP[i] = offXlP[i] = 0;
+        offAl[i] = offXl[i] = offZl[i] = offAlP[i] = offXlP[i] = offR[i] = 0;
         if (use_mfma) {
             // digit columns are read in whole 64-column tiles: each X buffer is
             // followed by 64 columns' worth of slack
             const size_t npad = (n + UT - 1) / UT * UT;
+            offR[i] = cur; cur = align_up(cur + n * 4, A);
             offAl[i] = cur; cur = align_up(cur + npad * KA, A);
             offXl[i] = cur; cur = align_up(cur + npad * KA, A);
             offZl[i] = cur; cur = align_up(cur + 2 * BMAX * KA, A);  // one per iteration parity
@@ -272,6 +278,7 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
         S.Zl = use_mfma ? (int8_t *)(base + offZl[i]) : nullptr;
         S.AlP = defer ? (int8_t *)(base + offAlP[i]) : nullptr;
         S.XlP = defer ? (int8_t *)(base + offXlP[i]) : nullptr;
+        S.rows = use_mfma ? (uint32_t *)(base + offR[i]) : nullptr;
         S.n = (uint32_t)n;
         S.ld = (uint32_t)n;
         S.m32 = m32 ? 1u : 0u;
@@ -300,11 +307,6 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     }
     RankMat *dmats = (RankMat *)(base + 0);
     RankLaunch rl;
-    rl.gx_off = (uint32_t *)(base + off_gx);
-    rl.near_off = (uint32_t *)(base + off_upd);
-    rl.far_off = (uint32_t *)(base + off_far);
-    rl.xf_off = (uint32_t *)(base + off_xf);
-    rl.plan = (RankPlan *)(base + off_plan);
     rl.trash = (uint64_t *)(base + off_trash);
     uint32_t *dactive = (uint32_t *)(base + off_active);
     *(uint32_t *)(himg.data() + off_active) = active;
@@ -323,6 +325,14 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     rl.grid_far = ((uint32_t)ctx->cu_count * (tu.far_per_cu ? tu.far_per_cu : (use_mfma ? 4 : 8)) + 7) / 8 * 8;
     rl.far_items = use_mfma ? 1u : 0u;
     rl.far_qw = tu.far_quarters ? tu.far_quarters : 2u;
+    RankLaunch rlp[2] = {rl, rl};  // by iteration parity: the plan buffers
+    for (int q = 0; q < 2; q++) {
+        rlp[q].gx_off = (uint32_t *)(base + off_gx[q]);
+        rlp[q].near_off = (uint32_t *)(base + off_upd[q]);
+        rlp[q].far_off = (uint32_t *)(base + off_far[q]);
+        rlp[q].xf_off = (uint32_t *)(base + off_xf[q]);
+        rlp[q].plan = (RankPlan *)(base + off_plan[q]);
+    }
     const int check_every = tu.check_every ? (int)tu.check_every : 8;
     std::vector<hipEvent_t> evs;
     if (profile) {
@@ -338,32 +348,38 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
     // far columns' X product (MFMA method) from the plan on, beside the gather and
     // near update; the far update once the near update is done (its digit rows
     // are written, and the two do not share the GPU), overlapping the finish and
-    // the next block elimination; the next plan (which commits
-    // the row order the far update reads through, and rewrites the work queue
-    // it takes items from) waits for it.  (check_every is even, so iteration
-    // parities repeat per block.)
+    // the next block elimination and plan.  The next gather (which rewrites the
+    // digit rows the far update reads, and reads pivot rows in columns it
+    // writes) waits for it.  MFMA method: the far update reaches its rows through
+    // the gather's row snapshot and its work queue through its iteration's plan
+    // buffer, so the next k_plan (which reorders perm and writes the other plan
+    // buffer) does not wait; the plain method's far update reads perm, so there
+    // the next plan waits.  (check_every is even, so iteration parities repeat
+    // per block.)
     hipStream_t fs = ctx->far_stream;
     auto enqueue_block = [&](bool prof) -> hipError_t {
         for (int s = 0; s < check_every; s++) {
             const uint32_t par = (uint32_t)(s & 1);
+            const RankLaunch &rq = rlp[par];
             hipError_t e;
             launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
-            if ((e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
-            launch_plan(st, dmats, nmat, rl, par);
+            if (!use_mfma && (e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
+            launch_plan(st, dmats, nmat, rq, par);
             if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(fs, ctx->ev_plan, 0)) != hipSuccess) return e;
             // far X: reads the plan, Z's digits (this parity) and the pivot rows, none
             // of which the gather, near update or finish writes
-            if (use_mfma) launch_mfma(fs, dmats, nmat, f, lp, rl, par, -2);
+            if (use_mfma) launch_mfma(fs, dmats, nmat, f, lp, rq, par, -2);
+            if (use_mfma && (e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
             if (use_mfma)
-                launch_mfma(st, dmats, nmat, f, lp, rl, par, -1);
+                launch_mfma(st, dmats, nmat, f, lp, rq, par, -1);
             else
-                launch_gather_plain(st, dmats, nmat, f, rl, par);
+                launch_gather_plain(st, dmats, nmat, f, rq, par);
             if (prof) hipEventRecord(evs[4 * s], st);
             if (use_mfma)
-                launch_mfma(st, dmats, nmat, f, lp, rl, par, 0);
+                launch_mfma(st, dmats, nmat, f, lp, rq, par, 0);
             else
-                launch_update_plain(st, dmats, nmat, f, rl, par, 0);
+                launch_update_plain(st, dmats, nmat, f, rq, par, 0);
             if (prof) hipEventRecord(evs[4 * s + 1], st);
             if ((e = hipEventRecord(ctx->ev_gather, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(fs, ctx->ev_gather, 0)) != hipSuccess) return e;
@@ -371,9 +387,9 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             // far update: columns the finish and the next block elimination do not touch
             if (prof) hipEventRecord(evs[4 * s + 2], fs);
             if (use_mfma)
-                launch_mfma(fs, dmats, nmat, f, lp, rl, par, 1);
+                launch_mfma(fs, dmats, nmat, f, lp, rq, par, 1);
             else
-                launch_update_plain(fs, dmats, nmat, f, rl, par, 1);
+                launch_update_plain(fs, dmats, nmat, f, rq, par, 1);
             if (prof) hipEventRecord(evs[4 * s + 3], fs);
             if ((e = hipEventRecord(ctx->ev_far, fs)) != hipSuccess) return e;
         }

@@ -189,7 +189,10 @@ __global__ __launch_bounds__(512) void k_gather_split(RankMat *mats, uint32_t nm
                     iu[u] = idx / (KSTEP / 4);
                     ku[u] = (idx - iu[u] * (KSTEP / 4)) * 4;
                     ok[u] = iu[u] < rows;
-                    mrow[u] = (size_t)S.perm[I.rank + r0 + (ok[u] ? iu[u] : 0)] * ld;
+                    const uint32_t prow = S.perm[I.rank + r0 + (ok[u] ? iu[u] : 0)];
+                    mrow[u] = (size_t)prow * ld;
+                    // the row snapshot of this iteration, for the far update
+                    if (ok[u] && ku[u] == 0 && pass == (pb > 0 ? 0u : 1u)) S.rows[r0 + iu[u]] = prow;
                 }
                 uint64_t x[2][4];
 #pragma unroll
@@ -358,7 +361,7 @@ constexpr uint32_t FSTEP = 32;  // columns per step
 struct FarItem {
     uint64_t *M;  // the working copy (u64 or, when m32, u32 entries), offset by c0 in far_item
     uint32_t c0;
-    gptr<const uint32_t> permg;
+    gptr<const uint32_t> permg;    // row r0 + i of the item is physical row permg[rank + r0 + i]
     gptr<const v4i> Ag[2], Xg[2];  // panel h: 0 the pending one, 1 this iteration's own
     uint32_t x, r0, nr, rank, ld, cbase, width, C;
     uint32_t scan_end;  // NEAR: nonzero results in columns c0 + c < scan_end are reported
@@ -539,14 +542,16 @@ __global__ __launch_bounds__(FAR_THREADS, 2) void k_update_far(RankMat *mats, ui
         it.M = S.M;
         it.c0 = I.c0;
         const bool m32 = uni32(S.m32) != 0;
-        it.permg = to_global((const uint32_t *)S.perm);
+        // rows through the gather's snapshot (rank + r -> rows[r]): the next k_plan
+        // may reorder perm while this update runs
+        it.permg = to_global((const uint32_t *)S.rows);
         it.Ag[0] = to_global((const v4i *)S.AlP);
         it.Ag[1] = to_global((const v4i *)S.Al);
         it.Xg[0] = to_global((const v4i *)S.XlP) + (size_t)(pair ? I.dxoff : 0u) * (KA / 16);
         it.Xg[1] = to_global((const v4i *)S.Xl);
         it.r0 = band * UT;
         it.nr = min(UT, O - it.r0);
-        it.rank = I.rank;
+        it.rank = 0;
         it.ld = S.ld;
         if (m32) {  // p < 2^32
             if (!pair)

@@ -62,6 +62,9 @@ struct RankMat {
                        // parity: the far X product of an iteration reads it while the next k_block_z writes
     int8_t *AlP;       // IT_PAIR: int8 limb planes of M[O, J'] for the pending panel's columns J'
     int8_t *XlP;       // IT_DEFER: int8 limb planes of its X (read by its near update and the next PAIR)
+    uint32_t *rows;    // MFMA path: physical row of position rank + r, written by the gather's row
+                       // tasks; the far update addresses its rows through it, so the next k_plan
+                       // may reorder perm while it runs
     uint32_t n, ld;
     uint32_t m32;            // M holds u32 entries (p < 2^32, MFMA method): half the update traffic
     uint32_t rank, col;      // pivots found; first column not yet eliminated

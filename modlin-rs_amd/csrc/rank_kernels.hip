@@ -334,8 +334,7 @@ __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, Field
         eta[ETA_Q + tid] = Qk[tid];
     }
     // The new order of the block -- the pivot rows in pivot order, then the other
-    // rows in block order -- is applied to perm by k_plan, once the previous
-    // iteration's update (which reads perm) has finished.
+    // rows in block order -- is applied to perm by k_plan.
     IterState &I = S.it[par];
     if (tid < pb) {
         const uint32_t r = rowid[Qk[tid]];
@@ -627,7 +626,10 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t n
     const uint32_t tid = threadIdx.x;
     // Commit the last iteration's missed-pivot swap, then the top blocks' new row
     // order (written by k_block_ge of this iteration, which saw the swap).  The
-    // previous iteration's far update, which read perm, has finished.
+    // previous iteration's far update may still run: the plain method's waits
+    // for it before this kernel (it reads perm); the MFMA method's reads its rows
+    // through the gather's snapshot instead, and this iteration's plan buffer is
+    // not the one it reads.
     for (uint32_t i = tid; i < nmat; i += blockDim.x) {
         RankMat &S = mats[i];
         const uint32_t pend = S.pend;
