@@ -1,10 +1,11 @@
 // Host orchestration of the rank engine: workspace layout in HBM, the panel
 // iteration loop and the context.  The reference's driver loop is
 // src/fp.rs:138-252 (one matrix); here every matrix of a batch advances one
-// panel per iteration.  An iteration is: block elimination and inverse (aux
-// stream), plan, gather, near update, finish (main stream), then the far update
-// (main stream) -- which overlaps the next iteration's block elimination: the
-// next panel's columns lie in the near part, already updated.
+// panel per iteration.  An iteration is: block elimination and inverse, plan,
+// gather, near update, finish (main stream), and the far columns' X product and
+// far update (far stream) -- the far update overlaps the next iteration's block
+// elimination and plan: the next panel's columns lie in the near part, already
+// updated.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
