@@ -97,12 +97,10 @@ __device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32
 // k_block_ge: pivot search and fraction-free elimination of the panel part of
 // the top block
 // ---------------------------------------------------------------------------
-// The block elimination runs on NW waves, wave v keeping the NC = 64 / NW
-// panel columns v + NW m.  Its step loop is bound by VALU issue, and every wave
-// repeats the pivot search: with the MFMA method's update beside it, NW = 8
-// (two waves per SIMD, enough to keep a SIMD issuing) is fastest; beside the
-// plain method's VALU-bound update, which shares the SIMDs' issue, NW = 16
-// (measured on the 64-matrix batch: 115.6 against 102.9 ms per batch with 8).
+// The block elimination runs on GE_WAVES waves, two per SIMD: the step loop is
+// bound by VALU issue, every wave repeats the pivot search, and two waves per
+// SIMD are what keeps a SIMD issuing.  Wave v keeps the panel columns v + 8 m.
+constexpr uint32_t GE_WAVES = 8, GE_COLS = BMAX / GE_WAVES;
 
 // One fraction-free step of a block row: x <- pi x - e y where the row takes it
 // (act), x's residue where it does not.  For 2^61 - 1 the choice is folded into
@@ -136,10 +134,10 @@ struct FFStep {
 // pivot column) and, with OWN, on the own column.  Their products are independent
 // and interleave (for 2^61 - 1 written stage by stage across the columns, so the
 // multiply-add chains interleave in the instruction stream too).
-template <int MODE, int NC, int M0, bool OWN>
-__device__ __forceinline__ void ge_columns(const FieldParams &f, const FFStep<MODE> &st, uint64_t (&x)[NC],
-                                           uint64_t &own, uint32_t q) {
-    constexpr int NS = OWN ? NC + 1 : NC;  // slot NC: the own column (y = 0)
+template <int MODE, int M0, bool OWN>
+__device__ __forceinline__ void ge_columns(const FieldParams &f, const FFStep<MODE> &st,
+                                           uint64_t (&x)[GE_COLS], uint64_t &own, uint32_t q) {
+    constexpr int NC = GE_COLS, NS = OWN ? NC + 1 : NC;  // slot NC: the own column (y = 0)
     uint64_t y[NC];
 #pragma unroll
     for (int m = M0; m < NC; m++) y[m] = lane_bcast(x[m], q);
@@ -187,10 +185,9 @@ __device__ __forceinline__ void ge_columns(const FieldParams &f, const FFStep<MO
     }
 }
 
-template <int MODE, uint32_t NW>
-__global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
-                                                      uint32_t defer) {
-    constexpr uint32_t NC = BMAX / NW;
+template <int MODE>
+__global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
+                                                            uint32_t defer) {
     RankMat &S = mats[blockIdx.x];
     // U: one padded row per block row: the block as loaded, then, after the
     // elimination, its final state (panel columns, and the own column COWN).
@@ -200,9 +197,6 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
     __shared__ uint64_t piK[BMAX];
     __shared__ uint32_t rowid[BMAX], Jl[BMAX], Qk[BMAX];
     if (S.done) return;
-    // The panel chain is the critical path while the previous far update runs on
-    // the same CUs: its waves take the SIMDs' issue first.
-    __builtin_amdgcn_s_setprio(3);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
@@ -221,17 +215,17 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
     }
     __syncthreads();
     {
-        // NW lanes per block row, NC panel columns each: all loads issued first.
-        const uint32_t q = tid / NW, c0 = tid % NW;
+        // GE_WAVES lanes per block row, GE_COLS panel columns each: all loads issued first.
+        const uint32_t q = tid / GE_WAVES, c0 = tid % GE_WAVES;
         const size_t src = (size_t)rowid[q < Bt ? q : 0] * ld + col;
-        uint64_t v[NC];
+        uint64_t v[GE_COLS];
 #pragma unroll
-        for (int m = 0; m < (int)NC; m++) {
-            const uint32_t c = c0 + NW * m;
+        for (int m = 0; m < (int)GE_COLS; m++) {
+            const uint32_t c = c0 + GE_WAVES * m;
             v[m] = (q < Bt && c < w) ? m_get(S, src + c) : 0;
         }
 #pragma unroll
-        for (int m = 0; m < (int)NC; m++) U[q][c0 + NW * m] = v[m];
+        for (int m = 0; m < (int)GE_COLS; m++) U[q][c0 + GE_WAVES * m] = v[m];
     }
     __syncthreads();
 
@@ -249,9 +243,9 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
     // read of this one).
     const uint32_t lane = tid & 63, wave = uni32(tid >> 6);
     const bool inblk = lane < Bt;
-    uint64_t x[NC];
+    uint64_t x[GE_COLS];
 #pragma unroll
-    for (int m = 0; m < (int)NC; m++) x[m] = U[lane][wave + NW * m];
+    for (int m = 0; m < (int)GE_COLS; m++) x[m] = U[lane][wave + GE_WAVES * m];
     uint64_t own = inblk ? 1 % f.p : 0;
     if (wave == 0) colb[0][lane] = x[0];
     __syncthreads();  // (also: every read of the loaded block precedes the final state's writes)
@@ -275,7 +269,7 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
             const bool act = free_row && lane != q && e != 0;
             const FFStep<MODE> st(pi, e, act);
             // this wave's columns after j: wave + 8 m for m >= mlo
-            const uint32_t mlo = j < wave ? 0u : (j - wave) / NW + 1;
+            const uint32_t mlo = j < wave ? 0u : (j - wave) / GE_WAVES + 1;
             if (wave == 0) {
                 eta[ETA_E + k * BMAX + lane] = act ? e : 0;  // the step's multipliers
                 if (lane == 0) {
@@ -284,26 +278,26 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
                     Qk[k] = q;
                 }
                 switch (mlo) {
-                    case 0: ge_columns<MODE, NC,0, true>(f, st, x, own, q); break;
-                    case 1: ge_columns<MODE, NC,1, true>(f, st, x, own, q); break;
-                    case 2: ge_columns<MODE, NC,2, true>(f, st, x, own, q); break;
-                    case 3: ge_columns<MODE, NC,3, true>(f, st, x, own, q); break;
-                    case 4: ge_columns<MODE, NC,4, true>(f, st, x, own, q); break;
-                    case 5: ge_columns<MODE, NC,5, true>(f, st, x, own, q); break;
-                    case 6: ge_columns<MODE, NC,6, true>(f, st, x, own, q); break;
-                    case 7: ge_columns<MODE, NC,7, true>(f, st, x, own, q); break;
-                    default: ge_columns<MODE, NC,NC, true>(f, st, x, own, q); break;  // own only
+                    case 0: ge_columns<MODE, 0, true>(f, st, x, own, q); break;
+                    case 1: ge_columns<MODE, 1, true>(f, st, x, own, q); break;
+                    case 2: ge_columns<MODE, 2, true>(f, st, x, own, q); break;
+                    case 3: ge_columns<MODE, 3, true>(f, st, x, own, q); break;
+                    case 4: ge_columns<MODE, 4, true>(f, st, x, own, q); break;
+                    case 5: ge_columns<MODE, 5, true>(f, st, x, own, q); break;
+                    case 6: ge_columns<MODE, 6, true>(f, st, x, own, q); break;
+                    case 7: ge_columns<MODE, 7, true>(f, st, x, own, q); break;
+                    default: ge_columns<MODE, 8, true>(f, st, x, own, q); break;
                 }
             } else {
                 switch (mlo) {
-                    case 0: ge_columns<MODE, NC,0, false>(f, st, x, own, q); break;
-                    case 1: ge_columns<MODE, NC,1, false>(f, st, x, own, q); break;
-                    case 2: ge_columns<MODE, NC,2, false>(f, st, x, own, q); break;
-                    case 3: ge_columns<MODE, NC,3, false>(f, st, x, own, q); break;
-                    case 4: ge_columns<MODE, NC,4, false>(f, st, x, own, q); break;
-                    case 5: ge_columns<MODE, NC,5, false>(f, st, x, own, q); break;
-                    case 6: ge_columns<MODE, NC,6, false>(f, st, x, own, q); break;
-                    case 7: ge_columns<MODE, NC,7, false>(f, st, x, own, q); break;
+                    case 0: ge_columns<MODE, 0, false>(f, st, x, own, q); break;
+                    case 1: ge_columns<MODE, 1, false>(f, st, x, own, q); break;
+                    case 2: ge_columns<MODE, 2, false>(f, st, x, own, q); break;
+                    case 3: ge_columns<MODE, 3, false>(f, st, x, own, q); break;
+                    case 4: ge_columns<MODE, 4, false>(f, st, x, own, q); break;
+                    case 5: ge_columns<MODE, 5, false>(f, st, x, own, q); break;
+                    case 6: ge_columns<MODE, 6, false>(f, st, x, own, q); break;
+                    case 7: ge_columns<MODE, 7, false>(f, st, x, own, q); break;
                     default: break;
                 }
             }
@@ -311,11 +305,11 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
             k++;
         }
         const uint32_t jn = j + 1;
-        if (jn < w && wave == jn % NW) {
+        if (jn < w && wave == jn % GE_WAVES) {
             uint64_t v = x[0];
 #pragma unroll
-            for (int m = 1; m < (int)NC; m++)
-                if (jn / NW == (uint32_t)m) v = x[m];
+            for (int m = 1; m < (int)GE_COLS; m++)
+                if (jn / GE_WAVES == (uint32_t)m) v = x[m];
             colb[jn & 1][lane] = canon_t<MODE>(v);
         }
         __syncthreads();
@@ -323,7 +317,7 @@ __global__ __launch_bounds__(64 * NW) void k_block_ge(RankMat *mats, FieldParams
     const uint32_t pb = k;
     // the final state, for the record
 #pragma unroll
-    for (int m = 0; m < (int)NC; m++) U[lane][wave + NW * m] = x[m];
+    for (int m = 0; m < (int)GE_COLS; m++) U[lane][wave + GE_WAVES * m] = x[m];
     if (wave == 0) U[lane][COWN] = own;
     __syncthreads();
 
@@ -450,7 +444,6 @@ __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, Fi
                                                                uint32_t L) {
     RankMat &S = mats[blockIdx.x / ZG];
     if (S.done) return;
-    __builtin_amdgcn_s_setprio(3);  // (the panel chain: see k_block_ge)
     const uint32_t pb = S.it[par].pb;
     const uint32_t s0 = (blockIdx.x % ZG) * ZSLOTS;
     if (s0 >= pb) {  // uniform over the workgroup: no slots, only zero digit columns
@@ -859,26 +852,18 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-// The block elimination on NW waves: 8 beside the MFMA method's update (L > 0),
-// 16 beside the plain method's.
-template <int MODE>
-static void launch_block_t(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                           uint32_t par, uint32_t L, uint32_t defer) {
-    if (L)
-        hipLaunchKernelGGL((k_block_ge<MODE, 8>), dim3(nmat), dim3(64 * 8), 0, st, dmats, f, b, par, defer);
-    else
-        hipLaunchKernelGGL((k_block_ge<MODE, 16>), dim3(nmat), dim3(64 * 16), 0, st, dmats, f, b, par, defer);
-    hipLaunchKernelGGL(k_block_z<MODE>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
-}
-
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
                   uint32_t L, uint32_t defer) {
-    if (f.mode == MODE_M61)
-        launch_block_t<MODE_M61>(st, dmats, nmat, f, b, par, L, defer);
-    else if (f.mode == MODE_M31)
-        launch_block_t<MODE_M31>(st, dmats, nmat, f, b, par, L, defer);
-    else
-        launch_block_t<MODE_GENERIC>(st, dmats, nmat, f, b, par, L, defer);
+    if (f.mode == MODE_M61) {
+        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
+    } else if (f.mode == MODE_M31) {
+        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
+    } else {
+        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
+        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
+    }
 }
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
