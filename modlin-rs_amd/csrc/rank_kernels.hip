@@ -3,8 +3,8 @@
 //
 // One iteration handles one column panel [col, jend) of every active matrix:
 //
-//   k_block_ge   one workgroup per matrix.  Loads the top block (the next BMAX logical rows) of the panel into LDS
-//                and eliminates it in place: a wave ballot + ffs finds the pivot
+//   k_block_ge   one workgroup per matrix.  Loads the top block (the next BMAX logical rows) of the panel into
+//                registers and eliminates it there: a wave ballot + ffs finds the pivot
 //                row (the first block row not yet a pivot with a nonzero in the
 //                column), and the elimination is fraction-free (pi*x - e*y), so no
 //                modular inverse sits on the per-column critical path.  It records
@@ -76,9 +76,9 @@ __device__ __forceinline__ void finish_iter(RankMat &S, uint32_t *active, uint32
         }
         if (mine != 0xffffffffu) atomicMin(&best, mine);
         __syncthreads();
-        // The far part of this iteration's update still reads perm (its digit rows
-        // follow the current order), so the swap to position rank is only recorded
-        // here; the next k_block_ge reads through it and the next k_plan commits it.
+        // The far part of this iteration's update may still run (the plain method's
+        // reads perm), so the swap to position rank is only recorded here; the next
+        // k_block_ge reads through it and the next k_plan commits it.
         if (tid == 0 && best != 0xffffffffu && best != S.rank) S.pend = best;
     }
     __syncthreads();  // every thread has read the state
