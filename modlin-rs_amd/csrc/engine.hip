@@ -20,7 +20,7 @@ namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer);
+                  uint32_t L, uint32_t defer, const RankLaunch *rlp);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
 void launch_collect(hipStream_t st, RankMat *dmats, uint32_t nmat, const uint32_t *active, uint64_t *out);
@@ -363,9 +363,12 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             const uint32_t par = (uint32_t)(s & 1);
             const RankLaunch &rq = rlp[par];
             hipError_t e;
-            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u);
-            if (!use_mfma && (e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
-            launch_plan(st, dmats, nmat, rq, par);
+            // MFMA method: the plan runs as one more workgroup of k_block_z
+            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u, use_mfma ? &rq : nullptr);
+            if (!use_mfma) {
+                if ((e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
+                launch_plan(st, dmats, nmat, rq, par);
+            }
             if ((e = hipEventRecord(ctx->ev_plan, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(fs, ctx->ev_plan, 0)) != hipSuccess) return e;
             // far X: reads the plan, Z's digits (this parity) and the pivot rows, none

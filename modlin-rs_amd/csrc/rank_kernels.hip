@@ -439,9 +439,20 @@ __device__ __forceinline__ void z_digits(int8_t *Zl, uint32_t L, uint32_t k, uin
     }
 }
 
+__device__ void plan_body(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *near_off, uint32_t *far_off,
+                          uint32_t *xf_off, RankPlan *plan, uint32_t par, uint32_t far_items);
+
+// with_plan: one more workgroup (the last) computes the iteration's plan beside
+// the slot workgroups -- the plan reads only k_block_ge's record and the matrices'
+// state, none of which this kernel writes, and this kernel reads nothing it writes.
 template <int MODE>
 __global__ __launch_bounds__(64 * (ZSLOTS + 1)) void k_block_z(RankMat *mats, FieldParams f, uint32_t par,
-                                                               uint32_t L) {
+                                                               uint32_t L, uint32_t nmat, RankLaunch rl,
+                                                               uint32_t with_plan) {
+    if (with_plan && blockIdx.x == nmat * ZG) {  // uniform
+        plan_body(mats, nmat, rl.gx_off, rl.near_off, rl.far_off, rl.xf_off, rl.plan, par, rl.far_items);
+        return;
+    }
     RankMat &S = mats[blockIdx.x / ZG];
     if (S.done) return;
     const uint32_t pb = S.it[par].pb;
@@ -618,12 +629,12 @@ __device__ __forceinline__ void block_incl_scan4(uint32_t *v, uint32_t (*wsum)[4
 
 constexpr uint32_t PLAN_THREADS = 1024;
 
-__global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off,
-                                                       uint32_t *near_off, uint32_t *far_off, uint32_t *xf_off,
-                                                       RankPlan *plan, uint32_t par, uint32_t far_items) {
+// The plan of one iteration, by one workgroup of any size up to PLAN_THREADS.
+__device__ void plan_body(RankMat *mats, uint32_t nmat, uint32_t *gx_off, uint32_t *near_off, uint32_t *far_off,
+                          uint32_t *xf_off, RankPlan *plan, uint32_t par, uint32_t far_items) {
     __shared__ uint32_t wsum[PLAN_THREADS / 64][4];
     __shared__ uint32_t base[4];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
     // Commit the last iteration's missed-pivot swap, then the top blocks' new row
     // order (written by k_block_ge of this iteration, which saw the swap).  The
     // previous iteration's far update may still run: the plain method's waits
@@ -651,7 +662,7 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t n
     if (tid < 4) base[tid] = 0;
     __syncthreads();
     uint32_t *const offs[4] = {gx_off, near_off, far_off, xf_off};
-    for (uint32_t i0 = 0; i0 < nmat; i0 += PLAN_THREADS) {
+    for (uint32_t i0 = 0; i0 < nmat; i0 += nt) {
         const uint32_t i = i0 + tid;
         uint32_t c[4] = {0, 0, 0, 0};
         if (i < nmat) plan_counts(mats[i], par, far_items, c[0], c[1], c[2], c[3]);
@@ -661,7 +672,7 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t n
 #pragma unroll
             for (int k = 0; k < 4; k++) offs[k][i] = base[k] + v[k] - c[k];
         __syncthreads();
-        if (tid == PLAN_THREADS - 1)
+        if (tid == nt - 1)
 #pragma unroll
             for (int k = 0; k < 4; k++) base[k] += v[k];
         __syncthreads();
@@ -674,6 +685,12 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t n
         plan->far_total = base[2];
         plan->xf_total = base[3];
     }
+}
+
+__global__ __launch_bounds__(PLAN_THREADS) void k_plan(RankMat *mats, uint32_t nmat, uint32_t *gx_off,
+                                                       uint32_t *near_off, uint32_t *far_off, uint32_t *xf_off,
+                                                       RankPlan *plan, uint32_t par, uint32_t far_items) {
+    plan_body(mats, nmat, gx_off, near_off, far_off, xf_off, plan, par, far_items);
 }
 
 // ---------------------------------------------------------------------------
@@ -852,18 +869,25 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// The block elimination and Z; with rlp, the iteration's plan too (one more
+// workgroup of k_block_z).
+template <int MODE>
+static void launch_block_t(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
+                           uint32_t par, uint32_t L, uint32_t defer, const RankLaunch *rlp) {
+    hipLaunchKernelGGL(k_block_ge<MODE>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
+    const RankLaunch rl = rlp ? *rlp : RankLaunch{};
+    hipLaunchKernelGGL(k_block_z<MODE>, dim3(nmat * ZG + (rlp ? 1 : 0)), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f,
+                       par, L, nmat, rl, rlp ? 1u : 0u);
+}
+
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer) {
-    if (f.mode == MODE_M61) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M61>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
-        hipLaunchKernelGGL(k_block_z<MODE_M61>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
-    } else if (f.mode == MODE_M31) {
-        hipLaunchKernelGGL(k_block_ge<MODE_M31>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
-        hipLaunchKernelGGL(k_block_z<MODE_M31>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
-    } else {
-        hipLaunchKernelGGL(k_block_ge<MODE_GENERIC>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
-        hipLaunchKernelGGL(k_block_z<MODE_GENERIC>, dim3(nmat * ZG), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f, par, L);
-    }
+                  uint32_t L, uint32_t defer, const RankLaunch *rlp) {
+    if (f.mode == MODE_M61)
+        launch_block_t<MODE_M61>(st, dmats, nmat, f, b, par, L, defer, rlp);
+    else if (f.mode == MODE_M31)
+        launch_block_t<MODE_M31>(st, dmats, nmat, f, b, par, L, defer, rlp);
+    else
+        launch_block_t<MODE_GENERIC>(st, dmats, nmat, f, b, par, L, defer, rlp);
 }
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {
