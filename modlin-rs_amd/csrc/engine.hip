@@ -20,7 +20,7 @@ namespace mdl {
 
 // rank_kernels.hip, mfma_update.hip
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer, const RankLaunch *rlp);
+                  uint32_t L, uint32_t defer, const RankLaunch *rlp, bool lds);
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par);
 void launch_finish(hipStream_t st, RankMat *dmats, uint32_t nmat, uint32_t *active, uint32_t par);
 void launch_collect(hipStream_t st, RankMat *dmats, uint32_t nmat, const uint32_t *active, uint64_t *out);
@@ -364,7 +364,8 @@ int rank_batch(modlin_ctx *ctx, uint64_t p, const modlin_mat_desc_t *mats, uint6
             const RankLaunch &rq = rlp[par];
             hipError_t e;
             // MFMA method: the plan runs as one more workgroup of k_block_z
-            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u, use_mfma ? &rq : nullptr);
+            launch_block(st, dmats, nmat, f, b, par, use_mfma ? lp.L : 0u, defer ? 1u : 0u, use_mfma ? &rq : nullptr,
+                         maxn >= GE_LDS_MIN_SIDE);
             if (!use_mfma) {
                 if ((e = hipStreamWaitEvent(st, ctx->ev_far, 0)) != hipSuccess) return e;
                 launch_plan(st, dmats, nmat, rq, par);

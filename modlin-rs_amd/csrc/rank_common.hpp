@@ -107,6 +107,7 @@ constexpr uint32_t ETA_OWNK = ETA_PI + BMAX;           // [step]
 constexpr uint32_t ETA_Q = ETA_OWNK + BMAX;            // [step]
 constexpr uint32_t ETA_WORDS = ETA_Q + BMAX;
 constexpr uint32_t ZG = 16;                            // workgroups per matrix in k_block_z
+constexpr uint64_t GE_LDS_MIN_SIDE = 24576;            // batches with a side this large: k_block_ge_lds
 constexpr uint32_t ZSLOTS = BMAX / ZG;                 // slot waves per workgroup (plus one inverse wave)
 
 

@@ -185,6 +185,10 @@ __device__ __forceinline__ void ge_columns(const FieldParams &f, const FFStep<MO
     }
 }
 
+__device__ void ge_commit(RankMat &S, uint32_t par, uint32_t defer, uint32_t pb, uint64_t piv, const uint32_t *rowid,
+                          const uint32_t *Jl, const uint32_t *Qk, uint32_t rank0, uint32_t col, uint32_t jend,
+                          uint32_t w, uint32_t Bt, uint32_t dpend);
+
 template <int MODE>
 __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
                                                             uint32_t defer) {
@@ -333,6 +337,137 @@ __global__ __launch_bounds__(64 * GE_WAVES) void k_block_ge(RankMat *mats, Field
         eta[ETA_OWNK + tid] = canon_t<MODE>(U[Qk[tid]][COWN]);
         eta[ETA_Q + tid] = Qk[tid];
     }
+    ge_commit(S, par, defer, pb, piv, rowid, Jl, Qk, rank0, col, jend, w, Bt, dpend);
+}
+
+// The block elimination in its LDS form, used for the largest matrices (sides of
+// GE_LDS_MIN_SIDE and more): 16 waves, the block in LDS, each step's columns read
+// from and written back to it, a barrier per step.  It is slower on its own than
+// k_block_ge (about 90 against 80 us a panel on a 65536^2 matrix), but with it
+// the far update that runs beside the panel chain takes 4.68 instead of 5.01 ms a
+// launch there (the 65536^2 planted-corank rank: 4.95 against 5.38 s; measured,
+// the cause not found), while on the 64-matrix batch and 8192^2 k_block_ge is
+// faster overall (17.2 against 17.5 ms, 22.7 against 23.7 ms).  The record and
+// the iteration state are the same.
+template <int MODE>
+__device__ __forceinline__ bool nz_t(uint64_t v) {
+    return MODE == MODE_M61 ? (v != 0 && v != P61) : v != 0;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_block_ge_lds(RankMat *mats, FieldParams f, uint32_t b, uint32_t par,
+                                                       uint32_t defer) {
+    RankMat &S = mats[blockIdx.x];
+    // One LDS row per block row: [0, BMAX) panel entries, then own (the
+    // coefficient on the row's own original row).
+    constexpr uint32_t COWN = BMAX, UW = BMAX + 1;
+    __shared__ uint64_t U[BMAX][UW];
+    __shared__ uint64_t Et[BMAX][BMAX];  // multiplier e of each block row at each step
+    __shared__ uint64_t piK[BMAX], ownK[BMAX];
+    __shared__ uint32_t rowid[BMAX], Jl[BMAX], Qk[BMAX];
+    if (S.done) return;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = S.n, ld = S.ld, rank0 = S.rank, col = S.col;
+    const uint32_t dpend = S.dpend;  // read before the barriers below; thread 0 clears it at the end
+    const uint32_t jend = min(col + b, n);
+    const uint32_t w = jend - col;
+    const uint32_t Bt = min((uint32_t)BMAX, n - rank0);
+    const uint32_t pend = S.pend;  // (as in k_block_ge)
+    if (tid < Bt) {
+        const uint32_t pos = rank0 + tid;
+        const uint32_t src = (pend == 0xffffffffu) ? pos : (pos == rank0 ? pend : (pos == pend ? rank0 : pos));
+        rowid[tid] = S.perm[src];
+    }
+    for (uint32_t idx = tid; idx < BMAX * BMAX; idx += blockDim.x) Et[idx / BMAX][idx % BMAX] = 0;
+    __syncthreads();
+    {
+        // 16 lanes per block row, 4 panel columns each: all loads issued first.
+        const uint32_t q = tid >> 4, c0 = tid & 15;
+        const size_t src = (size_t)rowid[q < Bt ? q : 0] * ld + col;
+        uint64_t v[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = c0 + 16 * m;
+            v[m] = (q < Bt && c < w) ? m_get(S, src + c) : 0;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) U[q][c0 + 16 * m] = v[m];
+        if (c0 == 0) U[q][COWN] = q < Bt ? 1 % f.p : 0;
+    }
+    __syncthreads();
+
+    // Wave v owns the panel columns v, v + 16, v + 32, v + 48 (and, at step j, the
+    // own column if v = j mod 16), lane = block row; the pivot search and the
+    // fraction-free step are those of k_block_ge.  A step reads column j and the
+    // lane's own columns before the ballot; row q and column j are only read, so
+    // no barrier separates the reads from the writes.
+    const uint32_t lane = tid & 63, wave = uni32(tid >> 6);
+    const bool inblk = lane < Bt;
+    uint64_t piv = 0;
+    uint32_t k = 0;
+    for (uint32_t j = 0; j < w && k < Bt; j++) {
+        const uint64_t colv = U[lane][j];
+        const bool own = wave == (j & 15);
+        uint64_t xv[5];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = wave + 16 * m;
+            if (c > j && c < w) xv[m] = U[lane][c];  // uniform over the wave
+        }
+        if (own) xv[4] = U[lane][COWN];
+        const bool free_row = inblk && !((piv >> lane) & 1);
+        const uint64_t mask = __ballot(free_row && nz_t<MODE>(colv));
+        if (mask == 0) continue;  // uniform: no pivot in this column
+        const uint32_t q = __ffsll((unsigned long long)mask) - 1;
+        const uint64_t pi = canon_t<MODE>(lane_bcast(colv, q));
+        const uint64_t e = canon_t<MODE>(colv);
+        const bool act = free_row && lane != q && e != 0;
+        if (wave == 0 && act) Et[k][lane] = e;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t c = wave + 16 * m;
+            if (c > j && c < w) {
+                const uint64_t y = lane_bcast(xv[m], q);
+                if (act) U[lane][c] = ff_lazy_t<MODE>(f, pi, xv[m], e, y);
+            }
+        }
+        if (own && act) U[lane][COWN] = ff_lazy_t<MODE>(f, pi, xv[4], e, 0);
+        if (tid == 0) {
+            Jl[k] = j;
+            piK[k] = pi;
+            ownK[k] = canon_t<MODE>(U[q][COWN]);  // row q takes no step now
+            Qk[k] = q;
+        }
+        __syncthreads();
+        piv |= 1ull << q;
+        k++;
+    }
+    const uint32_t pb = k;
+
+    // The record (the layout of k_block_ge's): a pivot row is final once it is the
+    // pivot, so U[r][s] = U[Q[r]][J_s] for s > r.
+    uint64_t *eta = S.Eta;
+    for (uint32_t idx = tid; idx < pb * BMAX; idx += blockDim.x) {
+        const uint32_t r = idx / BMAX, s = idx % BMAX;
+        eta[ETA_E + idx] = Et[r][s];
+        eta[ETA_U + idx] = (s > r && s < pb) ? canon_t<MODE>(U[Qk[r]][Jl[s]]) : 0;
+    }
+    if (tid < pb) {
+        eta[ETA_PI + tid] = piK[tid];
+        eta[ETA_OWNK + tid] = ownK[tid];
+        eta[ETA_Q + tid] = Qk[tid];
+    }
+    ge_commit(S, par, defer, pb, piv, rowid, Jl, Qk, rank0, col, jend, w, Bt, dpend);
+}
+
+// The end of a block elimination (either form), every thread of the workgroup:
+// the pivots and the block's new row order into the iteration's state, and the
+// iteration kind with its column ranges.
+__device__ void ge_commit(RankMat &S, uint32_t par, uint32_t defer, uint32_t pb, uint64_t piv, const uint32_t *rowid,
+                          const uint32_t *Jl, const uint32_t *Qk, uint32_t rank0, uint32_t col, uint32_t jend,
+                          uint32_t w, uint32_t Bt, uint32_t dpend) {
+    const uint32_t tid = threadIdx.x, n = S.n;
     // The new order of the block -- the pivot rows in pivot order, then the other
     // rows in block order -- is applied to perm by k_plan.
     IterState &I = S.it[par];
@@ -869,25 +1004,28 @@ __global__ __launch_bounds__(256) void k_collect(const RankMat *mats, uint32_t n
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-// The block elimination and Z; with rlp, the iteration's plan too (one more
-// workgroup of k_block_z).
+// The block elimination (the LDS form with lds) and Z; with rlp, the iteration's
+// plan too (one more workgroup of k_block_z).
 template <int MODE>
 static void launch_block_t(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b,
-                           uint32_t par, uint32_t L, uint32_t defer, const RankLaunch *rlp) {
-    hipLaunchKernelGGL(k_block_ge<MODE>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
+                           uint32_t par, uint32_t L, uint32_t defer, const RankLaunch *rlp, bool lds) {
+    if (lds)
+        hipLaunchKernelGGL(k_block_ge_lds<MODE>, dim3(nmat), dim3(1024), 0, st, dmats, f, b, par, defer);
+    else
+        hipLaunchKernelGGL(k_block_ge<MODE>, dim3(nmat), dim3(64 * GE_WAVES), 0, st, dmats, f, b, par, defer);
     const RankLaunch rl = rlp ? *rlp : RankLaunch{};
     hipLaunchKernelGGL(k_block_z<MODE>, dim3(nmat * ZG + (rlp ? 1 : 0)), dim3(64 * (ZSLOTS + 1)), 0, st, dmats, f,
                        par, L, nmat, rl, rlp ? 1u : 0u);
 }
 
 void launch_block(hipStream_t st, RankMat *dmats, uint32_t nmat, const FieldParams &f, uint32_t b, uint32_t par,
-                  uint32_t L, uint32_t defer, const RankLaunch *rlp) {
+                  uint32_t L, uint32_t defer, const RankLaunch *rlp, bool lds) {
     if (f.mode == MODE_M61)
-        launch_block_t<MODE_M61>(st, dmats, nmat, f, b, par, L, defer, rlp);
+        launch_block_t<MODE_M61>(st, dmats, nmat, f, b, par, L, defer, rlp, lds);
     else if (f.mode == MODE_M31)
-        launch_block_t<MODE_M31>(st, dmats, nmat, f, b, par, L, defer, rlp);
+        launch_block_t<MODE_M31>(st, dmats, nmat, f, b, par, L, defer, rlp, lds);
     else
-        launch_block_t<MODE_GENERIC>(st, dmats, nmat, f, b, par, L, defer, rlp);
+        launch_block_t<MODE_GENERIC>(st, dmats, nmat, f, b, par, L, defer, rlp, lds);
 }
 
 void launch_plan(hipStream_t st, RankMat *dmats, uint32_t nmat, const RankLaunch &rl, uint32_t par) {

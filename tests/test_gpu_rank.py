@@ -152,6 +152,20 @@ def test_planted_corank_large(ctx, p, method):
     assert _rank_all(ctx, p, [a], method) == [n - 64]
 
 
+@pytest.mark.parametrize("method", METHODS)
+def test_planted_corank_block_elimination_lds_form(ctx, method):
+    """Batches with a side of 24576 or more take the LDS form of the block
+    elimination: the planted-corank matrix at that side (built on the device,
+    4.5 GiB) has corank exactly 64 (size-independent property)."""
+    n = 24576
+    buf = workloads.c3_device(ctx, n, M61)
+    try:
+        print(f"planted lds form n={n} method={method}", flush=True)
+        assert ctx.rank_batch(modlin.Field(M61), [buf], method=method) == [n - 64]
+    finally:
+        buf.free()
+
+
 def test_readme_c1_both_methods(ctx):
     mm, ctl = workloads.c1()
     for method in METHODS:
@@ -341,3 +355,28 @@ def test_far_update_forms(ctx, p, tuning):
         assert _rank_all(ctx, p, mats, modlin.METHOD_MFMA) == want
     finally:
         ctx.set_tuning()
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("p", [M61, (1 << 31) - 1])
+def test_missed_pivots_in_large_matrices(ctx, p, method):
+    """Missed pivots while long updates run: the top 200 rows of 2048^2 and
+    3072^2 matrices are zero in their first 100 columns, so the first panels'
+    top blocks find no pivot there and every such column is a missed-pivot
+    restart (a row swap committed while the previous panel's far update may
+    still run).  Ranks by construction (size-independent): full rank for the
+    random matrices, n - 64 when the last 64 rows are planted combinations of
+    the first 128 rows of the modified matrix."""
+    rng = np.random.default_rng(p % 4099)
+    mats, want = [], []
+    for n in (2048, 3072):
+        a = rng.integers(0, p, size=(n, n), dtype=np.uint64)
+        a[:200, :100] = 0
+        mats.append(a)
+        want.append(n)
+        b = a.copy()
+        b[n - 64:] = modlin.combine_rows(p, b[:128], rng.integers(0, p, size=(64, 128), dtype=np.uint64))
+        mats.append(b)
+        want.append(n - 64)
+    print(f"missed pivots p={p} method={method}", flush=True)
+    assert _rank_all(ctx, p, mats, method) == want
